@@ -1,0 +1,123 @@
+"""Tile-shape autotuner for the collapsed-cone gather.
+
+A wavefront owns TX x TY x TZ = 64 output voxels.  Long z-runs give wide
+coalesced T loads; compact tiles shrink the per-lane spread of the clipped
+tap ranges.  The best shape depends on the grid, the cone set and the TERMA
+support, so it is measured: every candidate is timed with HIP events on the
+real plan and a synthetic TERMA of the given support, and the winner is
+cached per plan key in memory and in a JSON file.
+"""
+import json
+import os
+import threading
+
+import numpy as np
+
+from . import _lib
+from .device import DeviceArray, Event, Stream
+
+CANDIDATES = [(1, 1, 64), (1, 2, 32), (1, 4, 16), (2, 2, 16), (1, 8, 8),
+              (2, 4, 8), (4, 4, 4), (2, 8, 4), (4, 8, 2), (8, 8, 1)]
+
+CACHE_PATH = os.environ.get(
+    "CONEHEAD_AMD_TUNE_CACHE",
+    os.path.join(os.path.expanduser("~"), ".cache", "conehead_amd",
+                 "tiles.json"))
+
+_lock = threading.Lock()
+_memo = {}
+
+
+def default_tile(plan):
+    """Heuristic tile before (or without) tuning: a full z-run per wave when
+    the grid is at least 64 deep, otherwise fold the rest into y and x."""
+    key = _shape_key(plan)
+    if key in _memo:
+        return _memo[key]
+    Z = plan.shape[2]
+    if Z >= 64:
+        return (1, 1, 64)
+    if Z >= 32:
+        return (1, 2, 32)
+    if Z >= 16:
+        return (1, 4, 16)
+    return (2, 4, 8)
+
+
+def _shape_key(plan):
+    return "%dx%dx%d/%d" % (plan.shape + (plan.n_cones,))
+
+
+def _load_cache():
+    try:
+        with open(CACHE_PATH) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _save_cache(d):
+    try:
+        os.makedirs(os.path.dirname(CACHE_PATH), exist_ok=True)
+        tmp = CACHE_PATH + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(d, f, indent=1)
+        os.replace(tmp, CACHE_PATH)
+    except OSError:
+        pass
+
+
+def tune(plan, box=None, candidates=CANDIDATES, reps=3, stream=None,
+         use_cache=True):
+    """Time every candidate tile on `plan` and keep the fastest.
+
+    box: (lo0, lo1, lo2, hi0, hi1, hi2) support of the TERMA to tune for
+    (default: the whole grid).  Returns (best_tile, {tile: ms})."""
+    key = _shape_key(plan)
+    if box is not None:
+        key += "/" + ",".join(str(int(b)) for b in box)
+    if use_cache:
+        with _lock:
+            cached = _load_cache().get(key)
+        if cached:
+            tile = tuple(cached["tile"])
+            plan.set_tile(*tile)
+            _memo[_shape_key(plan)] = tile
+            return tile, {tuple(map(int, k.split("x"))): v
+                          for k, v in cached["ms"].items()}
+    shape = plan.shape
+    rng = np.random.default_rng(0)
+    t = np.zeros(shape, np.float32)
+    b = box or (0, 0, 0, shape[0] - 1, shape[1] - 1, shape[2] - 1)
+    t[b[0]:b[3] + 1, b[1]:b[4] + 1, b[2]:b[5] + 1] = rng.random(
+        (b[3] - b[0] + 1, b[4] - b[1] + 1, b[5] - b[2] + 1),
+        dtype=np.float32)
+    s = stream or Stream()
+    d_t = DeviceArray.from_host(t)
+    d_d = DeviceArray(shape, np.float32)
+    d_box = DeviceArray.from_host(np.asarray(b, np.int32))
+    from .engine import convolve
+    e0, e1 = Event(), Event()
+    times = {}
+    for tile in candidates:
+        if tile[0] > shape[0] * 2 and tile[1] > 1:
+            continue
+        plan.set_tile(*tile)
+        convolve(plan, d_t, d_d, d_box, stream=s)  # warm-up
+        e0.record(s)
+        for _ in range(reps):
+            convolve(plan, d_t, d_d, d_box, stream=s)
+        e1.record(s)
+        e1.sync()
+        times[tile] = e0.elapsed_ms(e1) / reps
+    best = min(times, key=times.get)
+    plan.set_tile(*best)
+    _memo[_shape_key(plan)] = best
+    if use_cache:
+        with _lock:
+            d = _load_cache()
+            d[key] = {"tile": list(best),
+                      "ms": {"x".join(map(str, k)): v
+                             for k, v in times.items()}}
+            _save_cache(d)
+    return best, times

@@ -1,0 +1,107 @@
+"""Conehead.calculate on the GPU engine (drop-in for conehead/conehead.py:520).
+
+The reference's flow -- density interpolation, hit test, effective depth,
+off-axis distance, fluence, softening and horn, TERMA, polyenergetic kernel,
+dose (conehead/conehead.py:522-773) -- runs as one fused TERMA launch and
+one collapsed-cone gather launch with every intermediate kept in HBM.  The
+dose is returned and stored in self.dose_grid.dose (the reference plots it
+and discards it).
+
+Settings keys are the reference's (example.py:27-58).  Optional keys:
+  thetas, phis, kernel -- the cone set handed to the convolution (default:
+      12 azimuths and all 48 polar keys of PolyenergeticKernel, the cone set
+      of the reference's convolve_c call, conehead/conehead.py:780-794);
+  semantics -- "convolve_c" (default) or "cuda_dose" (y-polar cones and the
+      index arithmetic of conehead/conehead.py:385-488; `kernel` is then a
+      [n_phi, >=5996] cumulative table and `phis` its angles);
+  density_xy_swap -- True (default) reproduces the reference's density
+      resampling, whose meshgrid(indexing='xy') evaluation at the grid nodes
+      swaps the x and y axes (conehead/conehead.py:562-567); False uses the
+      densities as given.
+"""
+import numpy as np
+
+from . import engine as _engine
+from .cones import ConeSet, default_thetas, kernel_phis
+from .device import DeviceArray, Stream
+from .dosegrid import DoseGrid
+from .kernel2 import PolyenergeticKernel
+
+
+def dose_grid_densities(phantom, swap_xy=True):
+    """Densities on the dose grid (the phantom's own grid).
+
+    The reference interpolates the phantom linearly at the nodes
+    linspace(origin, origin + n*s, n) of the same axes it interpolates on,
+    so every value is a node value; its meshgrid uses indexing='xy', which
+    lays the result out as [y][x][z]."""
+    d = np.asarray(phantom.densities, dtype=np.float32)
+    if not swap_xy:
+        return np.ascontiguousarray(d)
+    if d.shape[0] != d.shape[1]:
+        raise ValueError(
+            "the reference's xy-swapped density resampling is only defined "
+            "for phantoms with size[0] == size[1]; pass "
+            "settings['density_xy_swap'] = False")
+    return np.ascontiguousarray(d.transpose(1, 0, 2))
+
+
+class Conehead:
+
+    def __init__(self, engine=None):
+        self._engine = engine
+        self.dose_grid = None
+
+    @property
+    def engine(self):
+        if self._engine is None:
+            from .convolve_c import default_engine
+            self._engine = default_engine()
+        return self._engine
+
+    def calculate(self, source, block, phantom, settings):
+        """TERMA and dose of one beam; returns the FP32 dose grid."""
+        eng = self.engine
+        self.dose_grid = DoseGrid(phantom.size, phantom.origin,
+                                  phantom.spacing)
+        shape = tuple(int(s) for s in phantom.size)
+        origin = np.asarray(phantom.origin, dtype=np.float64)
+        spacing = np.asarray(phantom.spacing, dtype=np.float64)
+        dens = dose_grid_densities(phantom,
+                                   settings.get("density_xy_swap", True))
+        if dens.shape != shape:
+            raise ValueError("densities %s do not match size %s" %
+                             (dens.shape, shape))
+
+        semantics = settings.get("semantics", "convolve_c")
+        kernel = settings.get("kernel")
+        if semantics == "convolve_c":
+            kernel = kernel if kernel is not None else PolyenergeticKernel()
+            phis = settings.get("phis")
+            phis = kernel_phis(kernel) if phis is None else phis
+        else:
+            if kernel is None or settings.get("phis") is None:
+                raise ValueError("cuda_dose semantics needs settings['kernel']"
+                                 " (cumulative table) and settings['phis']")
+            phis = settings["phis"]
+        thetas = settings.get("thetas")
+        thetas = default_thetas(12) if thetas is None else thetas
+        cones = ConeSet(thetas, phis, kernel, semantics=semantics)
+
+        s = Stream()
+        d_rho = DeviceArray.from_host(dens, stream=s)
+        d_terma = DeviceArray(shape, np.float32)
+        d_box = DeviceArray((6,), np.int32)
+        d_raster = eng.raster(block, s)
+        beam = _engine.beam_struct(source, settings, d_raster, d_rho,
+                                   d_terma, d_box)
+        _engine.terma([beam], shape, origin, spacing, stream=s)
+        plan = eng.plan(shape, spacing, cones)
+        d_dose = DeviceArray(shape, np.float32)
+        _engine.convolve(plan, d_terma, d_dose, d_box, stream=s)
+        dose = d_dose.download(stream=s)
+        self.dose_grid_terma = d_terma.download(stream=s)
+        self.dose_grid_densities = dens
+        self.dose_grid.dose = dose
+        s.sync()
+        return dose

@@ -1,0 +1,92 @@
+// ch_internal.h -- shared declarations of libconehead_amd (gfx950 only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "../../include/conehead_amd.h"
+
+namespace ch {
+
+// Thread-local error message behind ch_last_error().
+void set_error(const char* fmt, ...);
+ch_status hip_fail(hipError_t e, const char* what, const char* file, int line);
+int32_t num_cus();
+
+#define CH_HIP(call)                                                        \
+    do {                                                                    \
+        hipError_t ch_e_ = (call);                                          \
+        if (ch_e_ != hipSuccess)                                            \
+            return ::ch::hip_fail(ch_e_, #call, __FILE__, __LINE__);        \
+    } while (0)
+
+#define CH_REQUIRE(cond, ...)                                               \
+    do {                                                                    \
+        if (!(cond)) {                                                      \
+            ::ch::set_error(__VA_ARGS__);                                   \
+            return CH_ERR_ARG;                                              \
+        }                                                                   \
+    } while (0)
+
+constexpr double kBig = 1000000000.0;  // dda_3d_c.pyx:52 big_number
+constexpr int32_t kMaxIndex = 5995;    // convolve_c.pyx:144
+constexpr int32_t kRowLenMin = kMaxIndex + 1;
+// Zero taps appended to every cone's tap list (>= the gather's unroll).
+constexpr int32_t kTapPad = 8;
+
+// Per-cone metadata of a plan (device and host copies).
+//  taps [tap_begin, tap_begin + n_taps): source byte offset and increment.
+//  ftab[f_begin[a] + q], q in [0, f_len[a]): first tap index m whose offset
+//  has |o_{m,a}| >= q; the last entry (q = f_len[a]-1) equals n_taps.
+struct ConeMeta {
+    int32_t tap_begin;
+    int32_t n_taps;
+    int32_t f_begin[3];
+    int32_t f_len[3];
+    int32_t step[3];
+    int32_t pad;
+};
+
+// Flat job descriptor of the gather kernels.
+struct ConvJob {
+    const float* terma;
+    void* dose;                // float* or double*
+    const int32_t* box;        // nullable
+    const int32_t* lin;        // per tap source byte offset 4 * (o . strides)
+    const float* k;            // per tap FP32 increment
+    const int32_t* ftab;
+    const ConeMeta* meta;
+    int32_t X, Y, Z;
+    int32_t n_cones;
+    int32_t accumulate;
+    int32_t ntx, nty, ntz;     // tiles per axis
+    int32_t tile_begin;        // first global tile of this job
+    int32_t pad;
+};
+
+void record_kernel_ms(float ms);
+bool timing_enabled();
+
+}  // namespace ch
+
+struct ch_plan {
+    int32_t shape[3];
+    double vsize[3];
+    int32_t n_cones;
+    int32_t semantics;
+    int32_t tap_stride;
+    int32_t f_stride;
+    int64_t n_taps_total;
+    int32_t max_taps;
+    int32_t tile[3];
+    int32_t accum_mode;
+    int32_t* d_lin;
+    float* d_k;
+    double* d_k64;
+    int32_t* d_off;
+    int32_t* d_ftab;
+    ch::ConeMeta* d_meta;
+    std::vector<ch::ConeMeta> h_meta;
+};

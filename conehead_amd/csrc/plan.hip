@@ -1,0 +1,301 @@
+// plan.hip -- per-cone tap tables of the collapsed-cone gather, built on the
+// device in FP64.
+//
+// convolve_c (conehead/convolve_c.pyx:88-192) runs, for every source voxel s
+// and cone c, the DDA from the centre of s and adds T(s) * k_m to the m-th
+// visited voxel.  The DDA offsets o_m = v_m - s, the t-values and hence the
+// kernel indices and increments depend only on (direction, voxel size), and
+// the path is monotone per axis, so voxel s + o_m is visited exactly when it
+// lies in the grid.  The convolution is therefore the translation-invariant
+// stencil dose(v) = sum_c sum_m k_{c,m} T(v - o_{c,m}); this file builds the
+// (o_{c,m}, k_{c,m}) lists once per (grid, voxel size, cone set, kernel).
+#include <cstring>
+
+#include "dda_core.h"
+
+namespace ch {
+
+// One lane per cone: walk the DDA from the origin of an unbounded grid while
+// every offset component could still land inside the grid, stop at the first
+// skipped kernel index (t >= 60.46 cm), record taps and first-step tables.
+__global__ __launch_bounds__(64) void build_taps_kernel(
+    int32_t n_cones, const double* __restrict__ dirs,
+    const int32_t* __restrict__ rows, const double* __restrict__ cum,
+    int32_t row_len, int32_t X, int32_t Y, int32_t Z, double v0, double v1,
+    double v2, int32_t semantics, int32_t tap_stride, int32_t f_stride,
+    int32_t* __restrict__ lin, float* __restrict__ k32,
+    double* __restrict__ k64, int32_t* __restrict__ off,
+    int32_t* __restrict__ ftab, ConeMeta* __restrict__ meta)
+{
+    const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n_cones)
+        return;
+    Dda d;
+    d.init(dirs[3 * c + 0], dirs[3 * c + 1], dirs[3 * c + 2], v0, v1, v2);
+    const double* row = cum + (int64_t)rows[c] * row_len;
+    const int32_t tb = c * tap_stride;
+    const int32_t fb0 = c * f_stride;
+    const int32_t fb1 = fb0 + X + 2;
+    const int32_t fb2 = fb1 + Y + 2;
+    int32_t c0 = 0, c1 = 0, c2 = 0;  // current offset
+    int32_t n0 = 0, n1 = 0, n2 = 0;  // steps taken per axis (= |offset|)
+    ftab[fb0] = 0;
+    ftab[fb1] = 0;
+    ftab[fb2] = 0;
+    int32_t n = 0, prev = -1;
+    const int32_t sy = Z, sx = Y * Z;
+    while (n0 < X && n1 < Y && n2 < Z && n < tap_stride - kTapPad) {
+        const int a = d.axis();
+        double t = a == 0 ? d.t0 : (a == 1 ? d.t1 : d.t2);
+        bool clamped = false;
+        const int32_t idx = kernel_index(t, semantics, &clamped);
+        if (idx < 0)
+            break;
+        const double k = prev < 0 ? row[idx] : row[idx] - row[prev];
+        lin[tb + n] = 4 * (c0 * sx + c1 * sy + c2);  // byte offset
+        k64[tb + n] = k;
+        k32[tb + n] = (float)k;
+        off[3 * (tb + n) + 0] = c0;
+        off[3 * (tb + n) + 1] = c1;
+        off[3 * (tb + n) + 2] = c2;
+        prev = idx;
+        ++n;
+        if (clamped)
+            break;  // later taps would add cum[5995] - cum[5995] = 0
+        d.advance(a, c0, c1, c2);
+        // The next tap (index n) is the first one with this step count.
+        if (a == 0) ftab[fb0 + (++n0)] = n;
+        else if (a == 1) ftab[fb1 + (++n1)] = n;
+        else ftab[fb2 + (++n2)] = n;
+    }
+    // Zero padding lets the gather read a whole unrolled chunk past the end.
+    for (int32_t q = n; q < n + kTapPad; ++q) {
+        lin[tb + q] = 0;
+        k64[tb + q] = 0.0;
+        k32[tb + q] = 0.0f;
+        off[3 * (tb + q) + 0] = 0;
+        off[3 * (tb + q) + 1] = 0;
+        off[3 * (tb + q) + 2] = 0;
+    }
+    ftab[fb0 + n0 + 1] = n;
+    ftab[fb1 + n1 + 1] = n;
+    ftab[fb2 + n2 + 1] = n;
+    ConeMeta m;
+    m.tap_begin = tb;
+    m.n_taps = n;
+    m.f_begin[0] = fb0;
+    m.f_begin[1] = fb1;
+    m.f_begin[2] = fb2;
+    m.f_len[0] = n0 + 2;
+    m.f_len[1] = n1 + 2;
+    m.f_len[2] = n2 + 2;
+    m.step[0] = d.s0;
+    m.step[1] = d.s1;
+    m.step[2] = d.s2;
+    m.pad = 0;
+    meta[c] = m;
+}
+
+static void plan_free(ch_plan* p)
+{
+    if (!p)
+        return;
+    hipFree(p->d_lin);
+    hipFree(p->d_k);
+    hipFree(p->d_k64);
+    hipFree(p->d_off);
+    hipFree(p->d_ftab);
+    hipFree(p->d_meta);
+    delete p;
+}
+
+}  // namespace ch
+
+using namespace ch;
+
+extern "C" {
+
+ch_status ch_plan_create(const int32_t h_shape[3], const double h_vsize[3],
+                         int32_t n_cones, const double* h_dirs,
+                         const int32_t* h_cone_row, int32_t n_rows,
+                         int32_t row_len, const double* h_cum,
+                         int32_t semantics, ch_plan** plan_out)
+{
+    CH_REQUIRE(plan_out && h_shape && h_vsize, "ch_plan_create: null pointer");
+    *plan_out = nullptr;
+    CH_REQUIRE(h_shape[0] > 0 && h_shape[1] > 0 && h_shape[2] > 0,
+               "ch_plan_create: empty grid %d x %d x %d", h_shape[0],
+               h_shape[1], h_shape[2]);
+    CH_REQUIRE((int64_t)h_shape[0] * h_shape[1] * h_shape[2] < (1LL << 29),
+               "ch_plan_create: grid too large for 32-bit byte offsets");
+    CH_REQUIRE(n_cones > 0 && h_dirs && h_cone_row,
+               "ch_plan_create: no cones");
+    CH_REQUIRE(n_rows > 0 && h_cum, "ch_plan_create: no kernel rows");
+    CH_REQUIRE(row_len >= kRowLenMin,
+               "ch_plan_create: kernel rows have %d values, need >= %d",
+               row_len, kRowLenMin);
+    CH_REQUIRE(semantics == CH_SEM_CONVOLVE_C ||
+                   semantics == CH_SEM_CUDA_DOSE,
+               "ch_plan_create: unknown semantics %d", semantics);
+    for (int a = 0; a < 3; ++a)
+        CH_REQUIRE(h_vsize[a] > 0, "ch_plan_create: voxel size <= 0");
+    for (int32_t c = 0; c < n_cones; ++c)
+        CH_REQUIRE(h_cone_row[c] >= 0 && h_cone_row[c] < n_rows,
+                   "ch_plan_create: cone %d uses kernel row %d of %d", c,
+                   h_cone_row[c], n_rows);
+
+    ch_plan* p = new ch_plan();
+    memset(p->shape, 0, sizeof p->shape);
+    for (int a = 0; a < 3; ++a) {
+        p->shape[a] = h_shape[a];
+        p->vsize[a] = h_vsize[a];
+    }
+    p->n_cones = n_cones;
+    p->semantics = semantics;
+    p->tap_stride = h_shape[0] + h_shape[1] + h_shape[2] + kTapPad;
+    p->f_stride = h_shape[0] + h_shape[1] + h_shape[2] + 6;
+    p->tile[0] = 1;
+    p->tile[1] = 1;
+    p->tile[2] = 64;
+    p->accum_mode = 0;
+    p->d_lin = nullptr;
+    p->d_k = nullptr;
+    p->d_k64 = nullptr;
+    p->d_off = nullptr;
+    p->d_ftab = nullptr;
+    p->d_meta = nullptr;
+
+    const size_t nt = (size_t)n_cones * p->tap_stride;
+    const size_t nf = (size_t)n_cones * p->f_stride;
+    double* d_dirs = nullptr;
+    int32_t* d_rows = nullptr;
+    double* d_cum = nullptr;
+    hipError_t e = hipSuccess;
+#define CH_TRY(x)                                                           \
+    do {                                                                    \
+        if (e == hipSuccess)                                                \
+            e = (x);                                                        \
+    } while (0)
+    CH_TRY(hipMalloc((void**)&p->d_lin, nt * sizeof(int32_t)));
+    CH_TRY(hipMalloc((void**)&p->d_k, nt * sizeof(float)));
+    CH_TRY(hipMalloc((void**)&p->d_k64, nt * sizeof(double)));
+    CH_TRY(hipMalloc((void**)&p->d_off, nt * 3 * sizeof(int32_t)));
+    CH_TRY(hipMalloc((void**)&p->d_ftab, nf * sizeof(int32_t)));
+    CH_TRY(hipMalloc((void**)&p->d_meta, n_cones * sizeof(ConeMeta)));
+    CH_TRY(hipMalloc((void**)&d_dirs, (size_t)n_cones * 3 * sizeof(double)));
+    CH_TRY(hipMalloc((void**)&d_rows, (size_t)n_cones * sizeof(int32_t)));
+    CH_TRY(hipMalloc((void**)&d_cum,
+                     (size_t)n_rows * row_len * sizeof(double)));
+    CH_TRY(hipMemcpy(d_dirs, h_dirs, (size_t)n_cones * 3 * sizeof(double),
+                     hipMemcpyHostToDevice));
+    CH_TRY(hipMemcpy(d_rows, h_cone_row, (size_t)n_cones * sizeof(int32_t),
+                     hipMemcpyHostToDevice));
+    CH_TRY(hipMemcpy(d_cum, h_cum, (size_t)n_rows * row_len * sizeof(double),
+                     hipMemcpyHostToDevice));
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(build_taps_kernel,
+                           dim3((unsigned)((n_cones + 63) / 64)), dim3(64), 0,
+                           nullptr, n_cones, d_dirs, d_rows, d_cum, row_len,
+                           h_shape[0], h_shape[1], h_shape[2], h_vsize[0],
+                           h_vsize[1], h_vsize[2], semantics, p->tap_stride,
+                           p->f_stride, p->d_lin, p->d_k, p->d_k64, p->d_off,
+                           p->d_ftab, p->d_meta);
+        e = hipGetLastError();
+    }
+    p->h_meta.resize(n_cones);
+    CH_TRY(hipMemcpy(p->h_meta.data(), p->d_meta,
+                     (size_t)n_cones * sizeof(ConeMeta),
+                     hipMemcpyDeviceToHost));
+#undef CH_TRY
+    hipFree(d_dirs);
+    hipFree(d_rows);
+    hipFree(d_cum);
+    if (e != hipSuccess) {
+        plan_free(p);
+        return hip_fail(e, "ch_plan_create", __FILE__, __LINE__);
+    }
+    p->n_taps_total = 0;
+    p->max_taps = 0;
+    for (int32_t c = 0; c < n_cones; ++c) {
+        p->n_taps_total += p->h_meta[c].n_taps;
+        if (p->h_meta[c].n_taps > p->max_taps)
+            p->max_taps = p->h_meta[c].n_taps;
+    }
+    *plan_out = p;
+    return CH_OK;
+}
+
+ch_status ch_plan_destroy(ch_plan* plan)
+{
+    plan_free(plan);
+    return CH_OK;
+}
+
+ch_status ch_plan_info(const ch_plan* plan, int32_t* n_cones, int64_t* n_taps,
+                       int32_t* max_taps)
+{
+    CH_REQUIRE(plan, "ch_plan_info: null plan");
+    if (n_cones)
+        *n_cones = plan->n_cones;
+    if (n_taps)
+        *n_taps = plan->n_taps_total;
+    if (max_taps)
+        *max_taps = plan->max_taps;
+    return CH_OK;
+}
+
+ch_status ch_plan_cone_taps(const ch_plan* plan, int32_t cone,
+                            int32_t* h_off_out, double* h_k_out, int32_t cap,
+                            int32_t* n_out)
+{
+    CH_REQUIRE(plan && n_out, "ch_plan_cone_taps: null pointer");
+    CH_REQUIRE(cone >= 0 && cone < plan->n_cones, "cone %d out of range",
+               cone);
+    const ConeMeta& m = plan->h_meta[cone];
+    *n_out = m.n_taps;
+    if (m.n_taps > cap) {
+        set_error("ch_plan_cone_taps: %d taps exceed capacity %d", m.n_taps,
+                  cap);
+        return CH_ERR_CAPACITY;
+    }
+    if (m.n_taps == 0)
+        return CH_OK;
+    CH_HIP(hipMemcpy(h_off_out, plan->d_off + 3 * (size_t)m.tap_begin,
+                     (size_t)m.n_taps * 3 * sizeof(int32_t),
+                     hipMemcpyDeviceToHost));
+    CH_HIP(hipMemcpy(h_k_out, plan->d_k64 + (size_t)m.tap_begin,
+                     (size_t)m.n_taps * sizeof(double),
+                     hipMemcpyDeviceToHost));
+    return CH_OK;
+}
+
+ch_status ch_plan_set_tile(ch_plan* plan, int32_t tx, int32_t ty, int32_t tz)
+{
+    CH_REQUIRE(plan, "ch_plan_set_tile: null plan");
+    CH_REQUIRE(tx > 0 && ty > 0 && tz > 0 && tx * ty * tz == 64,
+               "tile %dx%dx%d must hold 64 voxels", tx, ty, tz);
+    plan->tile[0] = tx;
+    plan->tile[1] = ty;
+    plan->tile[2] = tz;
+    return CH_OK;
+}
+
+ch_status ch_plan_get_tile(const ch_plan* plan, int32_t* tx, int32_t* ty,
+                           int32_t* tz)
+{
+    CH_REQUIRE(plan, "ch_plan_get_tile: null plan");
+    *tx = plan->tile[0];
+    *ty = plan->tile[1];
+    *tz = plan->tile[2];
+    return CH_OK;
+}
+
+ch_status ch_plan_set_accum(ch_plan* plan, int32_t mode)
+{
+    CH_REQUIRE(plan, "ch_plan_set_accum: null plan");
+    CH_REQUIRE(mode == 0 || mode == 1, "accumulation mode %d", mode);
+    plan->accum_mode = mode;
+    return CH_OK;
+}
+
+}  // extern "C"

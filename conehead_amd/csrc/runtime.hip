@@ -1,0 +1,275 @@
+// runtime.hip -- device, memory, stream and event plumbing of the C-ABI.
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "ch_internal.h"
+
+namespace ch {
+
+static thread_local std::string g_error;
+static thread_local float g_last_kernel_ms = -1.0f;
+static bool g_timing = false;
+static int32_t g_num_cus = 0;
+
+void set_error(const char* fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_error = buf;
+}
+
+ch_status hip_fail(hipError_t e, const char* what, const char* file, int line)
+{
+    set_error("%s failed: %s (%s:%d)", what, hipGetErrorString(e), file, line);
+    return e == hipErrorOutOfMemory ? CH_ERR_NOMEM : CH_ERR_HIP;
+}
+
+int32_t num_cus()
+{
+    if (g_num_cus <= 0) {
+        int dev = 0;
+        hipDeviceProp_t p;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipGetDeviceProperties(&p, dev) == hipSuccess)
+            g_num_cus = p.multiProcessorCount;
+        else
+            g_num_cus = 256;
+    }
+    return g_num_cus;
+}
+
+void record_kernel_ms(float ms) { g_last_kernel_ms = ms; }
+bool timing_enabled() { return g_timing; }
+
+// Grid-stride zero fill in 16-byte units plus a byte tail.
+__global__ void zero_kernel(uint8_t* p, size_t n)
+{
+    size_t n16 = n / 16;
+    size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    uint4* q = reinterpret_cast<uint4*>(p);
+    for (size_t j = i; j < n16; j += stride)
+        q[j] = make_uint4(0, 0, 0, 0);
+    for (size_t j = n16 * 16 + i; j < n; j += stride)
+        p[j] = 0;
+}
+
+}  // namespace ch
+
+using namespace ch;
+
+extern "C" {
+
+const char* ch_last_error(void) { return g_error.c_str(); }
+
+int32_t ch_version(void) { return 10000; }
+
+ch_status ch_init(int32_t device, int32_t* n_cu, int64_t* hbm_bytes,
+                  char* name_out, int32_t name_cap)
+{
+    int count = 0;
+    CH_HIP(hipGetDeviceCount(&count));
+    CH_REQUIRE(device >= 0 && device < count, "device %d out of range (%d)",
+               device, count);
+    CH_HIP(hipSetDevice(device));
+    hipDeviceProp_t p;
+    CH_HIP(hipGetDeviceProperties(&p, device));
+    g_num_cus = p.multiProcessorCount;
+    if (n_cu)
+        *n_cu = p.multiProcessorCount;
+    if (hbm_bytes)
+        *hbm_bytes = (int64_t)p.totalGlobalMem;
+    if (name_out && name_cap > 0) {
+        snprintf(name_out, (size_t)name_cap, "%s", p.gcnArchName);
+    }
+    return CH_OK;
+}
+
+ch_status ch_malloc(void** d_ptr, size_t bytes)
+{
+    CH_REQUIRE(d_ptr != nullptr, "ch_malloc: null output");
+    *d_ptr = nullptr;
+    if (bytes == 0)
+        bytes = 16;
+    CH_HIP(hipMalloc(d_ptr, bytes));
+    return CH_OK;
+}
+
+ch_status ch_free(void* d_ptr)
+{
+    if (d_ptr)
+        CH_HIP(hipFree(d_ptr));
+    return CH_OK;
+}
+
+ch_status ch_mem_info(size_t* free_bytes, size_t* total_bytes)
+{
+    size_t f = 0, t = 0;
+    CH_HIP(hipMemGetInfo(&f, &t));
+    if (free_bytes)
+        *free_bytes = f;
+    if (total_bytes)
+        *total_bytes = t;
+    return CH_OK;
+}
+
+ch_status ch_memcpy_h2d(void* d_dst, const void* h_src, size_t bytes,
+                        void* stream)
+{
+    if (bytes == 0)
+        return CH_OK;
+    CH_REQUIRE(d_dst && h_src, "ch_memcpy_h2d: null pointer");
+    CH_HIP(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice,
+                          (hipStream_t)stream));
+    return CH_OK;
+}
+
+ch_status ch_memcpy_d2h(void* h_dst, const void* d_src, size_t bytes,
+                        void* stream)
+{
+    if (bytes == 0)
+        return CH_OK;
+    CH_REQUIRE(h_dst && d_src, "ch_memcpy_d2h: null pointer");
+    CH_HIP(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost,
+                          (hipStream_t)stream));
+    CH_HIP(hipStreamSynchronize((hipStream_t)stream));
+    return CH_OK;
+}
+
+ch_status ch_memcpy_d2d(void* d_dst, const void* d_src, size_t bytes,
+                        void* stream)
+{
+    if (bytes == 0)
+        return CH_OK;
+    CH_REQUIRE(d_dst && d_src, "ch_memcpy_d2d: null pointer");
+    CH_HIP(hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice,
+                          (hipStream_t)stream));
+    return CH_OK;
+}
+
+ch_status ch_zero(void* d_ptr, size_t bytes, void* stream)
+{
+    if (bytes == 0)
+        return CH_OK;
+    CH_REQUIRE(d_ptr, "ch_zero: null pointer");
+    size_t n16 = (bytes + 15) / 16;
+    size_t blocks = (n16 + 255) / 256;
+    if (blocks > 8192)
+        blocks = 8192;
+    if (blocks == 0)
+        blocks = 1;
+    hipLaunchKernelGGL(zero_kernel, dim3((unsigned)blocks), dim3(256), 0,
+                       (hipStream_t)stream, (uint8_t*)d_ptr, bytes);
+    CH_HIP(hipGetLastError());
+    return CH_OK;
+}
+
+ch_status ch_host_alloc(void** h_ptr, size_t bytes)
+{
+    CH_REQUIRE(h_ptr, "ch_host_alloc: null output");
+    CH_HIP(hipHostMalloc(h_ptr, bytes ? bytes : 16, hipHostMallocDefault));
+    return CH_OK;
+}
+
+ch_status ch_host_free(void* h_ptr)
+{
+    if (h_ptr)
+        CH_HIP(hipHostFree(h_ptr));
+    return CH_OK;
+}
+
+ch_status ch_stream_create(void** stream)
+{
+    CH_REQUIRE(stream, "ch_stream_create: null output");
+    hipStream_t s;
+    // Default priority, non-blocking with respect to the null stream.
+    CH_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *stream = (void*)s;
+    return CH_OK;
+}
+
+ch_status ch_stream_destroy(void* stream)
+{
+    if (stream)
+        CH_HIP(hipStreamDestroy((hipStream_t)stream));
+    return CH_OK;
+}
+
+ch_status ch_stream_sync(void* stream)
+{
+    CH_HIP(hipStreamSynchronize((hipStream_t)stream));
+    return CH_OK;
+}
+
+ch_status ch_device_sync(void)
+{
+    CH_HIP(hipDeviceSynchronize());
+    return CH_OK;
+}
+
+ch_status ch_event_create(void** event)
+{
+    CH_REQUIRE(event, "ch_event_create: null output");
+    hipEvent_t e;
+    CH_HIP(hipEventCreate(&e));
+    *event = (void*)e;
+    return CH_OK;
+}
+
+ch_status ch_event_destroy(void* event)
+{
+    if (event)
+        CH_HIP(hipEventDestroy((hipEvent_t)event));
+    return CH_OK;
+}
+
+ch_status ch_event_record(void* event, void* stream)
+{
+    CH_HIP(hipEventRecord((hipEvent_t)event, (hipStream_t)stream));
+    return CH_OK;
+}
+
+ch_status ch_event_sync(void* event)
+{
+    CH_HIP(hipEventSynchronize((hipEvent_t)event));
+    return CH_OK;
+}
+
+ch_status ch_event_query(void* event, int32_t* done)
+{
+    hipError_t e = hipEventQuery((hipEvent_t)event);
+    if (e == hipSuccess) {
+        *done = 1;
+        return CH_OK;
+    }
+    if (e == hipErrorNotReady) {
+        *done = 0;
+        return CH_OK;
+    }
+    return hip_fail(e, "hipEventQuery", __FILE__, __LINE__);
+}
+
+ch_status ch_event_elapsed(float* ms, void* start, void* stop)
+{
+    CH_HIP(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
+    return CH_OK;
+}
+
+ch_status ch_set_timing(int32_t enabled)
+{
+    g_timing = enabled != 0;
+    return CH_OK;
+}
+
+ch_status ch_last_kernel_ms(float* ms)
+{
+    *ms = g_last_kernel_ms;
+    return CH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,425 @@
+// terma.hip -- fused TERMA: hit test, FP64-DDA radiological depth, off-axis
+// distance, three-source fluence, softening, horn and the polyenergetic sum,
+// one lane per dose-grid voxel, one launch per beam.
+//
+// The reference runs these as six kernels with a host round trip after each
+// (conehead/conehead.py:536-682).  Geometry (sub-sample positions, plane
+// hits, ray directions, DDA alphas) is FP64 with contraction off so that the
+// traversal and the aperture lookups match the FP64 oracle bit for bit; the
+// depth integral uses FP32 with Kahan compensation and the fluence and TERMA
+// arithmetic is FP32.
+#include <climits>
+#include <cstring>
+
+#include "dda_core.h"
+
+namespace ch {
+
+struct TermaParams {
+    double src[3];
+    double vy[3];
+    double tr0[3];     // row 0 of the world -> source rotation
+    double tr2[3];     // row 2
+    double origin[3];
+    double spacing[3];
+    double sad;
+    // Thresholds stay FP64 so the piecewise terms switch exactly where the
+    // FP64 oracle switches them.
+    double rInner, rOuter, softLimit, zAnn64;
+    float sPri, sAnn, zAnn, zExp, sExp, kExp;
+    float softRatio, hornRatio;
+    int32_t samples, n_energy;
+    int32_t X, Y, Z;
+    float ew[CH_MAX_ENERGIES];   // weight * energy * mu, folded on the host
+    float emu[CH_MAX_ENERGIES];  // mu
+    const float* raster;
+    const float* density;
+    float* terma;
+    int32_t* box;
+    float* debug;              // 4 planes of X*Y*Z or null
+    int32_t* status;           // geometry error flag
+};
+
+__device__ __forceinline__ double dot3(const double* a, double b0, double b1,
+                                       double b2)
+{
+    return a[0] * b0 + a[1] * b1 + a[2] * b2;
+}
+
+// conehead/conehead.py:50-68 on the 4000 x 4000 raster; a -1 index wraps
+// to the last row/column as Python/Numba negative indexing does.
+__device__ __forceinline__ float transmission(double px, double pz,
+                                              const float* __restrict__ r)
+{
+    px = floor(px * 100.0) + 2000.0;
+    pz = floor(pz * 100.0) + 2000.0;
+    if (px < 0 || px > 3999 || pz < 0 || pz > 3999)
+        return 0.0f;
+    int i = (int)px - 1, j = (int)pz - 1;
+    if (i < 0)
+        i += 4000;
+    if (j < 0)
+        j += 4000;
+    return r[i * 4000 + j];
+}
+
+__device__ __forceinline__ int wmin(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        v = min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ __forceinline__ int wmax(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Voxel centre and unit direction toward the source (conehead.py:226-238),
+// FP64 without contraction.
+__device__ __forceinline__ void ray_to_source(const TermaParams& P, int x,
+                                              int y, int z, double* pos,
+                                              double* r, double& mag,
+                                              double* dir)
+{
+    pos[0] = P.origin[0] + P.spacing[0] * x;
+    pos[1] = P.origin[1] + P.spacing[1] * y;
+    pos[2] = P.origin[2] + P.spacing[2] * z;
+    r[0] = P.src[0] - pos[0];
+    r[1] = P.src[1] - pos[1];
+    r[2] = P.src[2] - pos[2];
+    mag = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+    dir[0] = r[0] / mag;
+    dir[1] = r[1] / mag;
+    dir[2] = r[2] / mag;
+}
+
+// The traversal of each TERMA ray, for parity checks of the depth integral.
+__global__ __launch_bounds__(256) void terma_rays_kernel(
+    TermaParams P, int32_t n, const int32_t* __restrict__ vox_in,
+    int32_t cap, double* __restrict__ dirs, int32_t* __restrict__ vox,
+    double* __restrict__ tout, int32_t* __restrict__ count)
+{
+    const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n)
+        return;
+    int32_t c0 = vox_in[3 * r], c1 = vox_in[3 * r + 1], c2 = vox_in[3 * r + 2];
+    double pos[3], rv[3], dir[3], mag;
+    ray_to_source(P, c0, c1, c2, pos, rv, mag, dir);
+    dirs[3 * r] = dir[0];
+    dirs[3 * r + 1] = dir[1];
+    dirs[3 * r + 2] = dir[2];
+    Dda d;
+    d.init(dir[0], dir[1], dir[2], P.spacing[0], P.spacing[1], P.spacing[2]);
+    int32_t m = 0;
+    while (c0 >= 0 && c0 < P.X && c1 >= 0 && c1 < P.Y && c2 >= 0 &&
+           c2 < P.Z) {
+        if (m < cap) {
+            vox[((int64_t)r * cap + m) * 3 + 0] = c0;
+            vox[((int64_t)r * cap + m) * 3 + 1] = c1;
+            vox[((int64_t)r * cap + m) * 3 + 2] = c2;
+        }
+        const double t = d.advance(d.axis(), c0, c1, c2);
+        if (m < cap)
+            tout[(int64_t)r * cap + m] = t;
+        ++m;
+    }
+    count[r] = m;
+}
+
+__global__ __launch_bounds__(256) void terma_kernel(TermaParams P)
+{
+    const int64_t nv = (int64_t)P.X * P.Y * P.Z;
+    const int64_t li = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const bool inside = li < nv;
+    const int z = inside ? (int)(li % P.Z) : 0;
+    const int y = inside ? (int)((li / P.Z) % P.Y) : 0;
+    const int x = inside ? (int)(li / ((int64_t)P.Z * P.Y)) : 0;
+    float T = 0.0f;
+    if (inside) {
+        double pos[3];
+        pos[0] = P.origin[0] + P.spacing[0] * x;
+        pos[1] = P.origin[1] + P.spacing[1] * y;
+        pos[2] = P.origin[2] + P.spacing[2] * z;
+
+        // hit test (conehead.py:78-120)
+        const int ns = P.samples;
+        const double n3 = (double)(ns * ns * ns);
+        double off[3];
+        for (int a = 0; a < 3; ++a)
+            off[a] = P.spacing[a] / ns;
+        const double ndotn_src = dot3(P.vy, P.src[0], P.src[1], P.src[2]);
+        double blocked = 0.0;
+        bool bad = false;
+        for (int ix = 0; ix < ns; ++ix)
+            for (int iy = 0; iy < ns; ++iy)
+                for (int iz = 0; iz < ns; ++iz) {
+                    const double ps0 = pos[0] - P.spacing[0] / 2 + off[0] / 2 + off[0] * ix;
+                    const double ps1 = pos[1] - P.spacing[1] / 2 + off[1] / 2 + off[1] * iy;
+                    const double ps2 = pos[2] - P.spacing[2] / 2 + off[2] / 2 + off[2] * iz;
+                    const double u0 = P.src[0] - ps0, u1 = P.src[1] - ps1,
+                                 u2 = P.src[2] - ps2;
+                    const double ndotu = dot3(P.vy, u0, u1, u2);
+                    if (fabs(ndotu) < 1e-6) {
+                        bad = true;
+                        continue;
+                    }
+                    const double si = -ndotn_src / ndotu;
+                    const double pp0 = P.src[0] + si * u0 + 0.0;
+                    const double pp1 = P.src[1] + si * u1 + 0.0;
+                    const double pp2 = P.src[2] + si * u2 + 0.0;
+                    const double q0 = dot3(P.tr0, pp0, pp1, pp2);
+                    const double q2 = dot3(P.tr2, pp0, pp1, pp2);
+                    blocked = blocked + transmission(q0, q2, P.raster) / n3;
+                }
+
+        const bool dbg = P.debug != nullptr;
+        if (blocked != 0.0 || dbg) {
+            // radiological depth toward the source (conehead.py:209-254)
+            double rv[3], dir[3], mag, pos2[3];
+            ray_to_source(P, x, y, z, pos2, rv, mag, dir);
+            const double r0 = rv[0], r1 = rv[1], r2 = rv[2];
+            Dda d;
+            d.init(dir[0], dir[1], dir[2], P.spacing[0], P.spacing[1],
+                   P.spacing[2]);
+            int32_t c0 = x, c1 = y, c2 = z;
+            double tprev = 0.0;
+            float de = 0.0f, dc = 0.0f;
+            while (c0 >= 0 && c0 < P.X && c1 >= 0 && c1 < P.Y && c2 >= 0 &&
+                   c2 < P.Z) {
+                const float rho = P.density[((int64_t)c0 * P.Y + c1) * P.Z + c2];
+                const double t = d.advance(d.axis(), c0, c1, c2);
+                const float term = rho * (float)(t - tprev);
+                tprev = t;
+                const float yk = term - dc;
+                const float tk = de + yk;
+                dc = (tk - de) - yk;
+                de = tk;
+            }
+
+            // off-axis distance (conehead.py:286-314)
+            double oad = 0.0;
+            const double ndotr = dot3(P.vy, r0, r1, r2);
+            if (fabs(ndotr) < 1e-6) {
+                bad = true;
+            } else {
+                const double si = -ndotn_src / ndotr;
+                const double pp0 = P.src[0] + si * r0 + 0.0;
+                const double pp1 = P.src[1] + si * r1 + 0.0;
+                const double pp2 = P.src[2] + si * r2 + 0.0;
+                const double q0 = dot3(P.tr0, pp0, pp1, pp2);
+                const double q2 = dot3(P.tr2, pp0, pp1, pp2);
+                oad = sqrt(q0 * q0 + q2 * q2);
+            }
+
+            // fluence (conehead.py:320-360)
+            const float magf = (float)mag, oadf = (float)oad;
+            const float sad = (float)P.sad;
+            const float g = sad / magf;
+            const float fp = P.sPri * (g * g);
+            const double r_ann = oad * P.zAnn64 / P.sad;
+            float fa = 0.0f;
+            if (r_ann >= P.rInner && r_ann <= P.rOuter) {
+                const float a = sad - P.zAnn, b = magf - P.zAnn;
+                fa = P.sAnn * (a * a) / (b * b);
+            }
+            const float oc = oadf < 2.0f ? 2.0f : oadf;
+            const float r_exp = oc * P.zExp / sad;
+            const float ea = sad - P.zExp, eb = magf - P.zExp;
+            const float fe = P.sExp / r_exp * expf(-P.kExp * r_exp) *
+                             (ea * ea) / (eb * eb);
+            const float bl = (float)blocked;
+            const float flu = (fp + fa + fe) * bl;
+
+            // softening, horn (conehead.py:641-655), TERMA (:364-377)
+            const float fs = oad < P.softLimit
+                                 ? 1.0f / (1.0f - P.softRatio * oadf)
+                                 : 1.0f;
+            const float fh = 1.0f + oadf * P.hornRatio;
+            const float sd = fs * de;
+            float acc = 0.0f;
+            for (int i = 0; i < P.n_energy; ++i)
+                acc += P.ew[i] * flu * expf(-P.emu[i] * sd);
+            T = acc * fh * bl;
+            if (dbg) {
+                P.debug[li] = bl;
+                P.debug[nv + li] = de;
+                P.debug[2 * nv + li] = oadf;
+                P.debug[3 * nv + li] = flu;
+            }
+        }
+        P.terma[li] = T;
+        if (bad)
+            atomicOr(P.status, 1);
+    }
+    if (P.box) {
+        int mn0 = INT_MAX, mn1 = INT_MAX, mn2 = INT_MAX;
+        int mx0 = INT_MIN, mx1 = INT_MIN, mx2 = INT_MIN;
+        if (inside && T != 0.0f) {
+            mn0 = mx0 = x;
+            mn1 = mx1 = y;
+            mn2 = mx2 = z;
+        }
+        mn0 = wmin(mn0);
+        mn1 = wmin(mn1);
+        mn2 = wmin(mn2);
+        mx0 = wmax(mx0);
+        mx1 = wmax(mx1);
+        mx2 = wmax(mx2);
+        if ((threadIdx.x & 63) == 0 && mx0 != INT_MIN) {
+            atomicMin(P.box + 0, mn0);
+            atomicMin(P.box + 1, mn1);
+            atomicMin(P.box + 2, mn2);
+            atomicMax(P.box + 3, mx0);
+            atomicMax(P.box + 4, mx1);
+            atomicMax(P.box + 5, mx2);
+        }
+    }
+}
+
+__global__ __launch_bounds__(64) void terma_prep_kernel(int32_t* box,
+                                                        int32_t* status)
+{
+    if (box) {
+        if (threadIdx.x < 3)
+            box[threadIdx.x] = INT_MAX;
+        else if (threadIdx.x < 6)
+            box[threadIdx.x] = INT_MIN;
+    }
+    if (status && threadIdx.x == 0)
+        status[0] = 0;
+}
+
+static int32_t* g_status = nullptr;
+
+static TermaParams make_params(const ch_beam& B, const int32_t* h_shape,
+                               const double* h_origin,
+                               const double* h_spacing)
+{
+    TermaParams P;
+    memset(&P, 0, sizeof P);
+    for (int a = 0; a < 3; ++a) {
+        P.src[a] = B.source_position[a];
+        P.vy[a] = B.v_y[a];
+        P.tr0[a] = B.transform[a];
+        P.tr2[a] = B.transform[6 + a];
+        P.origin[a] = h_origin[a];
+        P.spacing[a] = h_spacing[a];
+    }
+    P.sad = B.sad;
+    P.sPri = (float)B.sPri;
+    P.sAnn = (float)B.sAnn;
+    P.zAnn = (float)B.zAnn;
+    P.zAnn64 = B.zAnn;
+    P.rInner = B.rInner;
+    P.rOuter = B.rOuter;
+    P.zExp = (float)B.zExp;
+    P.sExp = (float)B.sExp;
+    P.kExp = (float)B.kExp;
+    P.softRatio = (float)B.softRatio;
+    P.softLimit = B.softLimit;
+    P.hornRatio = (float)B.hornRatio;
+    P.samples = B.samples;
+    P.n_energy = B.n_energy;
+    P.X = h_shape[0];
+    P.Y = h_shape[1];
+    P.Z = h_shape[2];
+    for (int i = 0; i < B.n_energy; ++i) {
+        P.ew[i] = (float)(B.weight[i] * B.energy[i] * B.mu[i]);
+        P.emu[i] = (float)B.mu[i];
+    }
+    return P;
+}
+
+}  // namespace ch
+
+using namespace ch;
+
+extern "C" {
+
+ch_status ch_terma(int32_t n_beams, const ch_beam* h_beams,
+                   const int32_t h_shape[3], const double h_origin[3],
+                   const double h_spacing[3], float* d_debug, void* stream)
+{
+    CH_REQUIRE(n_beams >= 0, "n_beams < 0");
+    if (n_beams == 0)
+        return CH_OK;
+    CH_REQUIRE(h_beams && h_shape && h_origin && h_spacing,
+               "ch_terma: null pointer");
+    CH_REQUIRE(h_shape[0] > 0 && h_shape[1] > 0 && h_shape[2] > 0,
+               "ch_terma: empty grid");
+    const int64_t nv = (int64_t)h_shape[0] * h_shape[1] * h_shape[2];
+    CH_REQUIRE(nv < (1LL << 31), "ch_terma: grid too large");
+    hipStream_t s = (hipStream_t)stream;
+    if (!g_status)
+        CH_HIP(hipMalloc((void**)&g_status, 64 * sizeof(int32_t)));
+    // The geometry flag covers every beam of this call.
+    hipLaunchKernelGGL(terma_prep_kernel, dim3(1), dim3(64), 0, s, nullptr,
+                       g_status);
+    for (int32_t b = 0; b < n_beams; ++b) {
+        const ch_beam& B = h_beams[b];
+        CH_REQUIRE(B.d_raster && B.d_density && B.d_terma,
+                   "beam %d: null device buffer", b);
+        CH_REQUIRE(B.samples >= 1 && B.samples <= 16,
+                   "beam %d: fluenceResampling %d", b, B.samples);
+        CH_REQUIRE(B.n_energy >= 0 && B.n_energy <= CH_MAX_ENERGIES,
+                   "beam %d: %d energies", b, B.n_energy);
+        TermaParams P = make_params(B, h_shape, h_origin, h_spacing);
+        P.raster = B.d_raster;
+        P.density = B.d_density;
+        P.terma = B.d_terma;
+        P.box = B.d_box;
+        P.debug = d_debug ? d_debug + (size_t)b * 4 * nv : nullptr;
+        P.status = g_status;
+        if (P.box)
+            hipLaunchKernelGGL(terma_prep_kernel, dim3(1), dim3(64), 0, s,
+                               P.box, nullptr);
+        hipLaunchKernelGGL(terma_kernel, dim3((unsigned)((nv + 255) / 256)),
+                           dim3(256), 0, s, P);
+        CH_HIP(hipGetLastError());
+    }
+    return CH_OK;
+}
+
+ch_status ch_terma_rays(const ch_beam* h_beam, const int32_t h_shape[3],
+                        const double h_origin[3], const double h_spacing[3],
+                        int32_t n_rays, const int32_t* d_vox_in, int32_t cap,
+                        double* d_dirs, int32_t* d_vox, double* d_t,
+                        int32_t* d_count, void* stream)
+{
+    CH_REQUIRE(h_beam && h_shape && h_origin && h_spacing,
+               "ch_terma_rays: null pointer");
+    CH_REQUIRE(n_rays >= 0 && cap >= 0, "ch_terma_rays: bad sizes");
+    if (n_rays == 0)
+        return CH_OK;
+    CH_REQUIRE(d_vox_in && d_dirs && d_vox && d_t && d_count,
+               "ch_terma_rays: null device buffer");
+    TermaParams P = make_params(*h_beam, h_shape, h_origin, h_spacing);
+    hipLaunchKernelGGL(terma_rays_kernel,
+                       dim3((unsigned)((n_rays + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, P, n_rays, d_vox_in, cap, d_dirs,
+                       d_vox, d_t, d_count);
+    CH_HIP(hipGetLastError());
+    return CH_OK;
+}
+
+ch_status ch_terma_check(void* stream)
+{
+    if (!g_status)
+        return CH_OK;
+    int32_t flag = 0;
+    CH_HIP(hipMemcpyAsync(&flag, g_status, sizeof flag,
+                          hipMemcpyDeviceToHost, (hipStream_t)stream));
+    CH_HIP(hipStreamSynchronize((hipStream_t)stream));
+    if (flag) {
+        set_error("no intersection or line is within plane");
+        return CH_ERR_GEOMETRY;
+    }
+    return CH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,154 @@
+"""Device buffers, streams and events on top of the C-ABI.
+
+The library allocates HBM itself (ch_malloc); numpy arrays are the host side.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+
+class Stream:
+    """A HIP stream (default priority)."""
+
+    def __init__(self, handle=None):
+        self._own = handle is None
+        if handle is None:
+            h = ctypes.c_void_p()
+            _lib.check(_lib.lib().ch_stream_create(ctypes.byref(h)))
+            handle = h.value
+        self.handle = handle
+
+    def sync(self):
+        _lib.check(_lib.lib().ch_stream_sync(self.handle))
+
+    def close(self):
+        if self._own and self.handle:
+            _lib.lib().ch_stream_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+NULL_STREAM = None
+
+
+def _h(stream):
+    return stream.handle if isinstance(stream, Stream) else stream
+
+
+class Event:
+    def __init__(self):
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().ch_event_create(ctypes.byref(h)))
+        self.handle = h.value
+
+    def record(self, stream=None):
+        _lib.check(_lib.lib().ch_event_record(self.handle, _h(stream)))
+        return self
+
+    def sync(self):
+        _lib.check(_lib.lib().ch_event_sync(self.handle))
+
+    def done(self):
+        d = ctypes.c_int32()
+        _lib.check(_lib.lib().ch_event_query(self.handle, ctypes.byref(d)))
+        return bool(d.value)
+
+    def elapsed_ms(self, later):
+        ms = ctypes.c_float()
+        _lib.check(_lib.lib().ch_event_elapsed(ctypes.byref(ms), self.handle,
+                                               later.handle))
+        return float(ms.value)
+
+    def __del__(self):
+        try:
+            if self.handle:
+                _lib.lib().ch_event_destroy(self.handle)
+        except Exception:
+            pass
+
+
+class DeviceArray:
+    """An HBM allocation with a numpy-like shape and dtype."""
+
+    def __init__(self, shape, dtype=np.float32, zero=False, stream=None):
+        self.shape = tuple(int(s) for s in np.atleast_1d(shape)) \
+            if not isinstance(shape, tuple) else tuple(int(s) for s in shape)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape, dtype=np.int64)) * \
+            self.dtype.itemsize
+        p = ctypes.c_void_p()
+        _lib.check(_lib.lib().ch_malloc(ctypes.byref(p), self.nbytes))
+        self.ptr = p.value
+        if zero:
+            self.zero(stream)
+
+    @classmethod
+    def from_host(cls, arr, dtype=None, stream=None):
+        a = np.ascontiguousarray(arr, dtype=dtype or np.asarray(arr).dtype)
+        d = cls(a.shape, a.dtype)
+        d.upload(a, stream)
+        return d
+
+    @property
+    def size(self):
+        return int(np.prod(self.shape, dtype=np.int64))
+
+    def upload(self, arr, stream=None):
+        a = np.ascontiguousarray(arr, dtype=self.dtype)
+        if a.nbytes != self.nbytes:
+            raise ValueError("upload of %d bytes into %d" %
+                             (a.nbytes, self.nbytes))
+        _lib.check(_lib.lib().ch_memcpy_h2d(self.ptr, a.ctypes.data,
+                                            self.nbytes, _h(stream)))
+        if stream is not None:
+            # pageable source: make sure the copy has consumed it
+            _lib.check(_lib.lib().ch_stream_sync(_h(stream)))
+        return self
+
+    def download(self, out=None, stream=None):
+        if out is None:
+            out = np.empty(self.shape, self.dtype)
+        if not out.flags.c_contiguous or out.nbytes != self.nbytes or \
+                out.dtype != self.dtype:
+            raise ValueError("download target must be a contiguous %s %s" %
+                             (self.dtype, self.shape))
+        _lib.check(_lib.lib().ch_memcpy_d2h(out.ctypes.data, self.ptr,
+                                            self.nbytes, _h(stream)))
+        return out
+
+    def zero(self, stream=None):
+        _lib.check(_lib.lib().ch_zero(self.ptr, self.nbytes, _h(stream)))
+        return self
+
+    def free(self):
+        if self.ptr:
+            _lib.lib().ch_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def __repr__(self):
+        return "DeviceArray(%s, %s, 0x%x)" % (self.shape, self.dtype,
+                                             self.ptr or 0)
+
+
+def mem_info():
+    f = ctypes.c_size_t()
+    t = ctypes.c_size_t()
+    _lib.check(_lib.lib().ch_mem_info(ctypes.byref(f), ctypes.byref(t)))
+    return f.value, t.value
+
+
+def device_sync():
+    _lib.check(_lib.lib().ch_device_sync())

@@ -1,0 +1,257 @@
+"""Device-resident dose engine: plan store, aperture rasters, phantoms,
+TERMA and the collapsed-cone gather, all through the C-ABI.
+
+Everything that is reused across beams and requests stays in HBM: cone
+plans (tap tables) keyed by (grid shape, voxel size, cone set), aperture
+rasters keyed by their recipe, phantom densities keyed by the caller.
+"""
+import ctypes
+import hashlib
+import threading
+
+import numpy as np
+
+from . import _lib
+from .cones import ConeSet
+from .device import DeviceArray, Stream, _h
+from .nist import mu_water
+
+SETTINGS_KEYS = ("sPri", "sAnn", "zAnn", "rInner", "rOuter", "zExp", "sExp",
+                 "kExp", "softRatio", "softLimit", "hornRatio")
+
+SEMANTICS = {"convolve_c": _lib.CH_SEM_CONVOLVE_C,
+             "cuda_dose": _lib.CH_SEM_CUDA_DOSE}
+
+
+class Plan:
+    """A ch_plan: tap tables of one cone set on one grid, in HBM."""
+
+    def __init__(self, shape, voxel_size, cones):
+        L = _lib.lib()
+        self.shape = tuple(int(s) for s in shape)
+        self.voxel_size = tuple(float(v) for v in voxel_size)
+        self.cones = cones
+        shp = (ctypes.c_int32 * 3)(*self.shape)
+        vs = (ctypes.c_double * 3)(*self.voxel_size)
+        dirs = np.ascontiguousarray(cones.dirs, dtype=np.float64)
+        rows = np.ascontiguousarray(cones.rows, dtype=np.int32)
+        cum = np.ascontiguousarray(cones.cum, dtype=np.float64)
+        h = ctypes.c_void_p()
+        _lib.check(L.ch_plan_create(
+            shp, vs, len(rows), dirs.ctypes.data_as(_lib.c_f64p),
+            rows.ctypes.data_as(_lib.c_i32p), cum.shape[0], cum.shape[1],
+            cum.ctypes.data_as(_lib.c_f64p), SEMANTICS[cones.semantics],
+            ctypes.byref(h)))
+        self.handle = h.value
+        nc = ctypes.c_int32()
+        nt = ctypes.c_int64()
+        mx = ctypes.c_int32()
+        _lib.check(L.ch_plan_info(self.handle, ctypes.byref(nc),
+                                  ctypes.byref(nt), ctypes.byref(mx)))
+        self.n_cones, self.n_taps, self.max_taps = nc.value, nt.value, \
+            mx.value
+
+    def set_tile(self, tx, ty, tz):
+        _lib.check(_lib.lib().ch_plan_set_tile(self.handle, tx, ty, tz))
+
+    def tile(self):
+        a, b, c = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        _lib.check(_lib.lib().ch_plan_get_tile(self.handle, ctypes.byref(a),
+                                               ctypes.byref(b),
+                                               ctypes.byref(c)))
+        return a.value, b.value, c.value
+
+    def set_accum(self, mode):
+        _lib.check(_lib.lib().ch_plan_set_accum(self.handle, int(mode)))
+
+    def cone_taps(self, cone):
+        """(offsets int32[M,3], FP64 increments[M]) of one cone."""
+        cap = self.max_taps + 1
+        off = np.zeros((cap, 3), np.int32)
+        k = np.zeros(cap, np.float64)
+        n = ctypes.c_int32()
+        _lib.check(_lib.lib().ch_plan_cone_taps(
+            self.handle, int(cone), off.ctypes.data_as(_lib.c_i32p),
+            k.ctypes.data_as(_lib.c_f64p), cap, ctypes.byref(n)))
+        return off[:n.value].copy(), k[:n.value].copy()
+
+    def close(self):
+        if self.handle:
+            _lib.lib().ch_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def plan_key(shape, voxel_size, cones):
+    h = hashlib.sha1()
+    h.update(np.asarray(shape, np.int64).tobytes())
+    h.update(np.asarray(voxel_size, np.float64).tobytes())
+    for part in cones.key():
+        h.update(part.encode() if isinstance(part, str) else part)
+    return h.hexdigest()
+
+
+def support_box(d_terma, shape, d_box, stream=None):
+    shp = (ctypes.c_int32 * 3)(*[int(s) for s in shape])
+    _lib.check(_lib.lib().ch_support_box(d_terma.ptr, shp, d_box.ptr,
+                                         _h(stream)))
+
+
+def convolve(plan, d_terma, d_dose, d_box=None, accumulate=False,
+             stream=None):
+    """dose (+)= cone convolution of terma (FP32 device arrays)."""
+    _lib.check(_lib.lib().ch_convolve(plan.handle, d_terma.ptr, d_dose.ptr,
+                                      d_box.ptr if d_box is not None else None,
+                                      1 if accumulate else 0, _h(stream)))
+
+
+def convolve_f64(plan, d_terma, d_dose64, d_box=None, stream=None):
+    """FP64 dose += cone convolution of FP32 terma."""
+    _lib.check(_lib.lib().ch_convolve_f64(plan.handle, d_terma.ptr,
+                                          d_dose64.ptr,
+                                          d_box.ptr if d_box is not None
+                                          else None, _h(stream)))
+
+
+def convolve_batch(jobs, stream=None):
+    """One launch over many (plan, terma, dose, box, accumulate) jobs."""
+    n = len(jobs)
+    if n == 0:
+        return
+    arr = (_lib.ConvJob * n)()
+    for i, (plan, t, d, box, acc) in enumerate(jobs):
+        arr[i].plan = plan.handle
+        arr[i].d_terma = t.ptr
+        arr[i].d_dose = d.ptr
+        arr[i].d_box = box.ptr if box is not None else None
+        arr[i].accumulate = 1 if acc else 0
+    _lib.check(_lib.lib().ch_convolve_batch(n, arr, _h(stream)))
+
+
+def beam_struct(source, settings, d_raster, d_density, d_terma, d_box):
+    """ch_beam for one beam: geometry from `source`, fluence and spectrum
+    parameters from the reference's settings dict (example.py:27-58).
+    Energies and weights are float32 as in conehead/conehead.py:658-661."""
+    b = _lib.Beam()
+    pos = np.asarray(source.position, dtype=np.float64)
+    vy = np.asarray(source.v_y, dtype=np.float64)
+    tr = np.asarray(source.transform, dtype=np.float64).reshape(9)
+    for i in range(3):
+        b.source_position[i] = float(pos[i])
+        b.v_y[i] = float(vy[i])
+    for i in range(9):
+        b.transform[i] = float(tr[i])
+    b.sad = float(source.sad)
+    for key in SETTINGS_KEYS:
+        setattr(b, key, float(settings[key]))
+    b.samples = int(settings["fluenceResampling"])
+    e, w, mu = spectrum(settings)
+    if len(e) > _lib.CH_MAX_ENERGIES:
+        raise ValueError("at most %d energy bins" % _lib.CH_MAX_ENERGIES)
+    b.n_energy = len(e)
+    for i in range(len(e)):
+        b.energy[i] = float(e[i])
+        b.weight[i] = float(w[i])
+        b.mu[i] = float(mu[i])
+    b.d_raster = d_raster.ptr
+    b.d_density = d_density.ptr
+    b.d_terma = d_terma.ptr
+    b.d_box = d_box.ptr if d_box is not None else None
+    return b
+
+
+def spectrum(settings):
+    """(energies, weights, mu) of settings['energy_weights']."""
+    ew = settings["energy_weights"]
+    e = np.array([np.float32(k) for k in ew.keys()], dtype=np.float32)
+    w = np.array([np.float32(v) for v in ew.values()], dtype=np.float32)
+    mu = mu_water(e.astype(np.float64))
+    return e.astype(np.float64), w.astype(np.float64), mu
+
+
+def terma(beams, shape, origin, spacing, debug=None, stream=None,
+          check=True):
+    """Run the fused TERMA kernel for a list of ch_beam structs."""
+    n = len(beams)
+    if n == 0:
+        return
+    arr = (_lib.Beam * n)(*beams)
+    shp = (ctypes.c_int32 * 3)(*[int(s) for s in shape])
+    org = (ctypes.c_double * 3)(*[float(v) for v in origin])
+    spc = (ctypes.c_double * 3)(*[float(v) for v in spacing])
+    L = _lib.lib()
+    _lib.check(L.ch_terma(n, arr, shp, org, spc,
+                          debug.ptr if debug is not None else None,
+                          _h(stream)))
+    if check:
+        _lib.check(L.ch_terma_check(_h(stream)))
+
+
+class Engine:
+    """Device-resident caches shared by the API shims, the batched path and
+    the request loop (thread-safe)."""
+
+    def __init__(self, device=0):
+        self.info = _lib.init(device)
+        self._plans = {}
+        self._rasters = {}
+        self._lock = threading.RLock()
+
+    # -- plans -----------------------------------------------------------
+    def plan(self, shape, voxel_size, cones, tile=None, accum=None):
+        key = plan_key(shape, voxel_size, cones)
+        with self._lock:
+            p = self._plans.get(key)
+            if p is None:
+                p = Plan(shape, voxel_size, cones)
+                from .autotune import default_tile
+                p.set_tile(*(tile or default_tile(p)))
+                self._plans[key] = p
+            elif tile is not None:
+                p.set_tile(*tile)
+            if accum is not None:
+                p.set_accum(accum)
+            return p
+
+    # -- apertures -------------------------------------------------------
+    def raster(self, block, stream=None):
+        """Device raster of a Block, built on the GPU from its recipe."""
+        key = block.recipe()
+        with self._lock:
+            d = self._rasters.get(key)
+            if d is not None:
+                return d
+            d = DeviceArray((4000, 4000), np.float32)
+            L = _lib.lib()
+            if key[0] == "host":
+                d.upload(block._values, stream)
+            else:
+                for op in key:
+                    if op[0] == "zeros":
+                        d.zero(stream)
+                    elif op[0] == "square":
+                        _lib.check(L.ch_raster_square(d.ptr, op[1],
+                                                      _h(stream)))
+                    elif op[0] == "mlc":
+                        bound, a, b, jx, jy = [np.asarray(v, np.int32)
+                                               for v in op[1:]]
+                        _lib.check(L.ch_raster_mlc(
+                            d.ptr, len(a), bound.ctypes.data_as(_lib.c_i32p),
+                            a.ctypes.data_as(_lib.c_i32p),
+                            b.ctypes.data_as(_lib.c_i32p),
+                            jx.ctypes.data_as(_lib.c_i32p),
+                            jy.ctypes.data_as(_lib.c_i32p), _h(stream)))
+            if len(self._rasters) > 256:
+                self._rasters.clear()
+            self._rasters[key] = d
+            return d
+
+    def drop_caches(self):
+        with self._lock:
+            self._plans.clear()
+            self._rasters.clear()

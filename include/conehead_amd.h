@@ -1,0 +1,250 @@
+/*
+ * conehead_amd.h -- C-ABI of the MI355X collapsed-cone dose engine
+ * (libconehead_amd.so, hand-written HIP kernels for gfx950).
+ *
+ * Plain pointers and sizes only.  Arguments named d_* are device pointers
+ * (from ch_malloc), h_* are host pointers.  `stream` is a hipStream_t from
+ * ch_stream_create, or NULL for the null stream.  Every function returns a
+ * ch_status (0 on success); ch_last_error() gives the message of the last
+ * failure on the calling thread.  Nothing here calls exit().
+ *
+ * Array layout: C order [X][Y][Z] (z fastest), like every reference array.
+ */
+#ifndef CONEHEAD_AMD_H
+#define CONEHEAD_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t ch_status;
+#define CH_OK 0
+#define CH_ERR_ARG (-1)      /* bad argument (shape, size, null pointer)   */
+#define CH_ERR_CAPACITY (-2) /* output capacity too small                  */
+#define CH_ERR_HIP (-3)      /* HIP runtime error                          */
+#define CH_ERR_NOMEM (-4)    /* device allocation failed                   */
+#define CH_ERR_GEOMETRY (-5) /* ray parallel to the isocentre plane        */
+
+/* ------------------------------------------------------------ runtime -- */
+
+const char* ch_last_error(void);
+int32_t ch_version(void);
+/* Selects the device for the calling thread and reports its properties. */
+ch_status ch_init(int32_t device, int32_t* n_cu, int64_t* hbm_bytes,
+                  char* name_out, int32_t name_cap);
+ch_status ch_malloc(void** d_ptr, size_t bytes);
+ch_status ch_free(void* d_ptr);
+ch_status ch_mem_info(size_t* free_bytes, size_t* total_bytes);
+ch_status ch_memcpy_h2d(void* d_dst, const void* h_src, size_t bytes,
+                        void* stream);
+ch_status ch_memcpy_d2h(void* h_dst, const void* d_src, size_t bytes,
+                        void* stream);
+ch_status ch_memcpy_d2d(void* d_dst, const void* d_src, size_t bytes,
+                        void* stream);
+/* Zero-fills with a kernel (never hipMemset), so it is safe in captures. */
+ch_status ch_zero(void* d_ptr, size_t bytes, void* stream);
+ch_status ch_host_alloc(void** h_ptr, size_t bytes);   /* pinned */
+ch_status ch_host_free(void* h_ptr);
+ch_status ch_stream_create(void** stream);
+ch_status ch_stream_destroy(void* stream);
+ch_status ch_stream_sync(void* stream);
+ch_status ch_device_sync(void);
+ch_status ch_event_create(void** event);
+ch_status ch_event_destroy(void* event);
+ch_status ch_event_record(void* event, void* stream);
+ch_status ch_event_sync(void* event);
+/* 1 if the event has completed, 0 if not. */
+ch_status ch_event_query(void* event, int32_t* done);
+ch_status ch_event_elapsed(float* ms, void* start, void* stop);
+
+/* -------------------------------------------------------------- DDA ---- */
+
+/* One 3D-DDA traversal, run on the GPU in FP64.
+ * Replaces: cdef void dda_3d_c(float64* direction, int32* grid_shape,
+ *   int32* first_voxel, float64* voxel_size, result* r) nogil
+ *   (conehead/dda_3d_c.pxd:8, conehead/dda_3d_c.pyx:13-99).
+ * Differences: caller-owned flat outputs instead of the malloc'd `vector`
+ * elements of conehead/vector.pyx:2-54; `h_dir` is not mutated (|dir| is
+ * written to h_dir_abs_out when non-NULL, mirroring dda_3d_c.pyx:38-43);
+ * capacity overflow returns CH_ERR_CAPACITY with *n_out = required size. */
+ch_status ch_dda_3d(const double h_dir[3], const int32_t h_shape[3],
+                    const int32_t h_first[3], const double h_vsize[3],
+                    int32_t* h_vox_out, double* h_t_out, int32_t cap,
+                    int32_t* n_out, double* h_dir_abs_out);
+
+/* Many traversals in one launch (one ray per lane), same arithmetic.
+ * d_dirs: n x 3, d_first: n x 3; ray r writes d_vox[r*cap*3 ...] and
+ * d_t[r*cap ...] and its full step count to d_count[r] (entries beyond cap
+ * are not written).  Replaces the per-ray dda_3d_c calls of
+ * conehead/convolve_c.pyx:121-129 and the traversal of
+ * conehead/conehead.py:123-206 (cuda_dda_3d). */
+ch_status ch_dda_trace_batch(int32_t n_rays, const double* d_dirs,
+                             const int32_t* d_first, const int32_t h_shape[3],
+                             const double h_vsize[3], int32_t cap,
+                             int32_t* d_vox, double* d_t, int32_t* d_count,
+                             void* stream);
+
+/* ------------------------------------------------------ cone plan ------ */
+
+typedef struct ch_plan ch_plan;
+
+#define CH_SEM_CONVOLVE_C 0 /* |floor(100 t - 50)|, skip > 5995
+                               (conehead/convolve_c.pyx:138-145)       */
+#define CH_SEM_CUDA_DOSE 1  /* |floor(100 t - 5)|, clamp at 5995
+                               (conehead/conehead.py:453-463)          */
+
+/* Builds the per-cone tap tables on the device: for every cone, the FP64
+ * DDA from the origin (offsets o_m, kernel increments
+ * k_m = cum[idx_m] - cum[idx_{m-1}], k_0 = cum[idx_0]) and the per-axis
+ * first-step tables used to clip each output voxel's tap range.
+ * h_dirs: n_cones x 3 discretised directions; h_cone_row[c]: row of h_cum
+ * used by cone c; h_cum: n_rows x row_len (row_len >= 5996).
+ * Replaces the per-call kernel staging of conehead/convolve_c.pyx:66-68,
+ * :236-274 (process_kernel / get_single_cone_data) and the per-voxel index
+ * computation of :133-150. */
+ch_status ch_plan_create(const int32_t h_shape[3], const double h_vsize[3],
+                         int32_t n_cones, const double* h_dirs,
+                         const int32_t* h_cone_row, int32_t n_rows,
+                         int32_t row_len, const double* h_cum,
+                         int32_t semantics, ch_plan** plan_out);
+ch_status ch_plan_destroy(ch_plan* plan);
+/* Totals of the plan: number of cones, total taps, longest tap list. */
+ch_status ch_plan_info(const ch_plan* plan, int32_t* n_cones,
+                       int64_t* n_taps, int32_t* max_taps);
+/* Copies cone c's taps to the host (offsets n x 3, FP64 increments). */
+ch_status ch_plan_cone_taps(const ch_plan* plan, int32_t cone,
+                            int32_t* h_off_out, double* h_k_out, int32_t cap,
+                            int32_t* n_out);
+
+/* ------------------------------------------------------ convolution ---- */
+
+/* Support box of a TERMA grid: d_box[0..2] = min (x,y,z) with T != 0,
+ * d_box[3..5] = max; an all-zero grid gives lo > hi. */
+ch_status ch_support_box(const float* d_terma, const int32_t h_shape[3],
+                         int32_t* d_box, void* stream);
+
+/* dose[v] (+)= sum_c sum_m k_{c,m} * terma[v - o_{c,m}] over in-grid
+ * sources, i.e. convolve_c's result as a deterministic gather in FP32 with
+ * compensated sums.  d_box (nullable: whole grid) restricts the sources to
+ * the TERMA support box.  accumulate != 0 adds into d_dose like convolve_c
+ * does (conehead/convolve_c.pyx:192); 0 overwrites.
+ * Replaces: def convolve_c(float64[:,:,:] dose_grid_terma,
+ *   float64[:,:,:] dose_grid_dose, float64[:] dose_grid_dim,
+ *   float64[:] thetas, float64[:] phis, kernel)
+ *   (conehead/convolve_c.pyx:17-20) and cuda_dose
+ *   (conehead/conehead.py:385-488). */
+ch_status ch_convolve(const ch_plan* plan, const float* d_terma,
+                      float* d_dose, const int32_t* d_box, int32_t accumulate,
+                      void* stream);
+
+/* Same convolution with an FP64 dose accumulated in place, for the
+ * convolve_c drop-in (its dose grid is float64). */
+ch_status ch_convolve_f64(const ch_plan* plan, const float* d_terma,
+                          double* d_dose, const int32_t* d_box, void* stream);
+
+/* One persistent launch over many (terma, dose) jobs that may use different
+ * plans (grid shapes, voxel sizes, cone sets): workgroups pull tiles from a
+ * device work counter until every job is done. */
+typedef struct ch_conv_job {
+    const ch_plan* plan;
+    const float* d_terma;
+    float* d_dose;
+    const int32_t* d_box;  /* nullable */
+    int32_t accumulate;
+    int32_t reserved;
+} ch_conv_job;
+ch_status ch_convolve_batch(int32_t n_jobs, const ch_conv_job* h_jobs,
+                            void* stream);
+
+/* Tile shape (voxels per wavefront along x, y, z; product 64) used by the
+ * gather kernels for plans of this shape; set by the autotuner. */
+ch_status ch_plan_set_tile(ch_plan* plan, int32_t tx, int32_t ty, int32_t tz);
+ch_status ch_plan_get_tile(const ch_plan* plan, int32_t* tx, int32_t* ty,
+                           int32_t* tz);
+/* Accumulation mode: 0 = Kahan per tap, 1 = plain per cone line with Kahan
+ * across lines. */
+ch_status ch_plan_set_accum(ch_plan* plan, int32_t mode);
+
+/* ------------------------------------------------------------ TERMA ---- */
+
+#define CH_MAX_ENERGIES 32
+
+/* Beam description.  Geometry in cm, world frame of the dose grid.
+ * d_raster: 4000 x 4000 float32 transmission raster at 0.1 mm pitch
+ * (block.block_values, conehead/block.py:14-26). */
+typedef struct ch_beam {
+    double source_position[3];
+    double v_y[3];
+    double transform[9];        /* row-major world -> source rotation */
+    double sad;
+    double sPri, sAnn, zAnn, rInner, rOuter, zExp, sExp, kExp;
+    double softRatio, softLimit, hornRatio;
+    int32_t samples;            /* settings['fluenceResampling'] */
+    int32_t n_energy;
+    double energy[CH_MAX_ENERGIES];
+    double weight[CH_MAX_ENERGIES];
+    double mu[CH_MAX_ENERGIES];
+    const float* d_raster;
+    const float* d_density;     /* [X][Y][Z] densities on the dose grid */
+    float* d_terma;             /* output [X][Y][Z] */
+    int32_t* d_box;             /* output support box (6 ints), nullable */
+} ch_beam;
+
+/* TERMA for n beams on one dose grid in one launch: hit test, FP64-DDA
+ * radiological depth toward the source, off-axis distance, three-source
+ * fluence, softening, horn and the polyenergetic TERMA sum.
+ * Replaces the kernel chain cuda_hit_test, cuda_d_eff, cuda_oad,
+ * cuda_fluence, cuda_terma and the host glue between them
+ * (conehead/conehead.py:71-377, :536-682).
+ * d_debug (nullable): n_beams x 4 x X*Y*Z floats receiving blocked, d_eff,
+ * oad and fluence for each beam. */
+ch_status ch_terma(int32_t n_beams, const ch_beam* h_beams,
+                   const int32_t h_shape[3], const double h_origin[3],
+                   const double h_spacing[3], float* d_debug, void* stream);
+
+/* The FP64 traversals of the TERMA rays toward the source from n voxels
+ * (d_vox_in: n x 3 indices), with the same direction arithmetic as
+ * ch_terma: d_dirs (n x 3) receives each unit direction, d_vox / d_t /
+ * d_count the traversal as in ch_dda_trace_batch.  Replaces the ray set-up
+ * and cuda_dda_3d call of cuda_d_eff (conehead/conehead.py:226-246). */
+ch_status ch_terma_rays(const ch_beam* h_beam, const int32_t h_shape[3],
+                        const double h_origin[3], const double h_spacing[3],
+                        int32_t n_rays, const int32_t* d_vox_in, int32_t cap,
+                        double* d_dirs, int32_t* d_vox, double* d_t,
+                        int32_t* d_count, void* stream);
+
+/* Synchronises `stream` and returns CH_ERR_GEOMETRY when a ray of the last
+ * ch_terma call was parallel to the isocentre plane (the reference raises
+ * RuntimeError there, conehead/conehead.py:34-35). */
+ch_status ch_terma_check(void* stream);
+
+/* Opens a centred square of side `length` cm (sets it to 1, leaves the
+ * rest of the raster unchanged), exactly as Block.set_square
+ * (conehead/block.py:64-85). */
+ch_status ch_raster_square(float* d_raster, double length, void* stream);
+
+/* Multi-leaf collimator raster built on the device from a leaf table, all
+ * positions in integer tenths of a millimetre (floor(mm * 10), as
+ * conehead/block.py:117-120 converts them): n_pairs leaf pairs with
+ * boundaries h_bound[n_pairs+1] (across the leaves), bank A / B tip
+ * positions h_a, h_b (along leaf travel) and jaws h_jaw_x[2], h_jaw_y[2].
+ * Leaf edge transmission (0.2/0.5/0.75) and the 15-column tip taper follow
+ * conehead/block.py:130-196; the bank is centred on raster row 2000 and rows
+ * outside it are blocked.  Replaces Block._set_from_plan's raster. */
+ch_status ch_raster_mlc(float* d_raster, int32_t n_pairs,
+                        const int32_t* h_bound, const int32_t* h_a,
+                        const int32_t* h_b, const int32_t h_jaw_x[2],
+                        const int32_t h_jaw_y[2], void* stream);
+
+/* Kernel time of the last gather launch issued by this thread on `stream`,
+ * measured with HIP events around the launch when timing is enabled. */
+ch_status ch_set_timing(int32_t enabled);
+ch_status ch_last_kernel_ms(float* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CONEHEAD_AMD_H */

@@ -1,0 +1,132 @@
+"""The FP64 oracle's TERMA chain (oracle/ch_ref.c, ch_ref_terma) pinned by
+closed forms: radiological depth of water equals the geometric path to the
+grid exit, inverse-square fluence on the central axis, square-field blocked
+fractions in multiples of 1/27, and the spectrum sum at the axis."""
+import numpy as np
+import pytest
+
+from conehead_amd import engine as eng_mod
+from conehead_amd.block import Block
+from conehead_amd.source import Source
+from conehead_amd.workloads import SETTINGS
+
+
+def run(oracle, shape, spacing, origin, dens, gantry=0.0, side=10.0):
+    src = Source("varian_clinac_6MV")
+    if gantry:
+        src.gantry = gantry
+    blk = Block()
+    blk.set_square(side)
+    e, w, mu = eng_mod.spectrum(SETTINGS)
+    beam = oracle.make_beam(src.position, src.v_y, src.transform, src.sad,
+                            SETTINGS, e, w, mu)
+    out = oracle.terma(beam, shape, origin, spacing, dens, blk.block_values)
+    return out, src, (e, w, mu)
+
+
+def exit_distance(p, d, lo, hi):
+    """Distance from p along unit d to the box [lo, hi]."""
+    t = np.inf
+    for a in range(3):
+        if d[a] > 0:
+            t = min(t, (hi[a] - p[a]) / d[a])
+        elif d[a] < 0:
+            t = min(t, (lo[a] - p[a]) / d[a])
+    return t
+
+
+@pytest.fixture(scope="module")
+def water(oracle):
+    n, s = 31, 0.4
+    shape = (n, n, n)
+    origin = np.array([-(n - 1) / 2 * s, s / 2, -(n - 1) / 2 * s])
+    dens = np.ones(shape, np.float32)
+    out, src, spec = run(oracle, shape, [s] * 3, origin, dens)
+    return dict(out=out, src=src, spec=spec, shape=shape, s=s,
+                origin=origin)
+
+
+def test_water_depth_is_geometric_exit(water):
+    s, origin, shape = water["s"], water["origin"], water["shape"]
+    src = np.asarray(water["src"].position, np.float64)
+    lo = origin - s / 2
+    hi = origin + s * (np.array(shape) - 0.5)
+    for idx in [(15, 0, 15), (15, 20, 15), (3, 12, 17), (0, 5, 30)]:
+        p = origin + s * np.array(idx)
+        d = (src - p) / np.linalg.norm(src - p)
+        expect = exit_distance(p, d, lo, hi)
+        assert water["out"]["d_eff"][idx] == pytest.approx(expect, rel=1e-12)
+
+
+def test_central_axis_fluence_inverse_square(water):
+    st = SETTINGS
+    sad = 100.0
+    for j in [0, 7, 20]:
+        idx = (15, j, 15)
+        y = water["origin"][1] + water["s"] * j
+        r = sad + y
+        r_e = 2.0 * st["zExp"] / sad               # OAD clamped at 2 cm
+        expect = (st["sPri"] * (sad / r) ** 2 +
+                  st["sExp"] / r_e * np.exp(-st["kExp"] * r_e) *
+                  (sad - st["zExp"]) ** 2 / (r - st["zExp"]) ** 2)
+        assert water["out"]["oad"][idx] == pytest.approx(0.0, abs=1e-12)
+        assert water["out"]["blocked"][idx] == pytest.approx(1.0, rel=1e-14)
+        assert water["out"]["fluence"][idx] == pytest.approx(expect,
+                                                             rel=1e-12)
+
+
+def test_square_field_blocked_fractions(water):
+    b = water["out"]["blocked"]
+    frac = b * 27.0
+    assert np.allclose(frac, np.round(frac), atol=1e-9)
+    assert b[15, 0, 15] == pytest.approx(1.0) and b[0, 0, 0] == 0.0
+    # 10 x 10 cm at the isocentre: voxel centres |x| < 5 cm are open near
+    # the surface, |x| > 5.4 cm blocked
+    xs = water["origin"][0] + water["s"] * np.arange(31)
+    inside = np.abs(xs) < 4.7
+    outside = np.abs(xs) > 5.5
+    assert np.allclose(b[inside, 0, 15], 1.0, rtol=1e-14)
+    assert np.all(b[outside, 0, 15] == 0.0)
+
+
+def test_terma_spectrum_sum_on_axis(water):
+    e, w, mu = water["spec"]
+    for j in [0, 10, 20]:
+        idx = (15, j, 15)
+        d = water["out"]["d_eff"][idx]
+        flu = water["out"]["fluence"][idx]
+        expect = flu * np.sum(w * np.exp(-mu * d) * e * mu)   # fs = fh = 1
+        assert water["out"]["terma"][idx] == pytest.approx(expect, rel=1e-12)
+
+
+def test_heterogeneous_depth_integral(oracle):
+    """A lung slab halves nothing but its own path: the radiological depth
+    along the central axis is the density-weighted path length."""
+    n, s = 21, 0.5
+    shape = (n, n, n)
+    origin = np.array([-(n - 1) / 2 * s, s / 2, -(n - 1) / 2 * s])
+    dens = np.ones(shape, np.float32)
+    dens[:, 4:8, :] = np.float32(0.25)
+    out, _, _ = run(oracle, shape, [s] * 3, origin, dens)
+    # axis voxel j: path from its centre back to y = 0 through voxels j..0
+    for j in [2, 6, 12]:
+        rho = dens[10, :j + 1, 10].astype(np.float64)
+        seg = np.full(j + 1, s)
+        seg[0] = s / 2
+        expect = np.sum(rho[::-1] * seg)
+        assert out["d_eff"][10, j, 10] == pytest.approx(expect, rel=1e-12)
+
+
+def test_rotated_gantry_is_symmetric(oracle):
+    """Gantry 90 on a cube centred at the isocentre gives the gantry-0 TERMA
+    rotated by 90 degrees about z."""
+    n, s = 15, 0.5
+    shape = (n, n, n)
+    origin = np.array([-(n - 1) / 2 * s] * 3)
+    dens = np.ones(shape, np.float32)
+    a, _, _ = run(oracle, shape, [s] * 3, origin, dens, gantry=0.0, side=4)
+    b, _, _ = run(oracle, shape, [s] * 3, origin, dens, gantry=90.0, side=4)
+    # gantry 90 puts the source at +x: beam along -x; x' = y, y' = -x
+    rot = np.transpose(a["terma"], (1, 0, 2))[::-1, :, :]
+    np.testing.assert_allclose(b["terma"], rot, rtol=1e-9,
+                               atol=1e-12 * a["terma"].max())
