@@ -1,11 +1,12 @@
-"""Tile-shape autotuner for the collapsed-cone gather.
+"""Row-length and accumulation autotuner for the collapsed-cone gather.
 
-A wavefront owns TX x TY x TZ = 64 output voxels.  Long z-runs give wide
-coalesced T loads; compact tiles shrink the per-lane spread of the clipped
-tap ranges.  The best shape depends on the grid, the cone set and the TERMA
-support, so it is measured: every candidate is timed with HIP events on the
-real plan and a synthetic TERMA of the given support, and the winner is
-cached per plan key in memory and in a JSON file.
+A wavefront owns one output row of 64 or 128 z-voxels (one or two per lane).
+Longer rows halve the per-cone set-up per output and load 8 bytes per lane
+per tap, but widen the union of the lanes' clipped tap ranges (taps that
+read zeros for part of the row).  The best choice depends on the grid, the
+cone set and the TERMA support, so it is measured: every candidate is timed
+with HIP events on the real plan and a synthetic TERMA of the given support,
+and the winner is cached per plan key in memory and in a JSON file.
 """
 import json
 import os
@@ -16,8 +17,7 @@ import numpy as np
 from . import _lib
 from .device import DeviceArray, Event, Stream
 
-CANDIDATES = [(1, 1, 64), (1, 2, 32), (1, 4, 16), (2, 2, 16), (1, 8, 8),
-              (2, 4, 8), (4, 4, 4), (2, 8, 4), (4, 8, 2), (8, 8, 1)]
+CANDIDATES = [(1, 1, 64), (1, 2, 32), (1, 4, 16), (1, 8, 8)]
 
 CACHE_PATH = os.environ.get(
     "CONEHEAD_AMD_TUNE_CACHE",
@@ -29,19 +29,12 @@ _memo = {}
 
 
 def default_tile(plan):
-    """Heuristic tile before (or without) tuning: a full z-run per wave when
-    the grid is at least 64 deep, otherwise fold the rest into y and x."""
+    """Heuristic row length before (or without) tuning: two outputs per
+    lane (128-voxel rows) when the grid is deeper than 64, else one."""
     key = _shape_key(plan)
     if key in _memo:
         return _memo[key]
-    Z = plan.shape[2]
-    if Z >= 64:
-        return (1, 1, 64)
-    if Z >= 32:
-        return (1, 2, 32)
-    if Z >= 16:
-        return (1, 4, 16)
-    return (2, 4, 8)
+    return (1, 2, 32)
 
 
 def _shape_key(plan):
@@ -100,8 +93,6 @@ def tune(plan, box=None, candidates=CANDIDATES, reps=3, stream=None,
     e0, e1 = Event(), Event()
     times = {}
     for tile in candidates:
-        if tile[0] > shape[0] * 2 and tile[1] > 1:
-            continue
         plan.set_tile(*tile)
         convolve(plan, d_t, d_d, d_box, stream=s)  # warm-up
         e0.record(s)

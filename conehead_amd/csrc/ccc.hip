@@ -1,12 +1,19 @@
 // ccc.hip -- collapsed-cone convolution as a deterministic stencil gather.
 //
-// dose(v) = sum_c sum_m k_{c,m} * T(v - o_{c,m}), over sources v - o inside
-// the grid and inside the TERMA support box (T is zero outside it).  One
-// wavefront owns a tile of 64 output voxels (TX x TY x TZ, z fastest); for
-// each cone it clips every lane's tap range with the per-axis first-step
-// tables (the path is monotone per axis, so the taps whose source lies in
-// the box form one contiguous range), then walks the union of the ranges
-// with wave-uniform tap loads and per-lane predicated, coalesced T loads.
+// dose(v) = sum_c sum_m k_{c,m} * T(v - o_{c,m}) over sources v - o inside
+// the grid (T is zero outside its support box, so sources are clipped to
+// the box).  The gather reads a padded copy of T: every z-row carries kZPad
+// zeros on both ends and every x-plane kYPad zero rows on both sides.
+//
+// One wavefront owns a 1 x TY x TZ tile of outputs (64 lanes, z fastest).
+// For a cone, x is uniform, so the box clips the tap list to one range from
+// the per-axis first-step tables; along y and z the per-voxel ranges shift
+// monotonically, and the union over the tile is taken from its end points.
+// Every tap of the union is applied to every lane without a predicate: a
+// lane whose source falls outside the box reads a zero (inside the grid, or
+// in the padding, which is at least one tile wide).  Tap offsets and
+// increments are wave-uniform scalar loads; T loads are TY runs of TZ
+// contiguous floats.
 // No atomics: every output is written once, in a fixed order, so results
 // are bitwise reproducible alone or inside any batch.
 #include <algorithm>
@@ -31,22 +38,8 @@ __device__ __forceinline__ auto as_const(const T* p)
     return (__attribute__((address_space(4))) const T*)(p);
 }
 
-// Field-wise copies out of the constant address space (a struct copy
+// Field-wise copy out of the constant address space (a struct copy
 // constructor cannot bind to an address_space(4) object).
-__device__ __forceinline__ ConeMeta load_meta(const c_meta* p)
-{
-    ConeMeta m;
-    m.tap_begin = p->tap_begin;
-    m.n_taps = p->n_taps;
-    for (int a = 0; a < 3; ++a) {
-        m.f_begin[a] = p->f_begin[a];
-        m.f_len[a] = p->f_len[a];
-        m.step[a] = p->step[a];
-    }
-    m.pad = 0;
-    return m;
-}
-
 __device__ __forceinline__ ConvJob load_job(const c_job* p)
 {
     ConvJob j;
@@ -62,28 +55,10 @@ __device__ __forceinline__ ConvJob load_job(const c_job* p)
     j.Z = p->Z;
     j.n_cones = p->n_cones;
     j.accumulate = p->accumulate;
-    j.ntx = p->ntx;
     j.nty = p->nty;
     j.ntz = p->ntz;
     j.tile_begin = p->tile_begin;
-    j.pad = 0;
     return j;
-}
-
-__device__ __forceinline__ int wave_min(int v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1)
-        v = min(v, __shfl_xor(v, o, 64));
-    return v;
-}
-
-__device__ __forceinline__ int wave_max(int v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1)
-        v = max(v, __shfl_xor(v, o, 64));
-    return v;
 }
 
 __device__ __forceinline__ void kahan_add(float& s, float& c, float x)
@@ -94,10 +69,13 @@ __device__ __forceinline__ void kahan_add(float& s, float& c, float x)
     s = t;
 }
 
-// Tap range [mlo, mhi) of one axis for coordinate v, box [lo, hi].
-__device__ __forceinline__ void axis_range(int v, int lo, int hi, int step,
-                                           const c_i32* f, int f_len,
-                                           int& mlo, int& mhi)
+// Tap range [a, b) of one axis for a uniform coordinate v and box [lo, hi]:
+// taps whose source coordinate v - o lies in the box.  Out-of-range
+// arguments clamp to the ends of the table, which yields empty ranges and
+// keeps a and b monotone in v.
+__device__ __forceinline__ void urange(int v, int lo, int hi, int step,
+                                       const c_i32* f, int f_len, int& a,
+                                       int& b)
 {
     int L, U;
     if (step > 0) {
@@ -107,87 +85,60 @@ __device__ __forceinline__ void axis_range(int v, int lo, int hi, int step,
         L = lo - v;
         U = hi - v;
     }
-    const int last = f_len - 1;  // index of the sentinel (= n_taps)
-    const int A = L > 0 ? L : 0;
-    if (U < 0 || A >= last) {
-        mhi = 0;
-        return;
-    }
-    const int Ub = U + 1 < last ? U + 1 : last;
-    mlo = max(mlo, f[A]);
-    mhi = min(mhi, f[Ub]);
+    const int last = f_len - 1;  // sentinel entry, = n_taps
+    L = L < 0 ? 0 : (L > last ? last : L);
+    U = U + 1 < 0 ? 0 : (U + 1 > last ? last : U + 1);
+    a = f[L];
+    b = f[U];
 }
 
-// Taps per unrolled step; every cone's tap list is padded with kUnroll
-// zero taps (offset 0, increment 0) so a chunk may read past its end.
-constexpr int kUnroll = 8;
+constexpr int kUnroll = 8;  // taps per step; tap lists carry kTapPad zeros
 
-// Accumulates taps [t0, t1) of one cone into (acc, ac).  PRED selects the
-// per-lane predicate: 0 none (every lane active; t1 - t0 is a multiple of
-// kUnroll), 1 t >= mlo && t < lim, 2 t < mhi.  Inactive lanes read their own voxel and add 0, so the loads
-// of a chunk are unconditional and all in flight together.
-template <int ACC, int PRED>
-__device__ __forceinline__ void tap_run(const char* __restrict__ Tb,
-                                        uint32_t base4, const c_i32* ml,
-                                        const c_f32* mk, int t0, int t1,
-                                        int mlo, int mhi, int lim,
-                                        float& acc, float& ac)
+// Sum of one chunk of kUnroll taps into a cone line's accumulator:
+//   ACC 0  Kahan-compensated per tap,
+//   ACC 1  plain FMA along the line (the line total is compensated),
+//   ACC 2  plain sum of the chunk, Kahan-compensated across chunks.
+template <int ACC>
+__device__ __forceinline__ void accumulate_chunk(float& acc, float& ac,
+                                                 const float* kc,
+                                                 const float* v)
 {
-    if (t0 >= t1)
-        return;
-    int o[kUnroll];
-    float kv[kUnroll];
+    if (ACC == 0) {
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
-        o[u] = ml[t0 + u];
-        kv[u] = mk[t0 + u];
-    }
-    for (int t = t0; t < t1; t += kUnroll) {
-        float v[kUnroll], kc[kUnroll];
+        for (int u = 0; u < kUnroll; ++u)
+            kahan_add(acc, ac, kc[u] * v[u]);
+    } else if (ACC == 1) {
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u) {
-            bool act = true;
-            if (PRED == 1)
-                act = (t + u >= mlo) && (t + u < lim);
-            else if (PRED == 2)
-                act = t + u < mhi;
-            const uint32_t off = act ? base4 - (uint32_t)o[u] : base4;
-            v[u] = *reinterpret_cast<const float*>(Tb + off);
-            kc[u] = act ? kv[u] : 0.0f;
-        }
-        // Prefetch the next chunk's taps while the T loads are in flight;
-        // the padding keeps t + kUnroll .. t + 2 kUnroll - 1 readable.
-        if (t + kUnroll < t1) {
+        for (int u = 0; u < kUnroll; ++u)
+            acc = fmaf(kc[u], v[u], acc);
+    } else {
+        float part = kc[0] * v[0];
 #pragma unroll
-            for (int u = 0; u < kUnroll; ++u) {
-                o[u] = ml[t + kUnroll + u];
-                kv[u] = mk[t + kUnroll + u];
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < kUnroll; ++u) {
-            if (ACC == 0)
-                kahan_add(acc, ac, kc[u] * v[u]);
-            else
-                acc = fmaf(kc[u], v[u], acc);
-        }
+        for (int u = 1; u < kUnroll; ++u)
+            part = fmaf(kc[u], v[u], part);
+        kahan_add(acc, ac, part);
     }
 }
 
-template <int TX, int TY, int TZ, int ACC, typename OutT>
+// One wavefront owns a 1 x TY x TZ output tile (TY * TZ = 64 lanes, z
+// fastest).  Returns the tile's T reads relative to the padded allocation
+// through base4 (bytes); every tap applies to every lane.
+template <int TY, int TZ, int ACC, typename OutT>
 __device__ __forceinline__ void gather_tile(const ConvJob& j, int tile,
                                             int lane)
 {
-    static_assert(TX * TY * TZ == 64, "one wavefront per tile");
+    static_assert(TY * TZ == 64 && TY <= kYPad && TZ <= kZPad, "tile");
     const int tz = tile % j.ntz;
     const int r = tile / j.ntz;
     const int ty = r % j.nty;
-    const int tx = r / j.nty;
-    const int lz = lane % TZ;
-    const int ly = (lane / TZ) % TY;
-    const int lx = lane / (TZ * TY);
-    const int x = tx * TX + lx, y = ty * TY + ly, z = tz * TZ + lz;
-    const bool valid = x < j.X && y < j.Y && z < j.Z;
+    const int x = r / j.nty;
+    const int y0 = ty * TY, z0 = tz * TZ;
+    const int yl = min(y0 + TY - 1, j.Y - 1);  // last valid y of the tile
+    const int zl = min(z0 + TZ - 1, j.Z - 1);  // last valid z of the tile
+    const int y = y0 + lane / TZ;
+    const int z = z0 + lane % TZ;
+    const int Zp = j.Z + 2 * kZPad;
+    const int Yp = j.Y + 2 * kYPad;
 
     int lo0 = 0, lo1 = 0, lo2 = 0;
     int hi0 = j.X - 1, hi1 = j.Y - 1, hi2 = j.Z - 1;
@@ -200,80 +151,107 @@ __device__ __forceinline__ void gather_tile(const ConvJob& j, int tile,
         hi1 = min(hi1, b[4]);
         hi2 = min(hi2, b[5]);
     }
-    const bool any = lo0 <= hi0 && lo1 <= hi1 && lo2 <= hi2;
-    const int base = valid ? (x * j.Y + y) * j.Z + z : 0;
-    const uint32_t base4 = (uint32_t)base * 4u;
-    const char* __restrict__ Tb = reinterpret_cast<const char*>(j.terma);
+    // byte offset of this lane's own voxel in the padded T allocation
+    const uint32_t base4 =
+        (uint32_t)(((x * Yp + y + kYPad) * Zp + z + kZPad) * 4);
+    const char* __restrict__ Tb = reinterpret_cast<const char*>(j.terma) -
+                                  (size_t)(kYPad * Zp + kZPad) * 4;
     const c_meta* meta = as_const(j.meta);
     const c_i32* ftab = as_const(j.ftab);
     const c_i32* lin = as_const(j.lin);
     const c_f32* kk = as_const(j.k);
 
     float sum = 0.0f, comp = 0.0f;
-    if (any) {
+    if (lo0 <= hi0 && lo1 <= hi1 && lo2 <= hi2) {
         for (int c = 0; c < j.n_cones; ++c) {
-            const ConeMeta m = load_meta(meta + c);
-            int mlo = 0, mhi = m.n_taps;
-            axis_range(x, lo0, hi0, m.step[0], ftab + m.f_begin[0],
-                       m.f_len[0], mlo, mhi);
-            axis_range(y, lo1, hi1, m.step[1], ftab + m.f_begin[1],
-                       m.f_len[1], mlo, mhi);
-            axis_range(z, lo2, hi2, m.step[2], ftab + m.f_begin[2],
-                       m.f_len[2], mlo, mhi);
-            const bool part = valid && mhi > mlo;
-            // union [ulo, uhi) and common part [blo, bhi) over the lanes
-            // that have taps
-            const int ulo = __builtin_amdgcn_readfirstlane(
-                wave_min(part ? mlo : INT_MAX));
-            const int uhi = __builtin_amdgcn_readfirstlane(
-                wave_max(part ? mhi : INT_MIN));
-            const int blo = __builtin_amdgcn_readfirstlane(
-                wave_max(part ? mlo : INT_MIN));
-            const int bhi = __builtin_amdgcn_readfirstlane(
-                wave_min(part ? mhi : INT_MAX));
+            const c_meta* m = meta + c;
+            int ax, bx, ay1, by1, ay2, by2, az1, bz1, az2, bz2;
+            urange(x, lo0, hi0, m->step[0], ftab + m->f_begin[0],
+                   m->f_len[0], ax, bx);
+            urange(y0, lo1, hi1, m->step[1], ftab + m->f_begin[1],
+                   m->f_len[1], ay1, by1);
+            urange(z0, lo2, hi2, m->step[2], ftab + m->f_begin[2],
+                   m->f_len[2], az1, bz1);
+            if (TY > 1)
+                urange(yl, lo1, hi1, m->step[1], ftab + m->f_begin[1],
+                       m->f_len[1], ay2, by2);
+            else {
+                ay2 = ay1;
+                by2 = by1;
+            }
+            urange(zl, lo2, hi2, m->step[2], ftab + m->f_begin[2],
+                   m->f_len[2], az2, bz2);
+            // union of the tile's per-voxel ranges (each axis is monotone,
+            // so its extremes sit at the tile's end points)
+            const int ulo = max(max(ax, min(ay1, ay2)), min(az1, az2));
+            const int uhi = min(min(bx, max(by1, by2)), max(bz1, bz2));
             if (ulo >= uhi)
-                continue;  // no lane of this tile has a source in range
-            const c_i32* ml = lin + m.tap_begin;
-            const c_f32* mk = kk + m.tap_begin;
+                continue;
+            const c_i32* ml = lin + m->tap_begin;
+            const c_f32* mk = kk + m->tap_begin;
             float acc = 0.0f, ac = 0.0f;
-            if (part) {
-                if (blo < bhi) {
-                    // head: lanes start at their own mlo
-                    tap_run<ACC, 1>(Tb, base4, ml, mk, ulo, blo, mlo, mhi,
-                                    blo, acc, ac);
-                    // body: every participating lane is active
-                    const int nb = (bhi - blo) / kUnroll * kUnroll;
-                    tap_run<ACC, 0>(Tb, base4, ml, mk, blo, blo + nb, mlo,
-                                    mhi, blo + nb, acc, ac);
-                    // tail: rest of the body and lanes ending at their mhi
-                    tap_run<ACC, 2>(Tb, base4, ml, mk, blo + nb, uhi, mlo,
-                                    mhi, uhi, acc, ac);
-                } else {
-                    tap_run<ACC, 1>(Tb, base4, ml, mk, ulo, uhi, mlo, mhi,
-                                    mhi, acc, ac);
+            int o[kUnroll];
+            float kv[kUnroll];
+#pragma unroll
+            for (int u = 0; u < kUnroll; ++u) {
+                o[u] = ml[ulo + u];
+                kv[u] = mk[ulo + u];
+            }
+            // whole chunks: no per-tap condition at all
+            const int tend = ulo + (uhi - ulo) / kUnroll * kUnroll;
+            int t = ulo;
+            for (; t < tend; t += kUnroll) {
+                float v[kUnroll], kc[kUnroll];
+#pragma unroll
+                for (int u = 0; u < kUnroll; ++u) {
+                    v[u] = *reinterpret_cast<const float*>(
+                        Tb + (base4 - (uint32_t)o[u]));
+                    kc[u] = kv[u];
                 }
+                // next chunk's taps while the loads are in flight (the
+                // padding keeps t + 2 kUnroll - 1 readable)
+#pragma unroll
+                for (int u = 0; u < kUnroll; ++u) {
+                    o[u] = ml[t + kUnroll + u];
+                    kv[u] = mk[t + kUnroll + u];
+                }
+                accumulate_chunk<ACC>(acc, ac, kc, v);
+            }
+            // last partial chunk: taps past uhi are real taps of the cone
+            // and are masked (uniformly) to offset 0, increment 0
+            if (t < uhi) {
+                float v[kUnroll], kc[kUnroll];
+#pragma unroll
+                for (int u = 0; u < kUnroll; ++u) {
+                    const bool in = t + u < uhi;
+                    v[u] = *reinterpret_cast<const float*>(
+                        Tb + (base4 - (uint32_t)(in ? o[u] : 0)));
+                    kc[u] = in ? kv[u] : 0.0f;
+                }
+                accumulate_chunk<ACC>(acc, ac, kc, v);
             }
             kahan_add(sum, comp, acc);
         }
     }
-    if (valid) {
-        OutT* d = reinterpret_cast<OutT*>(j.dose);
-        const OutT prev = j.accumulate ? d[base] : (OutT)0;
-        d[base] = prev + (OutT)sum;
+    if (y <= yl && z <= zl) {
+        OutT* d = reinterpret_cast<OutT*>(j.dose) +
+                  ((size_t)x * j.Y + y) * j.Z + z;
+        const OutT prev = j.accumulate ? *d : (OutT)0;
+        *d = prev + (OutT)sum;
     }
 }
 
-template <int TX, int TY, int TZ, int ACC, typename OutT>
+template <int TY, int TZ, int ACC, typename OutT>
 __global__ __launch_bounds__(256) void gather_single(ConvJob j, int n_tiles)
 {
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int tile = blockIdx.x * 4 + wave;
     if (tile >= n_tiles)
         return;
-    gather_tile<TX, TY, TZ, ACC, OutT>(j, tile, threadIdx.x & 63);
+    gather_tile<TY, TZ, ACC, OutT>(j, tile, threadIdx.x & 63);
 }
 
-template <int TX, int TY, int TZ, int ACC, typename OutT>
+template <int TY, int TZ, int ACC, typename OutT>
 __global__ __launch_bounds__(256) void gather_batch(const ConvJob* jobs,
                                                     int n_jobs, int n_tiles)
 {
@@ -291,8 +269,26 @@ __global__ __launch_bounds__(256) void gather_batch(const ConvJob* jobs,
             hi = mid - 1;
     }
     const ConvJob jb = load_job(J + lo);
-    gather_tile<TX, TY, TZ, ACC, OutT>(jb, tile - jb.tile_begin,
-                                       threadIdx.x & 63);
+    gather_tile<TY, TZ, ACC, OutT>(jb, tile - jb.tile_begin,
+                                   threadIdx.x & 63);
+}
+
+// T [X][Y][Z] -> padded [X][Y + 2 kYPad][Z + 2 kZPad] with zero borders.
+__global__ __launch_bounds__(256) void pad_kernel(const float* __restrict__ T,
+                                                  float* __restrict__ P,
+                                                  int X, int Y, int Z)
+{
+    const int Zp = Z + 2 * kZPad, Yp = Y + 2 * kYPad;
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= (int64_t)X * Yp * Zp)
+        return;
+    const int zz = (int)(i % Zp) - kZPad;
+    const int64_t r = i / Zp;
+    const int yy = (int)(r % Yp) - kYPad;
+    const int64_t x = r / Yp;
+    P[i] = (zz >= 0 && zz < Z && yy >= 0 && yy < Y)
+               ? T[(x * Y + yy) * Z + zz]
+               : 0.0f;
 }
 
 // ----------------------------------------------------------- dispatch ----
@@ -303,43 +299,36 @@ struct Kernels {
     void (*batch)(const ConvJob*, int, int);
 };
 
-#define CH_TILE_LIST(X)                                                     \
-    X(1, 1, 64) X(1, 2, 32) X(1, 4, 16) X(2, 2, 16) X(1, 8, 8) X(2, 4, 8)   \
-    X(4, 4, 4) X(2, 8, 4) X(4, 8, 2) X(8, 8, 1)
-
-template <int ACC, typename OutT>
-static bool pick(int tx, int ty, int tz, Kernels<OutT>& k)
-{
-#define CH_CASE(a, b, c)                                                    \
-    if (tx == a && ty == b && tz == c) {                                    \
-        k.single = gather_single<a, b, c, ACC, OutT>;                       \
-        k.batch = gather_batch<a, b, c, ACC, OutT>;                         \
-        return true;                                                        \
-    }
-    CH_TILE_LIST(CH_CASE)
-#undef CH_CASE
-    return false;
-}
-
 template <typename OutT>
 static ch_status kernels_for(const ch_plan* p, Kernels<OutT>& k)
 {
-    bool ok = p->accum_mode == 0
-                  ? pick<0, OutT>(p->tile[0], p->tile[1], p->tile[2], k)
-                  : pick<1, OutT>(p->tile[0], p->tile[1], p->tile[2], k);
-    if (!ok) {
-        set_error("no gather kernel for tile %dx%dx%d", p->tile[0],
-                  p->tile[1], p->tile[2]);
-        return CH_ERR_ARG;
+    const int ty = p->tile[1], tz = p->tile[2];
+#define CH_PICK(TY, TZ, A)                                                  \
+    if (ty == TY && tz == TZ && p->accum_mode == A) {                       \
+        k.single = gather_single<TY, TZ, A, OutT>;                          \
+        k.batch = gather_batch<TY, TZ, A, OutT>;                            \
+        return CH_OK;                                                       \
     }
-    return CH_OK;
+#define CH_PICK3(TY, TZ) CH_PICK(TY, TZ, 0) CH_PICK(TY, TZ, 1) CH_PICK(TY, TZ, 2)
+    CH_PICK3(1, 64)
+    CH_PICK3(2, 32)
+    CH_PICK3(4, 16)
+    CH_PICK3(8, 8)
+#undef CH_PICK3
+#undef CH_PICK
+    set_error("no gather kernel for tile 1x%dx%d, accumulation %d", ty, tz,
+              p->accum_mode);
+    return CH_ERR_ARG;
 }
 
-static ConvJob make_job(const ch_plan* p, const float* terma, void* dose,
+// `padded` is the start of the padded allocation; the job points at its
+// interior voxel (0, 0, 0).
+static ConvJob make_job(const ch_plan* p, const float* padded, void* dose,
                         const int32_t* box, int32_t accumulate)
 {
+    const int Zp = p->shape[2] + 2 * kZPad;
     ConvJob j;
-    j.terma = terma;
+    j.terma = padded + (size_t)kYPad * Zp + kZPad;
     j.dose = dose;
     j.box = box;
     j.lin = p->d_lin;
@@ -351,12 +340,21 @@ static ConvJob make_job(const ch_plan* p, const float* terma, void* dose,
     j.Z = p->shape[2];
     j.n_cones = p->n_cones;
     j.accumulate = accumulate;
-    j.ntx = (j.X + p->tile[0] - 1) / p->tile[0];
     j.nty = (j.Y + p->tile[1] - 1) / p->tile[1];
     j.ntz = (j.Z + p->tile[2] - 1) / p->tile[2];
     j.tile_begin = 0;
-    j.pad = 0;
     return j;
+}
+
+static int64_t job_tiles(const ConvJob& j)
+{
+    return (int64_t)j.X * j.nty * j.ntz;
+}
+
+static size_t padded_elems(const ch_plan* p)
+{
+    return (size_t)p->shape[0] * (p->shape[1] + 2 * kYPad) *
+           (p->shape[2] + 2 * kZPad);
 }
 
 // Optional HIP-event timing of the gather launches (bench and profiling).
@@ -386,6 +384,79 @@ struct LaunchTimer {
     }
 };
 
+// Per-stream workspaces: the padded TERMA copies and the batch job table.
+// Kernels on one stream run in order, so a stream's workspace is reused by
+// its next call without further synchronisation; the pinned job-table
+// staging buffer is reused only after the event of its last copy.
+struct Workspace {
+    hipStream_t stream;
+    float* pad;
+    size_t pad_elems;
+    ConvJob* d_jobs;
+    ConvJob* h_jobs;
+    int job_cap;
+    hipEvent_t jobs_done;
+};
+static std::vector<Workspace*> g_ws;
+static std::mutex g_ws_mu;
+
+static ch_status workspace(hipStream_t s, size_t pad_elems, int n_jobs,
+                           Workspace** out)
+{
+    Workspace* w = nullptr;
+    for (auto* x : g_ws)
+        if (x->stream == s)
+            w = x;
+    if (!w) {
+        w = new Workspace();
+        w->stream = s;
+        w->pad = nullptr;
+        w->pad_elems = 0;
+        w->d_jobs = nullptr;
+        w->h_jobs = nullptr;
+        w->job_cap = 0;
+        CH_HIP(hipEventCreateWithFlags(&w->jobs_done, hipEventDisableTiming));
+        g_ws.push_back(w);
+    }
+    if (w->pad_elems < pad_elems) {
+        // the old buffer may still be read by queued kernels of this stream
+        CH_HIP(hipStreamSynchronize(s));
+        CH_HIP(hipFree(w->pad));
+        w->pad = nullptr;
+        w->pad_elems = 0;
+        CH_HIP(hipMalloc((void**)&w->pad, pad_elems * sizeof(float)));
+        w->pad_elems = pad_elems;
+    }
+    if (n_jobs > 0) {
+        CH_HIP(hipEventSynchronize(w->jobs_done));
+        if (w->job_cap < n_jobs) {
+            CH_HIP(hipStreamSynchronize(s));
+            CH_HIP(hipFree(w->d_jobs));
+            CH_HIP(hipHostFree(w->h_jobs));
+            w->d_jobs = nullptr;
+            w->h_jobs = nullptr;
+            const int cap = std::max(n_jobs, std::max(64, 2 * w->job_cap));
+            CH_HIP(hipMalloc((void**)&w->d_jobs, cap * sizeof(ConvJob)));
+            CH_HIP(hipHostMalloc((void**)&w->h_jobs, cap * sizeof(ConvJob),
+                                 hipHostMallocDefault));
+            w->job_cap = cap;
+        }
+    }
+    *out = w;
+    return CH_OK;
+}
+
+static ch_status pad_into(const float* T, float* P, const ch_plan* p,
+                          hipStream_t s)
+{
+    const int64_t n = (int64_t)padded_elems(p);
+    hipLaunchKernelGGL(pad_kernel, dim3((unsigned)((n + 255) / 256)),
+                       dim3(256), 0, s, T, P, p->shape[0], p->shape[1],
+                       p->shape[2]);
+    CH_HIP(hipGetLastError());
+    return CH_OK;
+}
+
 template <typename OutT>
 static ch_status convolve_one(const ch_plan* p, const float* d_terma,
                               OutT* d_dose, const int32_t* d_box,
@@ -396,64 +467,44 @@ static ch_status convolve_one(const ch_plan* p, const float* d_terma,
     ch_status st = kernels_for<OutT>(p, k);
     if (st != CH_OK)
         return st;
-    ConvJob j = make_job(p, d_terma, d_dose, d_box, accumulate);
-    const int64_t n_tiles = (int64_t)j.ntx * j.nty * j.ntz;
+    hipStream_t s = (hipStream_t)stream;
+    std::lock_guard<std::mutex> lock(g_ws_mu);
+    Workspace* w = nullptr;
+    st = workspace(s, padded_elems(p), 0, &w);
+    if (st != CH_OK)
+        return st;
+    st = pad_into(d_terma, w->pad, p, s);
+    if (st != CH_OK)
+        return st;
+    ConvJob j = make_job(p, w->pad, d_dose, d_box, accumulate);
+    const int64_t n_tiles = job_tiles(j);
     CH_REQUIRE(n_tiles < INT_MAX / 2, "too many tiles");
     const unsigned blocks = (unsigned)((n_tiles + 3) / 4);
-    LaunchTimer timer((hipStream_t)stream);
-    hipLaunchKernelGGL(k.single, dim3(blocks), dim3(256), 0,
-                       (hipStream_t)stream, j, (int)n_tiles);
+    LaunchTimer timer(s);
+    hipLaunchKernelGGL(k.single, dim3(blocks), dim3(256), 0, s, j,
+                       (int)n_tiles);
     CH_HIP(hipGetLastError());
     timer.stop();
     return CH_OK;
 }
 
-// Per-stream device job tables for batched launches.
-struct JobTable {
-    hipStream_t stream;
-    ConvJob* d_jobs;
-    ConvJob* h_jobs;  // pinned
-    int cap;
-    hipEvent_t done;
-};
-static std::vector<JobTable> g_tables;
-static std::mutex g_tables_mu;
+// ------------------------------------------------------- support box ----
 
-static ch_status job_table(hipStream_t s, int n, JobTable** out)
+__device__ __forceinline__ int wave_min(int v)
 {
-    for (auto& t : g_tables)
-        if (t.stream == s) {
-            if (t.cap >= n) {
-                // The previous batch on this stream must have consumed the
-                // pinned staging copy before it is overwritten.
-                CH_HIP(hipEventSynchronize(t.done));
-                *out = &t;
-                return CH_OK;
-            }
-            CH_HIP(hipEventSynchronize(t.done));
-            CH_HIP(hipFree(t.d_jobs));
-            CH_HIP(hipHostFree(t.h_jobs));
-            t.cap = std::max(n, 2 * t.cap);
-            CH_HIP(hipMalloc((void**)&t.d_jobs, t.cap * sizeof(ConvJob)));
-            CH_HIP(hipHostMalloc((void**)&t.h_jobs, t.cap * sizeof(ConvJob),
-                                 hipHostMallocDefault));
-            *out = &t;
-            return CH_OK;
-        }
-    JobTable t;
-    t.stream = s;
-    t.cap = std::max(n, 64);
-    CH_HIP(hipMalloc((void**)&t.d_jobs, t.cap * sizeof(ConvJob)));
-    CH_HIP(hipHostMalloc((void**)&t.h_jobs, t.cap * sizeof(ConvJob),
-                         hipHostMallocDefault));
-    CH_HIP(hipEventCreateWithFlags(&t.done, hipEventDisableTiming));
-    CH_HIP(hipEventRecord(t.done, s));
-    g_tables.push_back(t);
-    *out = &g_tables.back();
-    return CH_OK;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        v = min(v, __shfl_xor(v, o, 64));
+    return v;
 }
 
-// ------------------------------------------------------- support box ----
+__device__ __forceinline__ int wave_max(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
 
 __global__ __launch_bounds__(64) void box_init_kernel(int32_t* box)
 {
@@ -540,17 +591,15 @@ ch_status ch_convolve_batch(int32_t n_jobs, const ch_conv_job* h_jobs,
         return CH_OK;
     CH_REQUIRE(h_jobs, "ch_convolve_batch: null jobs");
     hipStream_t s = (hipStream_t)stream;
-    std::lock_guard<std::mutex> lock(g_tables_mu);
-    // Group consecutive jobs that share tile shape and accumulation mode;
-    // each group is one launch over all of its jobs' tiles.
+    std::lock_guard<std::mutex> lock(g_ws_mu);
+    // Group consecutive jobs that share row length and accumulation mode;
+    // each group is one launch over all of its jobs' rows.
     int32_t i0 = 0;
     while (i0 < n_jobs) {
         const ch_plan* p0 = h_jobs[i0].plan;
         CH_REQUIRE(p0, "job %d: null plan", i0);
         int32_t i1 = i0 + 1;
         while (i1 < n_jobs && h_jobs[i1].plan &&
-               h_jobs[i1].plan->tile[0] == p0->tile[0] &&
-               h_jobs[i1].plan->tile[1] == p0->tile[1] &&
                h_jobs[i1].plan->tile[2] == p0->tile[2] &&
                h_jobs[i1].plan->accum_mode == p0->accum_mode)
             ++i1;
@@ -558,31 +607,42 @@ ch_status ch_convolve_batch(int32_t n_jobs, const ch_conv_job* h_jobs,
         ch_status st = kernels_for<float>(p0, k);
         if (st != CH_OK)
             return st;
-        JobTable* tab = nullptr;
-        st = job_table(s, i1 - i0, &tab);
+        size_t pad_total = 0;
+        for (int32_t i = i0; i < i1; ++i) {
+            CH_REQUIRE(h_jobs[i].d_terma && h_jobs[i].d_dose,
+                       "job %d: null buffer", i);
+            pad_total += padded_elems(h_jobs[i].plan);
+        }
+        Workspace* w = nullptr;
+        st = workspace(s, pad_total, i1 - i0, &w);
         if (st != CH_OK)
             return st;
         int64_t tiles = 0;
+        size_t pad_off = 0;
         for (int32_t i = i0; i < i1; ++i) {
             const ch_conv_job& hj = h_jobs[i];
-            CH_REQUIRE(hj.d_terma && hj.d_dose, "job %d: null buffer", i);
-            ConvJob j = make_job(hj.plan, hj.d_terma, hj.d_dose, hj.d_box,
+            float* P = w->pad + pad_off;
+            st = pad_into(hj.d_terma, P, hj.plan, s);
+            if (st != CH_OK)
+                return st;
+            pad_off += padded_elems(hj.plan);
+            ConvJob j = make_job(hj.plan, P, hj.d_dose, hj.d_box,
                                  hj.accumulate);
             j.tile_begin = (int32_t)tiles;
-            tiles += (int64_t)j.ntx * j.nty * j.ntz;
+            tiles += job_tiles(j);
             CH_REQUIRE(tiles < INT_MAX / 2, "batch has too many tiles");
-            tab->h_jobs[i - i0] = j;
+            w->h_jobs[i - i0] = j;
         }
-        CH_HIP(hipMemcpyAsync(tab->d_jobs, tab->h_jobs,
+        CH_HIP(hipMemcpyAsync(w->d_jobs, w->h_jobs,
                               (size_t)(i1 - i0) * sizeof(ConvJob),
                               hipMemcpyHostToDevice, s));
+        CH_HIP(hipEventRecord(w->jobs_done, s));
         const unsigned blocks = (unsigned)((tiles + 3) / 4);
         LaunchTimer timer(s);
         hipLaunchKernelGGL(k.batch, dim3(blocks), dim3(256), 0, s,
-                           (const ConvJob*)tab->d_jobs, (int)(i1 - i0),
+                           (const ConvJob*)w->d_jobs, (int)(i1 - i0),
                            (int)tiles);
         CH_HIP(hipGetLastError());
-        CH_HIP(hipEventRecord(tab->done, s));
         timer.stop();
         i0 = i1;
     }

@@ -49,21 +49,26 @@ struct ConeMeta {
     int32_t pad;
 };
 
+// Zero voxels on both ends of every z-row (kZPad) and every y-column
+// (kYPad) of the padded TERMA copy the gather reads; they must be at least
+// the z- and y-extent of a wavefront's output tile.
+constexpr int32_t kZPad = 64;
+constexpr int32_t kYPad = 8;
+
 // Flat job descriptor of the gather kernels.
 struct ConvJob {
-    const float* terma;
-    void* dose;                // float* or double*
+    const float* terma;        // padded TERMA [X][Y + 2 kYPad][Z + 2 kZPad]
+    void* dose;                // float* or double*, [X][Y][Z]
     const int32_t* box;        // nullable
-    const int32_t* lin;        // per tap source byte offset 4 * (o . strides)
+    const int32_t* lin;        // per tap source byte offset in padded TERMA
     const float* k;            // per tap FP32 increment
     const int32_t* ftab;
     const ConeMeta* meta;
     int32_t X, Y, Z;
     int32_t n_cones;
     int32_t accumulate;
-    int32_t ntx, nty, ntz;     // tiles per axis
+    int32_t nty, ntz;          // tiles along y and z
     int32_t tile_begin;        // first global tile of this job
-    int32_t pad;
 };
 
 void record_kernel_ms(float ms);

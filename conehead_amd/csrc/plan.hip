@@ -43,7 +43,8 @@ __global__ __launch_bounds__(64) void build_taps_kernel(
     ftab[fb1] = 0;
     ftab[fb2] = 0;
     int32_t n = 0, prev = -1;
-    const int32_t sy = Z, sx = Y * Z;
+    // strides of the padded TERMA copy the gather reads
+    const int32_t sy = Z + 2 * kZPad, sx = (Y + 2 * kYPad) * sy;
     while (n0 < X && n1 < Y && n2 < Z && n < tap_stride - kTapPad) {
         const int a = d.axis();
         double t = a == 0 ? d.t0 : (a == 1 ? d.t1 : d.t2);
@@ -126,7 +127,8 @@ ch_status ch_plan_create(const int32_t h_shape[3], const double h_vsize[3],
     CH_REQUIRE(h_shape[0] > 0 && h_shape[1] > 0 && h_shape[2] > 0,
                "ch_plan_create: empty grid %d x %d x %d", h_shape[0],
                h_shape[1], h_shape[2]);
-    CH_REQUIRE((int64_t)h_shape[0] * h_shape[1] * h_shape[2] < (1LL << 29),
+    CH_REQUIRE((int64_t)h_shape[0] * (h_shape[1] + 2 * kYPad) *
+                       (h_shape[2] + 2 * kZPad) < (1LL << 29),
                "ch_plan_create: grid too large for 32-bit byte offsets");
     CH_REQUIRE(n_cones > 0 && h_dirs && h_cone_row,
                "ch_plan_create: no cones");
@@ -155,9 +157,9 @@ ch_status ch_plan_create(const int32_t h_shape[3], const double h_vsize[3],
     p->tap_stride = h_shape[0] + h_shape[1] + h_shape[2] + kTapPad;
     p->f_stride = h_shape[0] + h_shape[1] + h_shape[2] + 6;
     p->tile[0] = 1;
-    p->tile[1] = 1;
-    p->tile[2] = 64;
-    p->accum_mode = 0;
+    p->tile[1] = 2;
+    p->tile[2] = 32;
+    p->accum_mode = 2;
     p->d_lin = nullptr;
     p->d_k = nullptr;
     p->d_k64 = nullptr;
@@ -272,8 +274,11 @@ ch_status ch_plan_cone_taps(const ch_plan* plan, int32_t cone,
 ch_status ch_plan_set_tile(ch_plan* plan, int32_t tx, int32_t ty, int32_t tz)
 {
     CH_REQUIRE(plan, "ch_plan_set_tile: null plan");
-    CH_REQUIRE(tx > 0 && ty > 0 && tz > 0 && tx * ty * tz == 64,
-               "tile %dx%dx%d must hold 64 voxels", tx, ty, tz);
+    CH_REQUIRE(tx == 1 && ty * tz == 64 && ty >= 1 && ty <= kYPad &&
+                   tz <= kZPad && (ty & (ty - 1)) == 0,
+               "tile %dx%dx%d: a wavefront owns 1 x TY x TZ = 64 voxels, "
+               "TY in {1, 2, 4, 8}",
+               tx, ty, tz);
     plan->tile[0] = tx;
     plan->tile[1] = ty;
     plan->tile[2] = tz;
@@ -293,7 +298,7 @@ ch_status ch_plan_get_tile(const ch_plan* plan, int32_t* tx, int32_t* ty,
 ch_status ch_plan_set_accum(ch_plan* plan, int32_t mode)
 {
     CH_REQUIRE(plan, "ch_plan_set_accum: null plan");
-    CH_REQUIRE(mode == 0 || mode == 1, "accumulation mode %d", mode);
+    CH_REQUIRE(mode >= 0 && mode <= 2, "accumulation mode %d", mode);
     plan->accum_mode = mode;
     return CH_OK;
 }

@@ -59,6 +59,21 @@ __global__ void zero_kernel(uint8_t* p, size_t n)
         p[j] = 0;
 }
 
+// Grid-stride 16-byte copy plus a byte tail (device to device).
+__global__ void copy_kernel(uint8_t* __restrict__ dst,
+                            const uint8_t* __restrict__ src, size_t n)
+{
+    size_t n16 = n / 16;
+    size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    const uint4* s = reinterpret_cast<const uint4*>(src);
+    uint4* d = reinterpret_cast<uint4*>(dst);
+    for (size_t j = i; j < n16; j += stride)
+        d[j] = s[j];
+    for (size_t j = n16 * 16 + i; j < n; j += stride)
+        dst[j] = src[j];
+}
+
 }  // namespace ch
 
 using namespace ch;
@@ -147,8 +162,17 @@ ch_status ch_memcpy_d2d(void* d_dst, const void* d_src, size_t bytes,
     if (bytes == 0)
         return CH_OK;
     CH_REQUIRE(d_dst && d_src, "ch_memcpy_d2d: null pointer");
-    CH_HIP(hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice,
-                          (hipStream_t)stream));
+    CH_REQUIRE((((uintptr_t)d_dst | (uintptr_t)d_src) & 15) == 0,
+               "ch_memcpy_d2d: pointers must be 16-byte aligned");
+    size_t blocks = (bytes / 16 + 255) / 256;
+    if (blocks > 16384)
+        blocks = 16384;
+    if (blocks == 0)
+        blocks = 1;
+    hipLaunchKernelGGL(copy_kernel, dim3((unsigned)blocks), dim3(256), 0,
+                       (hipStream_t)stream, (uint8_t*)d_dst,
+                       (const uint8_t*)d_src, bytes);
+    CH_HIP(hipGetLastError());
     return CH_OK;
 }
 

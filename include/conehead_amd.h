@@ -42,6 +42,7 @@ ch_status ch_memcpy_h2d(void* d_dst, const void* h_src, size_t bytes,
                         void* stream);
 ch_status ch_memcpy_d2h(void* h_dst, const void* d_src, size_t bytes,
                         void* stream);
+/* Device-to-device copy by a kernel (16-byte aligned pointers). */
 ch_status ch_memcpy_d2d(void* d_dst, const void* d_src, size_t bytes,
                         void* stream);
 /* Zero-fills with a kernel (never hipMemset), so it is safe in captures. */

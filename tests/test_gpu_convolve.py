@@ -85,15 +85,13 @@ def test_raw_48_bins_and_strided_dose(oracle, kern):
 
 
 @pytest.mark.parametrize("tile", [(1, 1, 64), (1, 2, 32), (1, 4, 16),
-                                  (2, 2, 16), (1, 8, 8), (2, 4, 8),
-                                  (4, 4, 4), (2, 8, 4), (4, 8, 2),
-                                  (8, 8, 1)])
-@pytest.mark.parametrize("accum", [0, 1])
+                                  (1, 8, 8)])
+@pytest.mark.parametrize("accum", [0, 1, 2])
 def test_tiles_and_accumulation_modes(oracle, engine, kern, tile, accum):
     from conehead_amd import engine as E
     from conehead_amd.device import DeviceArray
     rng = np.random.default_rng(3)
-    shape = (19, 21, 67)
+    shape = (19, 21, 203)     # partial last tiles in y and z
     T = beam_like_terma(shape, rng)
     cs, k = cones(kern, 6, 4)
     vs = (0.3, 0.3, 0.3)
@@ -110,7 +108,7 @@ def test_tiles_and_accumulation_modes(oracle, engine, kern, tile, accum):
     # without the support box the result is the same sum
     E.convolve(plan, d_t, d_d, None)
     assert rel_err(d_d.download(), ref) < TOL
-    plan.set_accum(0)
+    plan.set_accum(2)
 
 
 def test_batch_equals_single_bitwise(engine, kern):
