@@ -91,7 +91,7 @@ def main():
     d_t = DeviceArray(shape, np.float32)
     d_d = DeviceArray(shape, np.float32)
     d_box = DeviceArray((6,), np.int32)
-    d_r = eng.raster(blk, s)
+    d_r = eng.aperture(blk, s)
     beam = E.beam_struct(src, st, d_r, d_rho, d_t, d_box)
     plan = eng.plan(shape, case.spacing, case.cones)
 

@@ -39,6 +39,23 @@ class CHError(RuntimeError):
         self.status = status
 
 
+CH_APERTURE_RASTER = 0
+CH_APERTURE_RECTS = 1
+CH_APERTURE_MLC = 2
+
+
+class Aperture(ctypes.Structure):
+    """Mirror of ch_aperture."""
+
+    _fields_ = [
+        ("kind", ctypes.c_int32),
+        ("n", ctypes.c_int32),
+        ("d_raster", c_vp),
+        ("d_table", c_vp),
+        ("bbox", ctypes.c_int32 * 4),
+    ]
+
+
 class Beam(ctypes.Structure):
     """Mirror of ch_beam."""
 
@@ -63,7 +80,7 @@ class Beam(ctypes.Structure):
         ("energy", ctypes.c_double * CH_MAX_ENERGIES),
         ("weight", ctypes.c_double * CH_MAX_ENERGIES),
         ("mu", ctypes.c_double * CH_MAX_ENERGIES),
-        ("d_raster", c_vp),
+        ("aperture", Aperture),
         ("d_density", c_vp),
         ("d_terma", c_vp),
         ("d_box", c_vp),
@@ -140,8 +157,10 @@ SIGNATURES = {
                                        ctypes.c_int32, c_vp, c_vp, c_vp, c_vp,
                                        c_vp]),
     "ch_raster_square": (ctypes.c_int32, [c_vp, ctypes.c_double, c_vp]),
-    "ch_raster_mlc": (ctypes.c_int32, [c_vp, ctypes.c_int32, c_i32p, c_i32p,
-                                       c_i32p, c_i32p, c_i32p, c_vp]),
+    "ch_mlc_table_len": (ctypes.c_int32, [ctypes.c_int32]),
+    "ch_mlc_table": (ctypes.c_int32, [c_vp, ctypes.c_int32, c_i32p, c_i32p,
+                                      c_i32p, c_i32p, c_i32p, c_i32p]),
+    "ch_raster_mlc": (ctypes.c_int32, [c_vp, c_vp, c_vp]),
     "ch_set_timing": (ctypes.c_int32, [ctypes.c_int32]),
     "ch_last_kernel_ms": (ctypes.c_int32, [c_f32p]),
 }

@@ -10,9 +10,12 @@ beam's aperture never crosses PCIe.  `block_values` materialises the same
 raster on the host for callers of the reference API; once a caller has taken
 that array (and may have edited it), the engine uploads it instead.
 """
+import itertools
+
 import numpy as np
 
 RASTER = 4000
+_UIDS = itertools.count()
 
 
 def _leaf_area(width, height):
@@ -83,6 +86,7 @@ class Block:
         self.ynum = np.int32(RASTER)
         self.yres = self.ynum / (self.ymax - self.ymin)
         self._recipe = [("zeros",)]
+        self._uid = next(_UIDS)
         self._values = None
         self._host_owned = False
         self._version = 0
@@ -94,7 +98,7 @@ class Block:
         """Tuple describing how the raster is built, or ("host", version)
         when the host array has been handed out."""
         if self._host_owned:
-            return ("host", id(self), self._version)
+            return ("host", self._uid, self._version)
         return tuple(self._recipe)
 
     @property

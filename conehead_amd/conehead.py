@@ -92,7 +92,7 @@ class Conehead:
         d_rho = DeviceArray.from_host(dens, stream=s)
         d_terma = DeviceArray(shape, np.float32)
         d_box = DeviceArray((6,), np.int32)
-        d_raster = eng.raster(block, s)
+        d_raster = eng.aperture(block, s)
         beam = _engine.beam_struct(source, settings, d_raster, d_rho,
                                    d_terma, d_box)
         _engine.terma([beam], shape, origin, spacing, stream=s)

@@ -1,0 +1,98 @@
+// aperture.h -- beam aperture transmission at a raster cell (row i, col j)
+// of the 4000 x 4000, 0.1 mm grid of conehead/block.py, evaluated from a
+// compact description.  The raster builder (raster.hip) and the TERMA hit
+// test (terma.hip) call the same functions, so the analytic lookups and the
+// materialised raster agree bit for bit.
+#pragma once
+
+#include "ch_internal.h"
+
+namespace ch {
+
+// Leaf transmission table of conehead/block.py:155-164 at one element:
+// 1 - area, area = ones with edge rows 0.2/0.5/0.75 (the rows counted from
+// the far edge are assigned last and win) and the last 15 columns scaled
+// by numpy.linspace(1, 0, 15) (arange * step + start, last point exact).
+// Compiled without contraction.
+__device__ __forceinline__ float leaf_value(int rr, int cc, int w, int h)
+{
+    float e = 1.0f;
+    if (rr == w - 1) e = 0.20f;
+    else if (rr == w - 2) e = 0.50f;
+    else if (rr == w - 3) e = 0.75f;
+    else if (rr == 0) e = 0.20f;
+    else if (rr == 1) e = 0.50f;
+    else if (rr == 2) e = 0.75f;
+    if (cc >= h - 15) {
+        const int q = cc - (h - 15);
+        const double step = (0.0 - 1.0) / 14;
+        const double lin = q == 14 ? 0.0 : (double)q * step + 1.0;
+        e = (float)((double)e * lin);
+    }
+    return 1.0f - e;
+}
+
+// Leaf-table layout (int32): [0] n_pairs, [1] W (bank width), [2] roff
+// (raster row of bank row 0), [3] jx0, [4] jx1 (column jaws, tenths of mm),
+// [5] jy0, [6] jy1 (bank-row jaws), [7] unused, then bound[n_pairs + 1],
+// ends of bank A [n_pairs], ends of bank B [n_pairs].
+constexpr int kMlcHeader = 8;
+
+// Leaf bank and jaws of conehead/block.py:130-196 (bank centred on raster
+// row 2000; rows outside the bank are blocked).  Leaves are contiguous and
+// ordered, so at most one contains a row: it is found by bisection.
+__device__ __forceinline__ float mlc_value(const int32_t* __restrict__ t,
+                                           int row, int col)
+{
+    const int n_pairs = t[0], W = t[1];
+    const int rr = row - t[2];
+    if (rr < 0 || rr >= W)
+        return 0.0f;
+    const int32_t* bound = t + kMlcHeader;
+    const int32_t* ea = bound + n_pairs + 1;
+    const int32_t* eb = ea + n_pairs;
+    const int shift = abs(bound[0]);
+    float v = 1.0f;
+    // largest n with bound[n] + shift <= rr
+    int lo = 0, hi = n_pairs - 1;
+    if (bound[0] + shift <= rr) {
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (bound[mid] + shift <= rr)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const int rmin = bound[lo] + shift;
+        const int w = abs(bound[lo + 1] - bound[lo]);
+        if (rr < rmin + w) {
+            const int cmax = 2000 + ea[lo];       // bank A, :187-188
+            if (col < cmax)
+                v = leaf_value(rr - rmin, col, w, cmax);
+            const int cmin = 2000 + eb[lo];       // bank B, :189-190
+            const int h = 3999 - cmin;
+            if (col >= cmin && col < 3999)
+                v = leaf_value(w - 1 - (rr - rmin), h - 1 - (col - cmin), w,
+                               h);
+        }
+    }
+    if (col < 2000 + t[3] || col >= 2000 + t[4])  // jaws, :193-196
+        v = 0.0f;
+    if (rr < t[5] || rr >= t[6])
+        v = 0.0f;
+    return v;
+}
+
+// Union of half-open rectangles [r0, r1) x [c0, c1) of ones
+// (Block.set_square, conehead/block.py:73-77), layout n x 4.
+__device__ __forceinline__ float rects_value(const int32_t* __restrict__ t,
+                                             int n, int row, int col)
+{
+    for (int k = 0; k < n; ++k)
+        if (row >= t[4 * k] && row < t[4 * k + 1] && col >= t[4 * k + 2] &&
+            col < t[4 * k + 3])
+            return 1.0f;
+    return 0.0f;
+}
+
+}  // namespace ch

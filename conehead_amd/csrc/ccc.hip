@@ -537,13 +537,15 @@ __global__ __launch_bounds__(256) void box_kernel(const float* __restrict__ T,
     mx0 = wave_max(mx0);
     mx1 = wave_max(mx1);
     mx2 = wave_max(mx2);
-    if ((threadIdx.x & 63) == 0 && mx0 != INT_MIN) {
-        atomicMin(box + 0, mn0);
-        atomicMin(box + 1, mn1);
-        atomicMin(box + 2, mn2);
-        atomicMax(box + 3, mx0);
-        atomicMax(box + 4, mx1);
-        atomicMax(box + 5, mx2);
+    // one partial box per wave; box_finalize reduces them (no atomics)
+    if ((threadIdx.x & 63) == 0) {
+        int32_t* p = box + (size_t)(i >> 6) * kBoxStride;
+        p[0] = mn0;
+        p[1] = mn1;
+        p[2] = mn2;
+        p[3] = mx0;
+        p[4] = mx1;
+        p[5] = mx2;
     }
 }
 
@@ -558,13 +560,20 @@ ch_status ch_support_box(const float* d_terma, const int32_t h_shape[3],
 {
     CH_REQUIRE(d_terma && h_shape && d_box, "ch_support_box: null pointer");
     const int64_t n = (int64_t)h_shape[0] * h_shape[1] * h_shape[2];
-    CH_REQUIRE(n > 0, "ch_support_box: empty grid");
-    hipLaunchKernelGGL(box_init_kernel, dim3(1), dim3(64), 0,
-                       (hipStream_t)stream, d_box);
+    CH_REQUIRE(n > 0 && n < (1LL << 31), "ch_support_box: grid size");
+    hipStream_t s = (hipStream_t)stream;
     const unsigned blocks = (unsigned)((n + 255) / 256);
-    hipLaunchKernelGGL(box_kernel, dim3(blocks), dim3(256), 0,
-                       (hipStream_t)stream, d_terma, h_shape[0], h_shape[1],
-                       h_shape[2], d_box);
+    void* parts = nullptr;
+    ch_status st = stream_scratch(s, kScratchBoxPartials,
+                                  (size_t)blocks * 4 * kBoxStride *
+                                      sizeof(int32_t),
+                                  &parts);
+    if (st != CH_OK)
+        return st;
+    hipLaunchKernelGGL(box_kernel, dim3(blocks), dim3(256), 0, s, d_terma,
+                       h_shape[0], h_shape[1], h_shape[2], (int32_t*)parts);
+    CH_HIP(hipGetLastError());
+    launch_box_finalize((const int32_t*)parts, (int)(blocks * 4), d_box, s);
     CH_HIP(hipGetLastError());
     return CH_OK;
 }

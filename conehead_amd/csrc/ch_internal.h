@@ -74,6 +74,19 @@ struct ConvJob {
 void record_kernel_ms(float ms);
 bool timing_enabled();
 
+// Device scratch owned by the library, one buffer per (stream, slot),
+// grown on demand.  Kernels on one stream run in order, so a stream's
+// buffer can be reused by its next launch without synchronisation.
+enum ScratchSlot { kScratchBoxPartials = 0, kScratchSlots = 1 };
+ch_status stream_scratch(hipStream_t s, int slot, size_t bytes, void** out);
+
+// Support box of T != 0 from per-wave partial boxes: kernels write
+// kBoxStride ints per wave (min x, y, z, max x, y, z) to `partials`, then
+// box_finalize reduces n_parts of them into box[0..5] (lo > hi if empty).
+constexpr int kBoxStride = 8;
+void launch_box_finalize(const int32_t* partials, int n_parts, int32_t* box,
+                         hipStream_t s);
+
 }  // namespace ch
 
 struct ch_plan {

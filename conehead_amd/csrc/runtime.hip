@@ -1,8 +1,11 @@
 // runtime.hip -- device, memory, stream and event plumbing of the C-ABI.
+#include <climits>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
+#include <vector>
 
 #include "ch_internal.h"
 
@@ -72,6 +75,84 @@ __global__ void copy_kernel(uint8_t* __restrict__ dst,
         d[j] = s[j];
     for (size_t j = n16 * 16 + i; j < n; j += stride)
         dst[j] = src[j];
+}
+
+// Reduces per-wave partial boxes (one block, grid-stride, then across
+// the block's waves through LDS).
+__global__ __launch_bounds__(256) void box_finalize_kernel(
+    const int32_t* __restrict__ parts, int n, int32_t* __restrict__ box)
+{
+    int v[6] = {INT_MAX, INT_MAX, INT_MAX, INT_MIN, INT_MIN, INT_MIN};
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const int32_t* p = parts + (size_t)i * kBoxStride;
+        for (int a = 0; a < 3; ++a) {
+            v[a] = min(v[a], p[a]);
+            v[3 + a] = max(v[3 + a], p[3 + a]);
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1)
+        for (int a = 0; a < 3; ++a) {
+            v[a] = min(v[a], __shfl_xor(v[a], o, 64));
+            v[3 + a] = max(v[3 + a], __shfl_xor(v[3 + a], o, 64));
+        }
+    __shared__ int s[4][6];
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        for (int a = 0; a < 6; ++a)
+            s[w][a] = v[a];
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int a = threadIdx.x;
+        int r = s[0][a];
+        for (int k = 1; k < (int)(blockDim.x >> 6); ++k)
+            r = a < 3 ? min(r, s[k][a]) : max(r, s[k][a]);
+        box[a] = r;
+    }
+}
+
+void launch_box_finalize(const int32_t* partials, int n_parts, int32_t* box,
+                         hipStream_t s)
+{
+    hipLaunchKernelGGL(box_finalize_kernel, dim3(1), dim3(256), 0, s,
+                       partials, n_parts, box);
+}
+
+struct StreamScratch {
+    hipStream_t stream;
+    void* ptr[kScratchSlots];
+    size_t bytes[kScratchSlots];
+};
+static std::vector<StreamScratch> g_scratch;
+static std::mutex g_scratch_mu;
+
+ch_status stream_scratch(hipStream_t s, int slot, size_t bytes, void** out)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    StreamScratch* e = nullptr;
+    for (auto& x : g_scratch)
+        if (x.stream == s)
+            e = &x;
+    if (!e) {
+        StreamScratch x;
+        x.stream = s;
+        for (int i = 0; i < kScratchSlots; ++i) {
+            x.ptr[i] = nullptr;
+            x.bytes[i] = 0;
+        }
+        g_scratch.push_back(x);
+        e = &g_scratch.back();
+    }
+    if (e->bytes[slot] < bytes) {
+        // queued kernels of this stream may still use the old buffer
+        CH_HIP(hipStreamSynchronize(s));
+        CH_HIP(hipFree(e->ptr[slot]));
+        e->ptr[slot] = nullptr;
+        e->bytes[slot] = 0;
+        CH_HIP(hipMalloc(&e->ptr[slot], bytes));
+        e->bytes[slot] = bytes;
+    }
+    *out = e->ptr[slot];
+    return CH_OK;
 }
 
 }  // namespace ch

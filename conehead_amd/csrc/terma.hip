@@ -9,8 +9,10 @@
 // depth integral uses FP32 with Kahan compensation and the fluence and TERMA
 // arithmetic is FP32.
 #include <climits>
+#include <cmath>
 #include <cstring>
 
+#include "aperture.h"
 #include "dda_core.h"
 
 namespace ch {
@@ -32,6 +34,9 @@ struct TermaParams {
     int32_t X, Y, Z;
     float ew[CH_MAX_ENERGIES];   // weight * energy * mu, folded on the host
     float emu[CH_MAX_ENERGIES];  // mu
+    int32_t ap_kind, ap_n;       // aperture (ch_aperture)
+    int32_t ap_bbox[4];
+    const int32_t* ap_table;
     const float* raster;
     const float* density;
     float* terma;
@@ -46,10 +51,11 @@ __device__ __forceinline__ double dot3(const double* a, double b0, double b1,
     return a[0] * b0 + a[1] * b1 + a[2] * b2;
 }
 
-// conehead/conehead.py:50-68 on the 4000 x 4000 raster; a -1 index wraps
-// to the last row/column as Python/Numba negative indexing does.
-__device__ __forceinline__ float transmission(double px, double pz,
-                                              const float* __restrict__ r)
+// conehead/conehead.py:50-68: raster cell floor(100 p) + 2000 - 1 of the
+// isocentre-plane point (a -1 index wraps to the last row/column as
+// Python/Numba negative indexing does), looked up in the beam's aperture.
+__device__ __forceinline__ float transmission(const TermaParams& P, double px,
+                                              double pz)
 {
     px = floor(px * 100.0) + 2000.0;
     pz = floor(pz * 100.0) + 2000.0;
@@ -60,7 +66,11 @@ __device__ __forceinline__ float transmission(double px, double pz,
         i += 4000;
     if (j < 0)
         j += 4000;
-    return r[i * 4000 + j];
+    if (P.ap_kind == CH_APERTURE_MLC)
+        return mlc_value(P.ap_table, i, j);
+    if (P.ap_kind == CH_APERTURE_RECTS)
+        return rects_value(P.ap_table, P.ap_n, i, j);
+    return P.raster[i * 4000 + j];
 }
 
 __device__ __forceinline__ int wmin(int v)
@@ -155,7 +165,38 @@ __global__ __launch_bounds__(256) void terma_kernel(TermaParams P)
         const double ndotn_src = dot3(P.vy, P.src[0], P.src[1], P.src[2]);
         double blocked = 0.0;
         bool bad = false;
-        for (int ix = 0; ix < ns; ++ix)
+        // Early out: if the voxel centre projects so far outside the
+        // aperture's bounding cells that no sub-sample can reach them, the
+        // blocked fraction is exactly 0.  With u = s - p, a = n.u and
+        // A = n.s, the plane image is s - A u / a; moving p by |d| <= h (half
+        // the voxel diagonal) moves it by at most
+        //   |A| h / |a'| (1 + |u'| / |a'|),  |a'| >= |a| - h, |u'| <= |u| + h.
+        // For |a| >= 4 h and |u| <= sqrt(5) |a| that is <= 6 |A| h / |a|;
+        // raster cells are plane coordinates times 100 (a rotation keeps
+        // lengths), plus 4 cells for the floor and index offsets.
+        bool skip = false;
+        {
+            const double h = 0.5 * sqrt(P.spacing[0] * P.spacing[0] +
+                                        P.spacing[1] * P.spacing[1] +
+                                        P.spacing[2] * P.spacing[2]);
+            const double u0 = P.src[0] - pos[0], u1 = P.src[1] - pos[1],
+                         u2 = P.src[2] - pos[2];
+            const double ndotu = dot3(P.vy, u0, u1, u2);
+            const double uu = u0 * u0 + u1 * u1 + u2 * u2;
+            if (fabs(ndotu) >= 4.0 * h + 1.0 && uu <= 5.0 * ndotu * ndotu) {
+                const double si = -ndotn_src / ndotu;
+                const double pp0 = P.src[0] + si * u0;
+                const double pp1 = P.src[1] + si * u1;
+                const double pp2 = P.src[2] + si * u2;
+                const double ci = dot3(P.tr0, pp0, pp1, pp2) * 100.0 + 1999.0;
+                const double cj = dot3(P.tr2, pp0, pp1, pp2) * 100.0 + 1999.0;
+                const double m =
+                    600.0 * h * fabs(ndotn_src / ndotu) + 4.0;
+                skip = ci + m < P.ap_bbox[0] || ci - m > P.ap_bbox[1] ||
+                       cj + m < P.ap_bbox[2] || cj - m > P.ap_bbox[3];
+            }
+        }
+        for (int ix = 0; ix < ns && !skip; ++ix)
             for (int iy = 0; iy < ns; ++iy)
                 for (int iz = 0; iz < ns; ++iz) {
                     const double ps0 = pos[0] - P.spacing[0] / 2 + off[0] / 2 + off[0] * ix;
@@ -174,7 +215,7 @@ __global__ __launch_bounds__(256) void terma_kernel(TermaParams P)
                     const double pp2 = P.src[2] + si * u2 + 0.0;
                     const double q0 = dot3(P.tr0, pp0, pp1, pp2);
                     const double q2 = dot3(P.tr2, pp0, pp1, pp2);
-                    blocked = blocked + transmission(q0, q2, P.raster) / n3;
+                    blocked = blocked + transmission(P, q0, q2) / n3;
                 }
 
         const bool dbg = P.debug != nullptr;
@@ -270,13 +311,15 @@ __global__ __launch_bounds__(256) void terma_kernel(TermaParams P)
         mx0 = wmax(mx0);
         mx1 = wmax(mx1);
         mx2 = wmax(mx2);
-        if ((threadIdx.x & 63) == 0 && mx0 != INT_MIN) {
-            atomicMin(P.box + 0, mn0);
-            atomicMin(P.box + 1, mn1);
-            atomicMin(P.box + 2, mn2);
-            atomicMax(P.box + 3, mx0);
-            atomicMax(P.box + 4, mx1);
-            atomicMax(P.box + 5, mx2);
+        // one partial box per wave; box_finalize reduces them (no atomics)
+        if ((threadIdx.x & 63) == 0) {
+            int32_t* p = P.box + (size_t)(li >> 6) * kBoxStride;
+            p[0] = mn0;
+            p[1] = mn1;
+            p[2] = mn2;
+            p[3] = mx0;
+            p[4] = mx1;
+            p[5] = mx2;
         }
     }
 }
@@ -362,25 +405,47 @@ ch_status ch_terma(int32_t n_beams, const ch_beam* h_beams,
                        g_status);
     for (int32_t b = 0; b < n_beams; ++b) {
         const ch_beam& B = h_beams[b];
-        CH_REQUIRE(B.d_raster && B.d_density && B.d_terma,
-                   "beam %d: null device buffer", b);
+        const ch_aperture& A = B.aperture;
+        CH_REQUIRE(B.d_density && B.d_terma, "beam %d: null device buffer",
+                   b);
+        CH_REQUIRE((A.kind == CH_APERTURE_RASTER && A.d_raster) ||
+                       (A.kind == CH_APERTURE_RECTS && A.n >= 0 &&
+                        (A.n == 0 || A.d_table)) ||
+                       (A.kind == CH_APERTURE_MLC && A.d_table),
+                   "beam %d: bad aperture (kind %d)", b, A.kind);
         CH_REQUIRE(B.samples >= 1 && B.samples <= 16,
                    "beam %d: fluenceResampling %d", b, B.samples);
         CH_REQUIRE(B.n_energy >= 0 && B.n_energy <= CH_MAX_ENERGIES,
                    "beam %d: %d energies", b, B.n_energy);
         TermaParams P = make_params(B, h_shape, h_origin, h_spacing);
-        P.raster = B.d_raster;
+        P.ap_kind = A.kind;
+        P.ap_n = A.n;
+        P.ap_table = A.d_table;
+        P.raster = A.d_raster;
+        for (int q = 0; q < 4; ++q)
+            P.ap_bbox[q] = A.bbox[q];
         P.density = B.d_density;
         P.terma = B.d_terma;
-        P.box = B.d_box;
         P.debug = d_debug ? d_debug + (size_t)b * 4 * nv : nullptr;
         P.status = g_status;
-        if (P.box)
-            hipLaunchKernelGGL(terma_prep_kernel, dim3(1), dim3(64), 0, s,
-                               P.box, nullptr);
-        hipLaunchKernelGGL(terma_kernel, dim3((unsigned)((nv + 255) / 256)),
-                           dim3(256), 0, s, P);
+        const unsigned blocks = (unsigned)((nv + 255) / 256);
+        P.box = nullptr;
+        if (B.d_box) {
+            void* parts = nullptr;
+            ch_status st = stream_scratch(s, kScratchBoxPartials,
+                                          (size_t)blocks * 4 * kBoxStride *
+                                              sizeof(int32_t),
+                                          &parts);
+            if (st != CH_OK)
+                return st;
+            P.box = (int32_t*)parts;
+        }
+        hipLaunchKernelGGL(terma_kernel, dim3(blocks), dim3(256), 0, s, P);
         CH_HIP(hipGetLastError());
+        if (B.d_box) {
+            launch_box_finalize(P.box, (int)(blocks * 4), B.d_box, s);
+            CH_HIP(hipGetLastError());
+        }
     }
     return CH_OK;
 }

@@ -133,10 +133,27 @@ def convolve_batch(jobs, stream=None):
     _lib.check(_lib.lib().ch_convolve_batch(n, arr, _h(stream)))
 
 
-def beam_struct(source, settings, d_raster, d_density, d_terma, d_box):
+class ApertureRef:
+    """A ch_aperture plus the device buffers it points into."""
+
+    def __init__(self, kind, n=0, raster=None, table=None,
+                 bbox=(0, 4000, 0, 4000)):
+        self.raster = raster
+        self.table = table
+        self.struct = _lib.Aperture()
+        self.struct.kind = kind
+        self.struct.n = n
+        self.struct.d_raster = raster.ptr if raster is not None else None
+        self.struct.d_table = table.ptr if table is not None else None
+        for i in range(4):
+            self.struct.bbox[i] = int(bbox[i])
+
+
+def beam_struct(source, settings, aperture, d_density, d_terma, d_box):
     """ch_beam for one beam: geometry from `source`, fluence and spectrum
     parameters from the reference's settings dict (example.py:27-58).
-    Energies and weights are float32 as in conehead/conehead.py:658-661."""
+    Energies and weights are float32 as in conehead/conehead.py:658-661.
+    `aperture` is an ApertureRef or a DeviceArray raster."""
     b = _lib.Beam()
     pos = np.asarray(source.position, dtype=np.float64)
     vy = np.asarray(source.v_y, dtype=np.float64)
@@ -158,7 +175,9 @@ def beam_struct(source, settings, d_raster, d_density, d_terma, d_box):
         b.energy[i] = float(e[i])
         b.weight[i] = float(w[i])
         b.mu[i] = float(mu[i])
-    b.d_raster = d_raster.ptr
+    if not isinstance(aperture, ApertureRef):
+        aperture = ApertureRef(_lib.CH_APERTURE_RASTER, raster=aperture)
+    b.aperture = aperture.struct
     b.d_density = d_density.ptr
     b.d_terma = d_terma.ptr
     b.d_box = d_box.ptr if d_box is not None else None
@@ -219,11 +238,26 @@ class Engine:
             return p
 
     # -- apertures -------------------------------------------------------
+    @staticmethod
+    def _mlc_table(op):
+        """Device leaf table and bounding cells of an ("mlc", ...) op."""
+        bound, a, b, jx, jy = [np.ascontiguousarray(v, np.int32)
+                               for v in op[1:]]
+        L = _lib.lib()
+        table = DeviceArray((L.ch_mlc_table_len(len(a)),), np.int32)
+        bbox = (ctypes.c_int32 * 4)()
+        _lib.check(L.ch_mlc_table(
+            table.ptr, len(a), bound.ctypes.data_as(_lib.c_i32p),
+            a.ctypes.data_as(_lib.c_i32p), b.ctypes.data_as(_lib.c_i32p),
+            jx.ctypes.data_as(_lib.c_i32p), jy.ctypes.data_as(_lib.c_i32p),
+            bbox))
+        return table, tuple(bbox)
+
     def raster(self, block, stream=None):
         """Device raster of a Block, built on the GPU from its recipe."""
         key = block.recipe()
         with self._lock:
-            d = self._rasters.get(key)
+            d = self._rasters.get(("raster", key))
             if d is not None:
                 return d
             d = DeviceArray((4000, 4000), np.float32)
@@ -238,18 +272,64 @@ class Engine:
                         _lib.check(L.ch_raster_square(d.ptr, op[1],
                                                       _h(stream)))
                     elif op[0] == "mlc":
-                        bound, a, b, jx, jy = [np.asarray(v, np.int32)
-                                               for v in op[1:]]
-                        _lib.check(L.ch_raster_mlc(
-                            d.ptr, len(a), bound.ctypes.data_as(_lib.c_i32p),
-                            a.ctypes.data_as(_lib.c_i32p),
-                            b.ctypes.data_as(_lib.c_i32p),
-                            jx.ctypes.data_as(_lib.c_i32p),
-                            jy.ctypes.data_as(_lib.c_i32p), _h(stream)))
-            if len(self._rasters) > 256:
-                self._rasters.clear()
-            self._rasters[key] = d
+                        table, _ = self._mlc_table(op)
+                        _lib.check(L.ch_raster_mlc(d.ptr, table.ptr,
+                                                   _h(stream)))
+                        _lib.check(L.ch_stream_sync(_h(stream)))
+            self._cache_put(("raster", key), d)
             return d
+
+    def aperture(self, block, stream=None):
+        """ApertureRef of a Block: analytic rectangles for squares, a leaf
+        table for an MLC, the raster otherwise (host-edited rasters)."""
+        from .block import square_bounds
+        key = block.recipe()
+        with self._lock:
+            ap = self._rasters.get(("aperture", key))
+            if ap is not None:
+                return ap
+            ops = list(key) if key[0] != "host" else None
+            if ops is not None and all(op[0] in ("zeros", "square")
+                                       for op in ops):
+                rects = []
+                for op in ops:
+                    if op[0] == "zeros":
+                        rects = []
+                    else:
+                        lo, hi = square_bounds(op[1])
+                        lo, hi = max(lo, 0), min(hi, 4000)
+                        if hi > lo:
+                            rects.append((lo, hi, lo, hi))
+                if rects:
+                    r = np.array(rects, np.int32)
+                    table = DeviceArray.from_host(r)
+                    bbox = (r[:, 0].min(), r[:, 1].max(), r[:, 2].min(),
+                            r[:, 3].max())
+                else:
+                    table, bbox = None, (0, 0, 0, 0)
+                ap = ApertureRef(_lib.CH_APERTURE_RECTS, len(rects),
+                                 table=table, bbox=bbox)
+            elif ops is not None and len(ops) == 1 and ops[0][0] == "mlc":
+                table, bbox = self._mlc_table(ops[0])
+                ap = ApertureRef(_lib.CH_APERTURE_MLC, table=table,
+                                 bbox=bbox)
+            else:
+                raster = self.raster(block, stream)
+                bbox = (0, 4000, 0, 4000)
+                if key[0] == "host":
+                    nz = np.nonzero(block._values)
+                    bbox = ((nz[0].min(), nz[0].max() + 1, nz[1].min(),
+                             nz[1].max() + 1) if len(nz[0]) else
+                            (0, 0, 0, 0))
+                ap = ApertureRef(_lib.CH_APERTURE_RASTER, raster=raster,
+                                 bbox=bbox)
+            self._cache_put(("aperture", key), ap)
+            return ap
+
+    def _cache_put(self, key, value):
+        if len(self._rasters) > 512:
+            self._rasters.clear()
+        self._rasters[key] = value
 
     def drop_caches(self):
         with self._lock:

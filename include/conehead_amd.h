@@ -173,9 +173,27 @@ ch_status ch_plan_set_accum(ch_plan* plan, int32_t mode);
 
 #define CH_MAX_ENERGIES 32
 
-/* Beam description.  Geometry in cm, world frame of the dose grid.
- * d_raster: 4000 x 4000 float32 transmission raster at 0.1 mm pitch
- * (block.block_values, conehead/block.py:14-26). */
+/* Beam aperture: the transmission of the 4000 x 4000 raster at 0.1 mm pitch
+ * (block.block_values, conehead/block.py:14-26), given either as the
+ * raster itself or analytically, so the hit test never reads a 64 MB array:
+ *   CH_APERTURE_RASTER  d_raster, 4000 x 4000 float32;
+ *   CH_APERTURE_RECTS   d_table = n x 4 int32 half-open cell rectangles
+ *                       [row0, row1) x [col0, col1) of ones (squares);
+ *   CH_APERTURE_MLC     d_table = a leaf table from ch_mlc_table.
+ * bbox = [row0, row1, col0, col1): every nonzero cell lies inside it (the
+ * hit test skips voxels whose projection provably misses it). */
+#define CH_APERTURE_RASTER 0
+#define CH_APERTURE_RECTS 1
+#define CH_APERTURE_MLC 2
+typedef struct ch_aperture {
+    int32_t kind;
+    int32_t n;
+    const float* d_raster;
+    const int32_t* d_table;
+    int32_t bbox[4];
+} ch_aperture;
+
+/* Beam description.  Geometry in cm, world frame of the dose grid. */
 typedef struct ch_beam {
     double source_position[3];
     double v_y[3];
@@ -188,7 +206,7 @@ typedef struct ch_beam {
     double energy[CH_MAX_ENERGIES];
     double weight[CH_MAX_ENERGIES];
     double mu[CH_MAX_ENERGIES];
-    const float* d_raster;
+    ch_aperture aperture;
     const float* d_density;     /* [X][Y][Z] densities on the dose grid */
     float* d_terma;             /* output [X][Y][Z] */
     int32_t* d_box;             /* output support box (6 ints), nullable */
@@ -227,18 +245,26 @@ ch_status ch_terma_check(void* stream);
  * (conehead/block.py:64-85). */
 ch_status ch_raster_square(float* d_raster, double length, void* stream);
 
-/* Multi-leaf collimator raster built on the device from a leaf table, all
- * positions in integer tenths of a millimetre (floor(mm * 10), as
- * conehead/block.py:117-120 converts them): n_pairs leaf pairs with
- * boundaries h_bound[n_pairs+1] (across the leaves), bank A / B tip
- * positions h_a, h_b (along leaf travel) and jaws h_jaw_x[2], h_jaw_y[2].
- * Leaf edge transmission (0.2/0.5/0.75) and the 15-column tip taper follow
- * conehead/block.py:130-196; the bank is centred on raster row 2000 and rows
- * outside it are blocked.  Replaces Block._set_from_plan's raster. */
-ch_status ch_raster_mlc(float* d_raster, int32_t n_pairs,
-                        const int32_t* h_bound, const int32_t* h_a,
-                        const int32_t* h_b, const int32_t h_jaw_x[2],
-                        const int32_t h_jaw_y[2], void* stream);
+/* Device leaf table of a multi-leaf collimator, all positions in integer
+ * tenths of a millimetre (floor(mm * 10), as conehead/block.py:117-120
+ * converts them): n_pairs leaf pairs with boundaries h_bound[n_pairs+1]
+ * (across the leaves), bank A / B tip positions h_a, h_b (along leaf
+ * travel) and jaws h_jaw_x[2], h_jaw_y[2].  Leaf edge transmission
+ * (0.2/0.5/0.75) and the 15-column tip taper follow conehead/block.py:
+ * 130-196; the bank is centred on raster row 2000 and rows outside it are
+ * blocked.  d_table holds ch_mlc_table_len(n_pairs) int32; h_bbox_out
+ * (nullable) receives the aperture's bounding cell box.  Replaces
+ * Block._set_from_plan's raster. */
+int32_t ch_mlc_table_len(int32_t n_pairs);
+ch_status ch_mlc_table(int32_t* d_table, int32_t n_pairs,
+                       const int32_t* h_bound, const int32_t* h_a,
+                       const int32_t* h_b, const int32_t h_jaw_x[2],
+                       const int32_t h_jaw_y[2], int32_t* h_bbox_out);
+
+/* Materialises the raster of a leaf table (the same cell function the
+ * analytic aperture evaluates). */
+ch_status ch_raster_mlc(float* d_raster, const int32_t* d_table,
+                        void* stream);
 
 /* Kernel time of the last gather launch issued by this thread on `stream`,
  * measured with HIP events around the launch when timing is enabled. */

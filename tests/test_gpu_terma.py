@@ -32,12 +32,19 @@ def compare(oracle, engine, shape, spacing, origin, dens, gantry, block,
     d_t = DeviceArray(shape, np.float32)
     d_box = DeviceArray((6,), np.int32)
     d_dbg = DeviceArray((4,) + tuple(shape), np.float32)
-    d_r = engine.raster(block)
-    beam = E.beam_struct(src, SETTINGS, d_r, d_rho, d_t, d_box)
+    # analytic aperture (rectangles / leaf table) with the bounding-box
+    # early out, and the materialised raster: identical TERMA bits
+    ap = engine.aperture(block)
+    beam = E.beam_struct(src, SETTINGS, ap, d_rho, d_t, d_box)
     E.terma([beam], shape, origin, spacing, debug=d_dbg)
     T = d_t.download()
     dbg = d_dbg.download()
     box = d_box.download()
+    d_t2 = DeviceArray(shape, np.float32)
+    beam2 = E.beam_struct(src, SETTINGS, engine.raster(block), d_rho, d_t2,
+                          None)
+    E.terma([beam2], shape, origin, spacing)
+    assert np.array_equal(d_t2.download(), T)
     assert np.array_equal(dbg[0], ref["blocked"].astype(np.float32))
     m = ref["blocked"] > 0
     np.testing.assert_allclose(dbg[1][m], ref["d_eff"][m], rtol=2e-6,
