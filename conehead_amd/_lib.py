@@ -84,6 +84,8 @@ class Beam(ctypes.Structure):
         ("d_density", c_vp),
         ("d_terma", c_vp),
         ("d_box", c_vp),
+        ("accumulate", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 

@@ -1,0 +1,209 @@
+"""Batched dose engine: many phantoms, beams and cone sets per launch.
+
+A DoseRequest is one phantom (densities on its dose grid), a list of beams
+and a cone set.  A batch of requests runs as one TERMA launch per beam and
+ONE gather launch over every convolution job of the batch (jobs of
+different grids, voxel sizes and cone sets share the launch; see
+ch_convolve_batch).  Two output modes:
+  per_beam=True   one dose grid per beam (each beam is its own gather job);
+  per_beam=False  the plan dose of all beams: their TERMA grids are summed
+                  (ch_terma accumulate mode) and convolved once -- exact,
+                  because convolve_c is linear in TERMA and its cones are
+                  fixed to the grid (conehead/convolve_c.pyx:101).
+Device buffers come from a size-keyed pool, densities can stay resident
+under a caller key, and the gather is deterministic, so a request's result
+is bitwise the same alone or inside any batch.
+"""
+import threading
+
+import numpy as np
+
+from . import engine as E
+from .device import DeviceArray, Event, Stream, _h
+
+PAD_Y = 8    # kYPad of the padded TERMA copy (ch_internal.h)
+PAD_Z = 64   # kZPad
+
+
+class DoseRequest:
+    """One dose calculation: phantom + beams + cones."""
+
+    def __init__(self, densities, origin, spacing, beams, cones,
+                 per_beam=True, key=None):
+        self.densities = np.ascontiguousarray(densities, np.float32)
+        self.shape = tuple(int(s) for s in self.densities.shape)
+        self.origin = np.asarray(origin, np.float64)
+        self.spacing = np.asarray(spacing, np.float64)
+        self.beams = list(beams)
+        self.cones = cones
+        self.per_beam = per_beam
+        self.key = key
+
+    @classmethod
+    def from_case(cls, case, per_beam=True, key=None):
+        return cls(case.densities, case.origin, case.spacing, case.beams,
+                   case.cones, per_beam, key)
+
+    @property
+    def n_voxels(self):
+        return int(np.prod(self.shape))
+
+    @property
+    def n_jobs(self):
+        return len(self.beams) if self.per_beam else 1
+
+    def device_bytes(self):
+        """HBM this request holds while in flight: densities, TERMA, dose
+        and box per job, and its padded TERMA copy in the gather workspace."""
+        X, Y, Z = self.shape
+        nv = self.n_voxels
+        padded = X * (Y + 2 * PAD_Y) * (Z + 2 * PAD_Z)
+        return 4 * (nv + self.n_jobs * (2 * nv + padded) + 8 * self.n_jobs)
+
+    def voxel_cone_updates(self):
+        """Dense work count Nv x C per beam, summed over the beams."""
+        return self.n_voxels * self.cones.n_cones * len(self.beams)
+
+
+class DevicePool:
+    """Reusable device buffers keyed by (shape, dtype)."""
+
+    def __init__(self, max_free_bytes=64 << 30):
+        self._free = {}
+        self._bytes = 0
+        self._max = max_free_bytes
+        self._lock = threading.Lock()
+
+    def get(self, shape, dtype=np.float32):
+        key = (tuple(shape), np.dtype(dtype).str)
+        with self._lock:
+            lst = self._free.get(key)
+            if lst:
+                a = lst.pop()
+                self._bytes -= a.nbytes
+                return a
+        return DeviceArray(tuple(shape), dtype)
+
+    def put(self, arr):
+        key = (arr.shape, arr.dtype.str)
+        with self._lock:
+            if self._bytes + arr.nbytes > self._max:
+                arr.free()
+                return
+            self._free.setdefault(key, []).append(arr)
+            self._bytes += arr.nbytes
+
+    def clear(self):
+        with self._lock:
+            for lst in self._free.values():
+                for a in lst:
+                    a.free()
+            self._free.clear()
+            self._bytes = 0
+
+
+class BatchHandle:
+    """A launched batch: its completion event and device outputs."""
+
+    def __init__(self, requests, outputs, temps, event, stream):
+        self.requests = requests
+        self.outputs = outputs     # per request: list of dose DeviceArrays
+        self.temps = temps         # buffers to return to the pool
+        self.event = event
+        self.stream = stream
+
+    def done(self):
+        return self.event.done()
+
+
+class BatchEngine:
+    """Runs batches of DoseRequests on one GPU."""
+
+    def __init__(self, engine=None, pool=None):
+        self.engine = engine or E.Engine()
+        self.pool = pool or DevicePool()
+        self._resident = {}
+        self._lock = threading.Lock()
+
+    # -- resident phantoms -------------------------------------------------
+    def densities(self, req, stream=None):
+        """Device densities of a request, kept resident under req.key."""
+        if req.key is not None:
+            with self._lock:
+                d = self._resident.get(req.key)
+            if d is not None:
+                return d, False
+        d = self.pool.get(req.shape)
+        d.upload(req.densities, stream)
+        if req.key is not None:
+            with self._lock:
+                self._resident[req.key] = d
+            return d, False
+        return d, True
+
+    def evict(self, key):
+        with self._lock:
+            d = self._resident.pop(key, None)
+        if d is not None:
+            self.pool.put(d)
+
+    # -- one batch ---------------------------------------------------------
+    def launch(self, requests, stream=None):
+        """Queue TERMA and one gather launch for `requests` on `stream`;
+        returns a BatchHandle (outputs valid once handle.done())."""
+        s = stream or Stream()
+        eng = self.engine
+        jobs, outputs, temps = [], [], []
+        for req in requests:
+            d_rho, own = self.densities(req, s)
+            if own:
+                temps.append(d_rho)
+            plan = eng.plan(req.shape, req.spacing, req.cones)
+            beams, outs = [], []
+            n_jobs = req.n_jobs
+            t_bufs = [self.pool.get(req.shape) for _ in range(n_jobs)]
+            boxes = [self.pool.get((8,), np.int32) for _ in range(n_jobs)]
+            for b, (src, blk, st) in enumerate(req.beams):
+                j = b if req.per_beam else 0
+                bs = E.beam_struct(src, st, eng.aperture(blk, s), d_rho,
+                                   t_bufs[j], boxes[j])
+                bs.accumulate = 0 if (req.per_beam or b == 0) else 1
+                beams.append(bs)
+            E.terma(beams, req.shape, req.origin, req.spacing, stream=s,
+                    check=False)
+            for j in range(n_jobs):
+                dose = self.pool.get(req.shape)
+                jobs.append((plan, t_bufs[j], dose, boxes[j], False))
+                outs.append(dose)
+            temps.extend(t_bufs)
+            temps.extend(boxes)
+            outputs.append(outs)
+        E.convolve_batch(jobs, stream=s)
+        ev = Event().record(s)
+        return BatchHandle(list(requests), outputs, temps, ev, s)
+
+    def collect(self, handle, download=True):
+        """Wait for a batch; returns per request a float32 array
+        [n_jobs][X][Y][Z] (or the device arrays if download=False) and
+        returns the batch's temporaries to the pool."""
+        handle.event.sync()
+        from . import _lib
+        _lib.check(_lib.lib().ch_terma_check(_h(handle.stream)))
+        results = []
+        for req, outs in zip(handle.requests, handle.outputs):
+            if download:
+                arr = np.empty((len(outs),) + req.shape, np.float32)
+                for i, d in enumerate(outs):
+                    d.download(arr[i], stream=handle.stream)
+                    self.pool.put(d)
+                results.append(arr)
+            else:
+                results.append(outs)
+        for t in handle.temps:
+            self.pool.put(t)
+        handle.temps = []
+        return results
+
+    def run(self, requests, stream=None):
+        """Launch and collect one batch."""
+        return self.collect(self.launch(requests, stream))

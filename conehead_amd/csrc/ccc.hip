@@ -446,6 +446,24 @@ static ch_status workspace(hipStream_t s, size_t pad_elems, int n_jobs,
     return CH_OK;
 }
 
+void release_gather_workspace(hipStream_t s)
+{
+    std::lock_guard<std::mutex> lock(g_ws_mu);
+    for (size_t i = 0; i < g_ws.size(); ++i) {
+        Workspace* w = g_ws[i];
+        if (w->stream != s)
+            continue;
+        (void)hipStreamSynchronize(s);
+        (void)hipFree(w->pad);
+        (void)hipFree(w->d_jobs);
+        (void)hipHostFree(w->h_jobs);
+        (void)hipEventDestroy(w->jobs_done);
+        delete w;
+        g_ws.erase(g_ws.begin() + i);
+        return;
+    }
+}
+
 static ch_status pad_into(const float* T, float* P, const ch_plan* p,
                           hipStream_t s)
 {
@@ -504,14 +522,6 @@ __device__ __forceinline__ int wave_max(int v)
     for (int o = 32; o > 0; o >>= 1)
         v = max(v, __shfl_xor(v, o, 64));
     return v;
-}
-
-__global__ __launch_bounds__(64) void box_init_kernel(int32_t* box)
-{
-    if (threadIdx.x < 3)
-        box[threadIdx.x] = INT_MAX;
-    else if (threadIdx.x < 6)
-        box[threadIdx.x] = INT_MIN;
 }
 
 __global__ __launch_bounds__(256) void box_kernel(const float* __restrict__ T,
@@ -573,7 +583,8 @@ ch_status ch_support_box(const float* d_terma, const int32_t h_shape[3],
     hipLaunchKernelGGL(box_kernel, dim3(blocks), dim3(256), 0, s, d_terma,
                        h_shape[0], h_shape[1], h_shape[2], (int32_t*)parts);
     CH_HIP(hipGetLastError());
-    launch_box_finalize((const int32_t*)parts, (int)(blocks * 4), d_box, s);
+    launch_box_finalize((const int32_t*)parts, (int)(blocks * 4), d_box, 0,
+                        s);
     CH_HIP(hipGetLastError());
     return CH_OK;
 }
@@ -600,36 +611,44 @@ ch_status ch_convolve_batch(int32_t n_jobs, const ch_conv_job* h_jobs,
         return CH_OK;
     CH_REQUIRE(h_jobs, "ch_convolve_batch: null jobs");
     hipStream_t s = (hipStream_t)stream;
+    for (int32_t i = 0; i < n_jobs; ++i) {
+        CH_REQUIRE(h_jobs[i].plan, "job %d: null plan", i);
+        CH_REQUIRE(h_jobs[i].d_terma && h_jobs[i].d_dose,
+                   "job %d: null buffer", i);
+    }
     std::lock_guard<std::mutex> lock(g_ws_mu);
-    // Group consecutive jobs that share row length and accumulation mode;
-    // each group is one launch over all of its jobs' rows.
-    int32_t i0 = 0;
-    while (i0 < n_jobs) {
+    // Jobs that share the tile shape and accumulation mode run as one
+    // launch (in order of first appearance); each job keeps its own output,
+    // so grouping changes no result.
+    std::vector<char> taken(n_jobs, 0);
+    std::vector<int32_t> group;
+    for (int32_t i0 = 0; i0 < n_jobs; ++i0) {
+        if (taken[i0])
+            continue;
         const ch_plan* p0 = h_jobs[i0].plan;
-        CH_REQUIRE(p0, "job %d: null plan", i0);
-        int32_t i1 = i0 + 1;
-        while (i1 < n_jobs && h_jobs[i1].plan &&
-               h_jobs[i1].plan->tile[2] == p0->tile[2] &&
-               h_jobs[i1].plan->accum_mode == p0->accum_mode)
-            ++i1;
+        group.clear();
+        for (int32_t i = i0; i < n_jobs; ++i)
+            if (!taken[i] && h_jobs[i].plan->tile[2] == p0->tile[2] &&
+                h_jobs[i].plan->accum_mode == p0->accum_mode) {
+                taken[i] = 1;
+                group.push_back(i);
+            }
+        const int32_t ng = (int32_t)group.size();
         Kernels<float> k;
         ch_status st = kernels_for<float>(p0, k);
         if (st != CH_OK)
             return st;
         size_t pad_total = 0;
-        for (int32_t i = i0; i < i1; ++i) {
-            CH_REQUIRE(h_jobs[i].d_terma && h_jobs[i].d_dose,
-                       "job %d: null buffer", i);
+        for (int32_t i : group)
             pad_total += padded_elems(h_jobs[i].plan);
-        }
         Workspace* w = nullptr;
-        st = workspace(s, pad_total, i1 - i0, &w);
+        st = workspace(s, pad_total, ng, &w);
         if (st != CH_OK)
             return st;
         int64_t tiles = 0;
         size_t pad_off = 0;
-        for (int32_t i = i0; i < i1; ++i) {
-            const ch_conv_job& hj = h_jobs[i];
+        for (int32_t g = 0; g < ng; ++g) {
+            const ch_conv_job& hj = h_jobs[group[g]];
             float* P = w->pad + pad_off;
             st = pad_into(hj.d_terma, P, hj.plan, s);
             if (st != CH_OK)
@@ -640,20 +659,20 @@ ch_status ch_convolve_batch(int32_t n_jobs, const ch_conv_job* h_jobs,
             j.tile_begin = (int32_t)tiles;
             tiles += job_tiles(j);
             CH_REQUIRE(tiles < INT_MAX / 2, "batch has too many tiles");
-            w->h_jobs[i - i0] = j;
+            w->h_jobs[g] = j;
         }
         CH_HIP(hipMemcpyAsync(w->d_jobs, w->h_jobs,
-                              (size_t)(i1 - i0) * sizeof(ConvJob),
+                              (size_t)ng * sizeof(ConvJob),
                               hipMemcpyHostToDevice, s));
         CH_HIP(hipEventRecord(w->jobs_done, s));
+        if (tiles == 0)
+            continue;
         const unsigned blocks = (unsigned)((tiles + 3) / 4);
         LaunchTimer timer(s);
         hipLaunchKernelGGL(k.batch, dim3(blocks), dim3(256), 0, s,
-                           (const ConvJob*)w->d_jobs, (int)(i1 - i0),
-                           (int)tiles);
+                           (const ConvJob*)w->d_jobs, ng, (int)tiles);
         CH_HIP(hipGetLastError());
         timer.stop();
-        i0 = i1;
     }
     return CH_OK;
 }

@@ -77,15 +77,24 @@ bool timing_enabled();
 // Device scratch owned by the library, one buffer per (stream, slot),
 // grown on demand.  Kernels on one stream run in order, so a stream's
 // buffer can be reused by its next launch without synchronisation.
-enum ScratchSlot { kScratchBoxPartials = 0, kScratchSlots = 1 };
+enum ScratchSlot {
+    kScratchBoxPartials = 0,  // per-wave partial support boxes
+    kScratchStatus = 1,       // TERMA geometry flag (one int32)
+    kScratchSlots = 2
+};
 ch_status stream_scratch(hipStream_t s, int slot, size_t bytes, void** out);
+// The slot's buffer if this stream has one, else nullptr (no allocation).
+void* stream_scratch_peek(hipStream_t s, int slot);
+// Free a stream's scratch and gather workspace (before it is destroyed).
+void release_stream_scratch(hipStream_t s);
+void release_gather_workspace(hipStream_t s);
 
 // Support box of T != 0 from per-wave partial boxes: kernels write
 // kBoxStride ints per wave (min x, y, z, max x, y, z) to `partials`, then
 // box_finalize reduces n_parts of them into box[0..5] (lo > hi if empty).
 constexpr int kBoxStride = 8;
 void launch_box_finalize(const int32_t* partials, int n_parts, int32_t* box,
-                         hipStream_t s);
+                         int widen, hipStream_t s);
 
 }  // namespace ch
 

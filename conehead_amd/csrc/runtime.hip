@@ -80,7 +80,8 @@ __global__ void copy_kernel(uint8_t* __restrict__ dst,
 // Reduces per-wave partial boxes (one block, grid-stride, then across
 // the block's waves through LDS).
 __global__ __launch_bounds__(256) void box_finalize_kernel(
-    const int32_t* __restrict__ parts, int n, int32_t* __restrict__ box)
+    const int32_t* __restrict__ parts, int n, int32_t* __restrict__ box,
+    int widen)
 {
     int v[6] = {INT_MAX, INT_MAX, INT_MAX, INT_MIN, INT_MIN, INT_MIN};
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
@@ -106,15 +107,17 @@ __global__ __launch_bounds__(256) void box_finalize_kernel(
         int r = s[0][a];
         for (int k = 1; k < (int)(blockDim.x >> 6); ++k)
             r = a < 3 ? min(r, s[k][a]) : max(r, s[k][a]);
+        if (widen)  // union with the box already there
+            r = a < 3 ? min(r, box[a]) : max(r, box[a]);
         box[a] = r;
     }
 }
 
 void launch_box_finalize(const int32_t* partials, int n_parts, int32_t* box,
-                         hipStream_t s)
+                         int widen, hipStream_t s)
 {
     hipLaunchKernelGGL(box_finalize_kernel, dim3(1), dim3(256), 0, s,
-                       partials, n_parts, box);
+                       partials, n_parts, box, widen);
 }
 
 struct StreamScratch {
@@ -153,6 +156,29 @@ ch_status stream_scratch(hipStream_t s, int slot, size_t bytes, void** out)
     }
     *out = e->ptr[slot];
     return CH_OK;
+}
+
+void release_stream_scratch(hipStream_t s)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    for (size_t i = 0; i < g_scratch.size(); ++i) {
+        if (g_scratch[i].stream != s)
+            continue;
+        (void)hipStreamSynchronize(s);
+        for (int k = 0; k < kScratchSlots; ++k)
+            (void)hipFree(g_scratch[i].ptr[k]);
+        g_scratch.erase(g_scratch.begin() + i);
+        return;
+    }
+}
+
+void* stream_scratch_peek(hipStream_t s, int slot)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    for (auto& x : g_scratch)
+        if (x.stream == s)
+            return x.ptr[slot];
+    return nullptr;
 }
 
 }  // namespace ch
@@ -300,8 +326,12 @@ ch_status ch_stream_create(void** stream)
 
 ch_status ch_stream_destroy(void* stream)
 {
-    if (stream)
-        CH_HIP(hipStreamDestroy((hipStream_t)stream));
+    if (stream) {
+        hipStream_t s = (hipStream_t)stream;
+        release_gather_workspace(s);
+        release_stream_scratch(s);
+        CH_HIP(hipStreamDestroy(s));
+    }
     return CH_OK;
 }
 

@@ -43,6 +43,7 @@ struct TermaParams {
     int32_t* box;
     float* debug;              // 4 planes of X*Y*Z or null
     int32_t* status;           // geometry error flag
+    int32_t accumulate;        // terma += this beam
 };
 
 __device__ __forceinline__ double dot3(const double* a, double b0, double b1,
@@ -293,7 +294,7 @@ __global__ __launch_bounds__(256) void terma_kernel(TermaParams P)
                 P.debug[3 * nv + li] = flu;
             }
         }
-        P.terma[li] = T;
+        P.terma[li] = P.accumulate ? P.terma[li] + T : T;
         if (bad)
             atomicOr(P.status, 1);
     }
@@ -324,20 +325,11 @@ __global__ __launch_bounds__(256) void terma_kernel(TermaParams P)
     }
 }
 
-__global__ __launch_bounds__(64) void terma_prep_kernel(int32_t* box,
-                                                        int32_t* status)
+__global__ __launch_bounds__(64) void terma_prep_kernel(int32_t* status)
 {
-    if (box) {
-        if (threadIdx.x < 3)
-            box[threadIdx.x] = INT_MAX;
-        else if (threadIdx.x < 6)
-            box[threadIdx.x] = INT_MIN;
-    }
-    if (status && threadIdx.x == 0)
+    if (threadIdx.x == 0)
         status[0] = 0;
 }
-
-static int32_t* g_status = nullptr;
 
 static TermaParams make_params(const ch_beam& B, const int32_t* h_shape,
                                const double* h_origin,
@@ -398,11 +390,13 @@ ch_status ch_terma(int32_t n_beams, const ch_beam* h_beams,
     const int64_t nv = (int64_t)h_shape[0] * h_shape[1] * h_shape[2];
     CH_REQUIRE(nv < (1LL << 31), "ch_terma: grid too large");
     hipStream_t s = (hipStream_t)stream;
-    if (!g_status)
-        CH_HIP(hipMalloc((void**)&g_status, 64 * sizeof(int32_t)));
-    // The geometry flag covers every beam of this call.
-    hipLaunchKernelGGL(terma_prep_kernel, dim3(1), dim3(64), 0, s, nullptr,
-                       g_status);
+    // The geometry flag is per stream and covers every beam of this call.
+    void* status = nullptr;
+    ch_status sst = stream_scratch(s, kScratchStatus, 256, &status);
+    if (sst != CH_OK)
+        return sst;
+    hipLaunchKernelGGL(terma_prep_kernel, dim3(1), dim3(64), 0, s,
+                       (int32_t*)status);
     for (int32_t b = 0; b < n_beams; ++b) {
         const ch_beam& B = h_beams[b];
         const ch_aperture& A = B.aperture;
@@ -426,8 +420,9 @@ ch_status ch_terma(int32_t n_beams, const ch_beam* h_beams,
             P.ap_bbox[q] = A.bbox[q];
         P.density = B.d_density;
         P.terma = B.d_terma;
+        P.accumulate = B.accumulate;
         P.debug = d_debug ? d_debug + (size_t)b * 4 * nv : nullptr;
-        P.status = g_status;
+        P.status = (int32_t*)status;
         const unsigned blocks = (unsigned)((nv + 255) / 256);
         P.box = nullptr;
         if (B.d_box) {
@@ -443,7 +438,8 @@ ch_status ch_terma(int32_t n_beams, const ch_beam* h_beams,
         hipLaunchKernelGGL(terma_kernel, dim3(blocks), dim3(256), 0, s, P);
         CH_HIP(hipGetLastError());
         if (B.d_box) {
-            launch_box_finalize(P.box, (int)(blocks * 4), B.d_box, s);
+            launch_box_finalize(P.box, (int)(blocks * 4), B.d_box,
+                                B.accumulate ? 1 : 0, s);
             CH_HIP(hipGetLastError());
         }
     }
@@ -474,10 +470,12 @@ ch_status ch_terma_rays(const ch_beam* h_beam, const int32_t h_shape[3],
 
 ch_status ch_terma_check(void* stream)
 {
-    if (!g_status)
+    const void* status =
+        stream_scratch_peek((hipStream_t)stream, kScratchStatus);
+    if (!status)
         return CH_OK;
     int32_t flag = 0;
-    CH_HIP(hipMemcpyAsync(&flag, g_status, sizeof flag,
+    CH_HIP(hipMemcpyAsync(&flag, status, sizeof flag,
                           hipMemcpyDeviceToHost, (hipStream_t)stream));
     CH_HIP(hipStreamSynchronize((hipStream_t)stream));
     if (flag) {

@@ -153,13 +153,17 @@ def _ellipsoid(shape, spacing, rng):
     return q <= 1.0
 
 
-def uneven(n_phantoms=32, seed=7, max_beams=9):
+def uneven(n_phantoms=32, seed=7, max_beams=9, xy=(64, 200), z=(64, 160),
+           spacing=(0.2, 0.5), n_theta=12, n_phis=(4, 6, 8, 12)):
+    """Phantoms of differing grids, voxel sizes, beam counts and cone sets
+    (dims drawn from `xy` and `z`, inclusive)."""
     rng = np.random.default_rng(seed)
     cases = []
     for p in range(n_phantoms):
-        shape = (int(rng.integers(64, 201)), int(rng.integers(64, 201)),
-                 int(rng.integers(64, 161)))
-        sp = float(rng.uniform(0.2, 0.5))
+        shape = (int(rng.integers(xy[0], xy[1] + 1)),
+                 int(rng.integers(xy[0], xy[1] + 1)),
+                 int(rng.integers(z[0], z[1] + 1)))
+        sp = float(rng.uniform(*spacing))
         dens = np.ones(shape, np.float32)
         for _ in range(int(rng.integers(1, 4))):
             dens[_ellipsoid(shape, sp, rng)] = rng.uniform(0.2, 0.35)
@@ -170,7 +174,7 @@ def uneven(n_phantoms=32, seed=7, max_beams=9):
         beams = [(source_at(float(rng.uniform(0, 360))),
                   square_block(float(rng.uniform(4.0, 15.0))), SETTINGS)
                  for _ in range(nb)]
-        n_phi = int(rng.choice([4, 6, 8, 12]))
+        n_phi = int(rng.choice(n_phis))
         cases.append(Case("uneven%02d" % p, dens, origin, (sp,) * 3, beams,
-                          cone_set(12, n_phi)))
+                          cone_set(n_theta, n_phi)))
     return cases

@@ -210,6 +210,8 @@ typedef struct ch_beam {
     const float* d_density;     /* [X][Y][Z] densities on the dose grid */
     float* d_terma;             /* output [X][Y][Z] */
     int32_t* d_box;             /* output support box (6 ints), nullable */
+    int32_t accumulate;         /* 1: d_terma += this beam, d_box widened */
+    int32_t reserved;
 } ch_beam;
 
 /* TERMA for n beams on one dose grid in one launch: hit test, FP64-DDA

@@ -1,0 +1,115 @@
+"""Batched engine: many phantoms, beams and cone sets in one gather launch.
+
+The gather has no atomics and a fixed per-voxel summation order, so a
+request's dose must be bitwise the same alone, inside any batch and in any
+batch order; the plan-dose mode (TERMA summed, one convolution) must equal
+the sum of the per-beam doses to FP32 rounding; one case is checked against
+the FP64 oracle chain (oracle TERMA -> oracle convolve_c) at 1e-4 of Dmax.
+"""
+import numpy as np
+import pytest
+
+from conehead_amd import engine as E
+from conehead_amd import workloads as W
+from conehead_amd.batch import BatchEngine, DoseRequest
+from conehead_amd.device import DeviceArray, Stream
+
+pytestmark = pytest.mark.gpu
+
+
+def small_cases():
+    return W.uneven(n_phantoms=5, seed=11, max_beams=3, xy=(16, 36),
+                    z=(14, 40), spacing=(0.4, 0.8), n_theta=6,
+                    n_phis=(4, 6))
+
+
+@pytest.fixture(scope="module")
+def be(engine):
+    return BatchEngine(engine)
+
+
+def test_batch_equals_alone(be):
+    cases = small_cases()
+    reqs = [DoseRequest.from_case(c) for c in cases]
+    s = Stream()
+    together = be.run(reqs, s)
+    alone = [be.run([r], s)[0] for r in reqs]
+    backwards = be.run(reqs[::-1], s)[::-1]
+    for i, (a, b, c) in enumerate(zip(together, alone, backwards)):
+        assert a.shape == (len(cases[i].beams),) + cases[i].shape
+        assert np.array_equal(a, b), i
+        assert np.array_equal(a, c), i
+        assert np.isfinite(a).all() and a.max() > 0
+
+
+def test_resident_densities(be):
+    c = small_cases()[0]
+    s = Stream()
+    ref = be.run([DoseRequest.from_case(c)], s)[0]
+    r1 = be.run([DoseRequest.from_case(c, key="ph0")], s)[0]
+    r2 = be.run([DoseRequest.from_case(c, key="ph0")], s)[0]
+    be.evict("ph0")
+    assert np.array_equal(ref, r1) and np.array_equal(ref, r2)
+
+
+def test_plan_dose_is_sum_of_beams(be):
+    cases = [c for c in small_cases() if len(c.beams) >= 2]
+    assert cases
+    c = cases[0]
+    s = Stream()
+    per_beam = be.run([DoseRequest.from_case(c, per_beam=True)], s)[0]
+    plan = be.run([DoseRequest.from_case(c, per_beam=False)], s)[0]
+    assert plan.shape == (1,) + c.shape
+    want = per_beam.astype(np.float64).sum(0)
+    err = np.abs(plan[0] - want).max() / want.max()
+    assert err < 1e-5, err
+
+
+def test_terma_accumulate(engine):
+    """ch_terma accumulate mode: T += beam and the box widens to the
+    union; equals the separately computed beams summed."""
+    c = [c for c in small_cases() if len(c.beams) >= 2][0]
+    s = Stream()
+    rho = DeviceArray.from_host(c.densities)
+    singles, boxes = [], []
+    for src, blk, st in c.beams:
+        t = DeviceArray(c.shape)
+        box = DeviceArray((8,), np.int32)
+        b = E.beam_struct(src, st, engine.aperture(blk, s), rho, t, box)
+        E.terma([b], c.shape, c.origin, c.spacing, stream=s)
+        singles.append(t.download(stream=s))
+        boxes.append(box.download(stream=s)[:6])
+    t = DeviceArray(c.shape)
+    box = DeviceArray((8,), np.int32)
+    bs = []
+    for i, (src, blk, st) in enumerate(c.beams):
+        b = E.beam_struct(src, st, engine.aperture(blk, s), rho, t, box)
+        b.accumulate = 1 if i else 0
+        bs.append(b)
+    E.terma(bs, c.shape, c.origin, c.spacing, stream=s)
+    acc = t.download(stream=s)
+    got_box = box.download(stream=s)[:6]
+    want = singles[0].copy()
+    for x in singles[1:]:
+        want = want + x          # the same FP32 additions, in beam order
+    assert np.array_equal(acc, want)
+    bx = np.array(boxes)
+    assert list(got_box[:3]) == list(bx[:, :3].min(0))
+    assert list(got_box[3:]) == list(bx[:, 3:].max(0))
+
+
+def test_batch_vs_oracle_chain(be, oracle):
+    c = small_cases()[1]
+    dose = be.run([DoseRequest.from_case(c)], Stream())[0]
+    e, w, mu = E.spectrum(W.SETTINGS)
+    for b, (src, blk, st) in enumerate(c.beams):
+        rb = oracle.make_beam(src.position, src.v_y, src.transform, src.sad,
+                              st, e, w, mu)
+        t = oracle.terma(rb, c.shape, c.origin, c.spacing, c.densities,
+                         blk.block_values, n_threads=8)["terma"]
+        ref = np.zeros(c.shape)
+        oracle.convolve_dirs(t, ref, c.spacing, c.cones.dirs, c.cones.rows,
+                             c.cones.cum, n_threads=8)
+        err = np.abs(dose[b] - ref).max() / ref.max()
+        assert err < 1e-4, (b, err)
+

@@ -1,0 +1,81 @@
+"""Host logic of the batched path and the request loop (no GPU calls):
+request accounting, HBM-budget admission and Poisson arrivals."""
+from concurrent.futures import Future
+
+import numpy as np
+import pytest
+
+from conehead_amd import workloads as W
+from conehead_amd.batch import DoseRequest
+from conehead_amd.serve import AdmissionQueue, poisson_arrivals
+
+
+class FakeRequest:
+    def __init__(self, nbytes, n_jobs=1):
+        self.nbytes = nbytes
+        self.n_jobs = n_jobs
+
+    def device_bytes(self):
+        return self.nbytes
+
+
+def item(nbytes, n_jobs=1):
+    return (FakeRequest(nbytes, n_jobs), Future(), 0.0)
+
+
+def test_request_accounting():
+    c = W.uneven(n_phantoms=1, seed=11, max_beams=3, xy=(16, 36),
+                 z=(14, 40), n_theta=6, n_phis=(4,))[0]
+    r = DoseRequest.from_case(c)
+    X, Y, Z = c.shape
+    assert r.n_jobs == len(c.beams)
+    assert DoseRequest.from_case(c, per_beam=False).n_jobs == 1
+    padded = X * (Y + 16) * (Z + 128)
+    assert r.device_bytes() == 4 * (X * Y * Z + r.n_jobs *
+                                    (2 * X * Y * Z + padded + 8))
+    assert r.voxel_cone_updates() == X * Y * Z * c.cones.n_cones * \
+        len(c.beams)
+
+
+def test_admission_fifo_and_budget():
+    q = AdmissionQueue(100, max_batch_jobs=4)
+    items = [item(30), item(30), item(30), item(30)]
+    for it in items:
+        assert q.push(it)
+    b1 = q.take_batch()
+    assert [x is y for x, y in zip(b1, items[:3])] == [True] * 3
+    assert q.in_use == 90 and len(q) == 1
+    assert q.take_batch() == []           # 90 + 30 > 100: waits
+    q.release(b1)
+    assert q.in_use == 0
+    b2 = q.take_batch()
+    assert b2 == [items[3]] and q.in_use == 30
+
+
+def test_admission_rejects_oversized():
+    q = AdmissionQueue(100)
+    it = item(101)
+    assert not q.push(it)
+    with pytest.raises(MemoryError):
+        it[1].result(timeout=0)
+    assert len(q) == 0
+
+
+def test_admission_job_limit_and_order():
+    q = AdmissionQueue(10 ** 9, max_batch_jobs=4)
+    a, b, c = item(1, 3), item(1, 2), item(1, 1)
+    for it in (a, b, c):
+        q.push(it)
+    assert q.take_batch() == [a]          # a + b would be 5 jobs
+    assert q.take_batch() == [b, c]       # FIFO: c never overtakes b
+    big = item(1, 9)                      # alone it may exceed the limit
+    q.push(big)
+    assert q.take_batch() == [big]
+
+
+def test_poisson_arrivals():
+    t = poisson_arrivals(200.0, 20000, seed=3)
+    assert np.all(np.diff(t) > 0)
+    gaps = np.diff(np.concatenate([[0.0], t]))
+    assert abs(gaps.mean() - 1 / 200.0) < 0.05 / 200.0
+    assert abs(gaps.std() / gaps.mean() - 1.0) < 0.05   # exponential CV=1
