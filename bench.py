@@ -1,19 +1,27 @@
 """Benchmark: beams/s of the collapsed-cone dose engine on one MI355X.
 
-Workload (default): the heterogeneous single beam -- 128^3 voxels of 2.5 mm
-with a bone slab (depth 3-5 cm, rho 1.85) and a lung slab (5-12 cm, rho
-0.25) plus N(0, 0.02^2) density noise (seed 1234), 6 MV point source at SAD
-100 cm, 10 x 10 cm field, 12 x 12 = 144 cones of the varian_6 kernel.
+Default workload (`value`): the heterogeneous single beam -- 128^3 voxels of
+2.5 mm with a bone slab (depth 3-5 cm, rho 1.85) and a lung slab (5-12 cm,
+rho 0.25) plus N(0, 0.02^2) density noise (seed 1234), 6 MV point source at
+SAD 100 cm, 10 x 10 cm field, 12 x 12 = 144 cones of the varian_6 kernel.
 One step = one beam: fused TERMA + support box + collapsed-cone gather, with
-densities, aperture raster and cone plan resident in HBM.
+densities, aperture and cone plan resident in HBM.
 
-Printed JSON line: value = beams/s; plus voxel-cone updates/s, DDA-step
-updates/s, per-step latency p50/p99, the max dose error against the FP64
-oracle (oracle TERMA -> oracle convolve_c) in units of Dmax, a roofline
-object for the gather kernel (12 B per voxel-cone update: FP32 TERMA read +
-FP32 dose read-modify-write) timed with HIP events on its own stream, and
-the CPU baseline: the FP64 oracle timed on this host's cores for the same
-beam.
+The same JSON line carries
+  roofline      the gather kernel: 12 B per voxel-cone update (FP32 TERMA
+                read + FP32 dose read-modify-write) over its launch time,
+                measured with HIP events on the stream it runs on;
+  cpu_baseline  the FP64 oracle (TERMA + convolve_c) timed on this host's
+                cores for the same beam;
+  batched       64 MLC beams at random gantry angles on the same phantom,
+                96 cones, all beams' gathers in one launch;
+  plain         the plain single-beam GPU path (per-stage synchronisation
+                and host round trips, scatter convolution with atomics) on
+                beams of the batched workload, and the batched/plain ratio.
+--workload uneven   32 phantoms of differing grids, voxel sizes, beam counts
+                    and cone sets in one batch (autotuned tiles).
+--workload serve    Poisson request streams drawn from the uneven workload,
+                    lambda swept 10-500 req/s, through the request loop.
 """
 import argparse
 import json
@@ -38,15 +46,24 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=50)
     p.add_argument("--warmup", type=int, default=5)
+    p.add_argument("--workload", default="hetero",
+                   choices=["hetero", "uneven", "serve"])
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--no-extras", action="store_true",
+                   help="skip the batched and plain sections")
     p.add_argument("--cpu-threads", type=int, default=0,
                    help="oracle threads (default: min(cpus, 16))")
     p.add_argument("--tile", default="auto",
                    help="gather tile TXxTYxTZ or 'auto' (autotune)")
+    p.add_argument("--plain-beams", type=int, default=4)
     p.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                    help="first run one device copy kernel of MIB MiB (the "
                         "known-size reference for PMC byte counters)")
     return p.parse_args()
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
 
 
 def traffic_from_profile():
@@ -62,26 +79,39 @@ def traffic_from_profile():
         return None
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print("bench.py runs on exactly one GPU: this build has no "
-              "multi-GPU path (--gpus %d requested)" % args.gpus,
-              file=sys.stderr)
-        return 2
+def cpu_threads(args):
+    return args.cpu_threads or min(os.cpu_count() or 1, 16)
 
+
+def oracle_beam(case, b, threads):
+    """FP64 oracle dose of beam b of a case; returns (dose, terma, updates,
+    seconds)."""
+    from conehead_amd import engine as E
+    from oracle import ref
+    src, blk, st = case.beams[b]
+    e, w, mu = E.spectrum(st)
+    rb = ref.make_beam(src.position, src.v_y, src.transform, src.sad, st, e,
+                       w, mu)
+    c0 = time.perf_counter()
+    out = ref.terma(rb, case.shape, case.origin, case.spacing,
+                    case.densities, blk.block_values, n_threads=threads)
+    dose = np.zeros(case.shape)
+    updates = ref.convolve_dirs(out["terma"], dose, case.spacing,
+                                case.cones.dirs, case.cones.rows,
+                                case.cones.cum, n_threads=threads)
+    return dose, out["terma"], updates, time.perf_counter() - c0
+
+
+def err_over_dmax(got, ref):
+    return float(np.abs(got - ref).max() / ref.max())
+
+
+# --------------------------------------------------------------- c2 ------
+def run_hetero(args, eng):
     from conehead_amd import _lib, autotune, engine as E
     from conehead_amd.device import DeviceArray, Event, Stream
     from conehead_amd.workloads import hetero
 
-    eng = E.Engine()
-    if args.calibrate_copy:
-        n = args.calibrate_copy << 20
-        a = DeviceArray((n,), np.uint8, zero=True)
-        b = DeviceArray((n,), np.uint8)
-        _lib.check(_lib.lib().ch_memcpy_d2d(b.ptr, a.ptr, n, None))
-        _lib.check(_lib.lib().ch_device_sync())
-        del a, b
     case = hetero()
     shape, nv = case.shape, case.n_voxels
     n_cones = case.cones.n_cones
@@ -104,13 +134,10 @@ def main():
         tile = tuple(int(v) for v in args.tile.split("x"))
         plan.set_tile(*tile)
 
-    def step():
+    for _ in range(args.warmup):
         E.terma([beam], shape, case.origin, case.spacing, stream=s,
                 check=False)
         E.convolve(plan, d_t, d_d, d_box, stream=s)
-
-    for _ in range(args.warmup):
-        step()
     s.sync()
 
     # timed region: K back-to-back steps, events at every step boundary and
@@ -135,60 +162,33 @@ def main():
     step_ms = np.array([ev[i].elapsed_ms(ev[i + 1]) for i in range(K)])
     gather_ms = np.array([g0[i].elapsed_ms(g1[i]) for i in range(K)])
     gpu_total_ms = ev[0].elapsed_ms(ev[K])
-    ms_per_step = wall * 1e3 / K
-    beams_per_s = K / wall
     dose = d_d.download(stream=s)
     terma_gpu = d_t.download(stream=s)
 
-    # CPU baseline and parity: the FP64 oracle on this host for the beam
-    result_err = None
-    cpu = None
+    result_err, cpu = None, None
     if not args.no_cpu_baseline:
-        from oracle import ref
-        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
-        e, w, mu = E.spectrum(st)
-        rb = ref.make_beam(src.position, src.v_y, src.transform, src.sad,
-                           st, e, w, mu)
-        c0 = time.perf_counter()
-        out = ref.terma(rb, shape, case.origin, case.spacing, case.densities,
-                        blk.block_values, n_threads=threads)
-        ref_dose = np.zeros(shape)
-        updates = ref.convolve_dirs(out["terma"], ref_dose, case.spacing,
-                                    case.cones.dirs, case.cones.rows,
-                                    case.cones.cum, n_threads=threads)
-        cpu_s = time.perf_counter() - c0
-        dmax = float(ref_dose.max())
+        threads = cpu_threads(args)
+        log("cpu baseline: oracle on %d threads" % threads)
+        ref_dose, ref_t, updates, cpu_s = oracle_beam(case, 0, threads)
         result_err = {
-            "dose_max_abs_err_over_dmax":
-                float(np.abs(dose - ref_dose).max() / dmax),
-            "terma_max_abs_err_over_tmax":
-                float(np.abs(terma_gpu - out["terma"]).max() /
-                      out["terma"].max()),
+            "dose_max_abs_err_over_dmax": err_over_dmax(dose, ref_dose),
+            "terma_max_abs_err_over_tmax": err_over_dmax(terma_gpu, ref_t),
         }
         cpu = {"value": 1.0 / cpu_s, "unit": "beams/s", "cores": threads,
                "kind": "port",
                "sample": "one full beam of this workload: FP64 TERMA + "
                          "convolve_c scatter over all %d T!=0 voxels x %d "
                          "cones (%d dose += T*k updates), %.2f s" %
-                         (int(np.count_nonzero(out["terma"])), n_cones,
-                          updates, cpu_s),
+                         (int(np.count_nonzero(ref_t)), n_cones, updates,
+                          cpu_s),
                "dda_step_updates": int(updates)}
 
     g_ms = float(gather_ms.mean())
     achieved = BYTES_PER_VOXEL_CONE * nv * n_cones / (g_ms * 1e-3) / 1e9
-    line = {
-        "metric": METRIC,
+    beams_per_s = K / wall
+    return {
         "value": beams_per_s,
-        "unit": "beams/s",
-        "n_gpus": 1,
-        "steps": K,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32 (f64 DDA alphas)",
-        "data": "synthetic (seeded phantom; EDKnrc varian_6 kernel)",
+        "ms_per_step": wall * 1e3 / K,
         "config": {
             "workload": "heterogeneous single beam: 128^3 @ 2.5 mm, lung "
                         "and bone slabs + N(0,0.02^2) noise, 10x10 cm field, "
@@ -214,8 +214,221 @@ def main():
             "traffic": traffic_from_profile(),
         },
         "cpu_baseline": cpu,
-        "device": _lib.device_info(),
     }
+
+
+# --------------------------------------------------------------- c3 ------
+def run_batched(args, eng, steps=5):
+    from conehead_amd.batch import BatchEngine, DoseRequest
+    from conehead_amd.device import Event, Stream
+    from conehead_amd.workloads import batched
+
+    case = batched()
+    be = BatchEngine(eng)
+    req = DoseRequest.from_case(case, key="batched-phantom")
+    s = Stream()
+    be.release(be.collect(be.launch([req], s), download=False))  # warm-up
+    times, gpu = [], []
+    out = None
+    for i in range(steps):
+        t0 = time.perf_counter()
+        e0 = Event().record(s)
+        h = be.launch([req], s)
+        e1 = Event().record(s)
+        e1.sync()
+        times.append((time.perf_counter() - t0) * 1e3)
+        gpu.append(e0.elapsed_ms(e1))
+        if i == steps - 1:
+            out = be.collect(h)[0]
+        else:
+            be.release(be.collect(h, download=False))
+    ms = float(np.median(times))
+    nb = len(case.beams)
+    r = {"workload": "64 MLC beams (gantry U[0,360), seed 2024, 60 leaf "
+                     "pairs x 5 mm, apertures 2-20 cm) on the 128^3 "
+                     "heterogeneous phantom, 96 cones, one gather launch",
+         "ms_per_batch": ms, "gpu_ms_per_batch": float(np.median(gpu)),
+         "beams_per_s": nb / (ms * 1e-3),
+         "voxel_cone_updates_per_s":
+             case.n_voxels * case.cones.n_cones * nb / (ms * 1e-3)}
+    if not args.no_cpu_baseline:
+        ref, _, _, cpu_s = oracle_beam(case, 0, cpu_threads(args))
+        r["error_vs_fp64_oracle_beam0"] = err_over_dmax(out[0], ref)
+    return r, case
+
+
+def run_plain(args, case, n_beams):
+    from conehead_amd.plain import plain_case
+    plain_case(case, 0)                                   # warm-up
+    t0 = time.perf_counter()
+    doses = [plain_case(case, b)[0] for b in range(n_beams)]
+    dt = time.perf_counter() - t0
+    return {"workload": "beams 0..%d of the batched workload, one at a "
+                        "time through the plain path" % (n_beams - 1),
+            "beams": n_beams, "beams_per_s": n_beams / dt,
+            "ms_per_beam": dt * 1e3 / n_beams}, doses
+
+
+# --------------------------------------------------------------- c4 ------
+def uneven_requests(per_beam=True):
+    from conehead_amd.batch import DoseRequest
+    from conehead_amd.workloads import uneven
+    cases = uneven()
+    return cases, [DoseRequest.from_case(c, per_beam, key=c.name)
+                   for c in cases]
+
+
+def tune_requests(eng, be, reqs, s):
+    """Autotune the tile of every distinct plan on its first beam's TERMA
+    support (cached per plan key)."""
+    from conehead_amd import autotune
+    from conehead_amd import engine as E
+    from conehead_amd.device import DeviceArray
+    seen = {}
+    for r in reqs:
+        plan = eng.plan(r.shape, r.spacing, r.cones)
+        if plan.handle in seen:
+            continue
+        d_rho, _ = be.densities(r, s)
+        t = DeviceArray(r.shape)
+        box = DeviceArray((8,), np.int32)
+        src, blk, st = r.beams[0]
+        E.terma([E.beam_struct(src, st, eng.aperture(blk, s), d_rho, t,
+                               box)], r.shape, r.origin, r.spacing, stream=s)
+        b = box.download(stream=s)[:6]
+        seen[plan.handle] = autotune.tune(plan, box=tuple(int(v) for v in b),
+                                          stream=s, use_cache=False)[0]
+    return seen
+
+
+def run_uneven(args, eng):
+    from conehead_amd.batch import BatchEngine
+    from conehead_amd.device import Event, Stream
+    log("uneven: generating phantoms")
+    cases, reqs = uneven_requests()
+    be = BatchEngine(eng)
+    s = Stream()
+    tiles = tune_requests(eng, be, reqs, s)
+    be.release(be.collect(be.launch(reqs, s), download=False))  # warm-up
+    times = []
+    for i in range(max(1, min(args.steps, 5))):
+        t0 = time.perf_counter()
+        h = be.launch(reqs, s)
+        Event().record(s).sync()
+        times.append((time.perf_counter() - t0) * 1e3)
+        if i == 0:
+            doses = be.collect(h)
+        else:
+            be.release(be.collect(h, download=False))
+    ms = float(np.median(times))
+    nb = sum(len(c.beams) for c in cases)
+    upd = sum(c.voxel_cone_updates() * len(c.beams) for c in cases)
+    r = {"metric": METRIC, "value": nb / (ms * 1e-3), "unit": "beams/s",
+         "n_gpus": 1, "steps": len(times), "warmup": 1,
+         "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+         "vs_baseline": None, "dtype": "f32 (f64 DDA alphas)",
+         "data": "synthetic (seeded phantoms; EDKnrc varian_6 kernel)",
+         "config": {"workload": "uneven batch: 32 phantoms (seed 7), dims "
+                                "64-200 x 64-200 x 64-160, 2-5 mm voxels, "
+                                "lung/bone ellipsoids, 1-9 beams, 48-144 "
+                                "cones, one batch",
+                    "beams": nb, "distinct_tiles": len(set(tiles.values()))},
+         "voxel_cone_updates_per_s": upd / (ms * 1e-3)}
+    if not args.no_cpu_baseline:
+        i = int(np.argmin([c.n_voxels * c.cones.n_cones for c in cases]))
+        ref, _, _, _ = oracle_beam(cases[i], 0, cpu_threads(args))
+        r["error_vs_fp64_oracle"] = {"case": cases[i].name, "beam": 0,
+                                     "dose_max_abs_err_over_dmax":
+                                         err_over_dmax(doses[i][0], ref)}
+    return r
+
+
+# --------------------------------------------------------------- c5 ------
+def run_serve(args, eng):
+    from conehead_amd.batch import BatchEngine
+    from conehead_amd.device import Stream
+    from conehead_amd.serve import DoseServer, poisson_arrivals, replay
+    log("serve: generating phantoms")
+    cases, pool = uneven_requests()
+    be = BatchEngine(eng)
+    s = Stream()
+    tune_requests(eng, be, pool, s)
+    be.release(be.collect(be.launch(pool, s), download=False))  # warm-up
+    rng = np.random.default_rng(99)
+    sweep = []
+    n_req = 64
+    for lam in (10, 20, 50, 100, 200, 500):
+        reqs = [pool[int(i)] for i in rng.integers(0, len(pool), n_req)]
+        arr = poisson_arrivals(float(lam), n_req, seed=99 + lam)
+        with DoseServer(be, max_inflight=2, max_batch_jobs=64) as srv:
+            futs = replay(srv, reqs, arr)
+            for f in futs:
+                f.result(timeout=300)
+        st = srv.stats()
+        st["lambda_req_per_s"] = lam
+        sweep.append(st)
+        log("serve lambda=%d: p50 %.1f ms p99 %.1f ms %.0f beams/s" %
+            (lam, st["latency_ms_p50"], st["latency_ms_p99"],
+             st["beams_per_s"]))
+    best = max(sweep, key=lambda d: d["beams_per_s"])
+    return {"metric": METRIC, "value": best["beams_per_s"],
+            "unit": "beams/s", "n_gpus": 1, "steps": n_req * len(sweep),
+            "warmup": 1, "ms_per_step": None, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32 (f64 DDA alphas)",
+            "data": "synthetic (seeded phantoms; EDKnrc varian_6 kernel)",
+            "config": {"workload": "serving: Poisson arrivals (seed 99), "
+                                   "64 requests per lambda drawn from the "
+                                   "uneven batch, resident densities, HBM "
+                                   "budget admission, 2 stream slots"},
+            "sweep": sweep}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py runs on exactly one GPU: this build has no "
+              "multi-GPU path (--gpus %d requested)" % args.gpus,
+              file=sys.stderr)
+        return 2
+
+    from conehead_amd import _lib
+    from conehead_amd import engine as E
+    from conehead_amd.device import DeviceArray
+
+    eng = E.Engine()
+    if args.calibrate_copy:
+        n = args.calibrate_copy << 20
+        a = DeviceArray((n,), np.uint8, zero=True)
+        b = DeviceArray((n,), np.uint8)
+        _lib.check(_lib.lib().ch_memcpy_d2d(b.ptr, a.ptr, n, None))
+        _lib.check(_lib.lib().ch_device_sync())
+        del a, b
+
+    if args.workload == "uneven":
+        line = run_uneven(args, eng)
+    elif args.workload == "serve":
+        line = run_serve(args, eng)
+    else:
+        log("hetero single beam")
+        h = run_hetero(args, eng)
+        line = {"metric": METRIC, "value": h.pop("value"),
+                "unit": "beams/s", "n_gpus": 1, "steps": args.steps,
+                "warmup": args.warmup, "ms_per_step": h.pop("ms_per_step"),
+                "higher_is_better": True, "scaling": "weak",
+                "vs_baseline": None, "dtype": "f32 (f64 DDA alphas)",
+                "data": "synthetic (seeded phantom; EDKnrc varian_6 kernel)",
+                "config": h.pop("config")}
+        line.update(h)
+        if not args.no_extras:
+            log("batched 64 beams")
+            bt, case = run_batched(args, eng)
+            log("plain path")
+            pl, _ = run_plain(args, case, args.plain_beams)
+            pl["batched_over_plain"] = bt["beams_per_s"] / pl["beams_per_s"]
+            line["batched"] = bt
+            line["plain"] = pl
+    line["device"] = _lib.device_info()
     print(json.dumps(line))
     return 0
 

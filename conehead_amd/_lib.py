@@ -147,6 +147,9 @@ SIGNATURES = {
     "ch_convolve_f64": (ctypes.c_int32, [c_vp, c_vp, c_vp, c_vp, c_vp]),
     "ch_convolve_batch": (ctypes.c_int32, [ctypes.c_int32,
                                            ctypes.POINTER(ConvJob), c_vp]),
+    "ch_convolve_scatter": (ctypes.c_int32, [
+        c_vp, c_vp, c_i32p, c_f64p, ctypes.c_int32, c_vp, c_vp, c_vp,
+        ctypes.c_int32, ctypes.c_int32, c_vp]),
     "ch_plan_set_tile": (ctypes.c_int32, [c_vp, ctypes.c_int32,
                                           ctypes.c_int32, ctypes.c_int32]),
     "ch_plan_get_tile": (ctypes.c_int32, [c_vp, c_i32p, c_i32p, c_i32p]),

@@ -1,12 +1,14 @@
-"""Row-length and accumulation autotuner for the collapsed-cone gather.
+"""Tile autotuner for the collapsed-cone gather.
 
-A wavefront owns one output row of 64 or 128 z-voxels (one or two per lane).
-Longer rows halve the per-cone set-up per output and load 8 bytes per lane
-per tap, but widen the union of the lanes' clipped tap ranges (taps that
-read zeros for part of the row).  The best choice depends on the grid, the
-cone set and the TERMA support, so it is measured: every candidate is timed
-with HIP events on the real plan and a synthetic TERMA of the given support,
-and the winner is cached per plan key in memory and in a JSON file.
+A wavefront owns one 1 x TY x TZ output tile (TY * TZ = 64 voxels).  Per
+cone it walks the union of its voxels' clipped tap ranges, so taller tiles
+(larger TY) share one set of tap loads across more y-rows but widen that
+union (taps that read zero padding for part of the tile), and the L1 sector
+reuse of neighbouring taps changes with the tile's z-extent.  The best
+choice depends on the grid, the cone set and the TERMA support, so it is
+measured: every candidate is timed with HIP events on the real plan and a
+synthetic TERMA of the given support, and the winner is cached per plan
+key in memory and in a JSON file.
 """
 import json
 import os
@@ -29,8 +31,8 @@ _memo = {}
 
 
 def default_tile(plan):
-    """Heuristic row length before (or without) tuning: two outputs per
-    lane (128-voxel rows) when the grid is deeper than 64, else one."""
+    """Tile before (or without) tuning: the last tuned tile of this grid
+    shape and cone count, else 1 x 2 x 32."""
     key = _shape_key(plan)
     if key in _memo:
         return _memo[key]

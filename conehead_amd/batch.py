@@ -204,6 +204,12 @@ class BatchEngine:
         handle.temps = []
         return results
 
+    def release(self, results):
+        """Return device outputs of collect(download=False) to the pool."""
+        for outs in results:
+            for d in outs:
+                self.pool.put(d)
+
     def run(self, requests, stream=None):
         """Launch and collect one batch."""
         return self.collect(self.launch(requests, stream))

@@ -23,9 +23,9 @@ COMMON = [
     "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument",
 ]
 # Files whose arithmetic must round every operation like the FP64 oracle.
-EXACT = {"dda.hip", "plan.hip", "terma.hip", "raster.hip"}
+EXACT = {"dda.hip", "plan.hip", "terma.hip", "raster.hip", "scatter.hip"}
 SOURCES = ["runtime.hip", "dda.hip", "plan.hip", "ccc.hip", "terma.hip",
-           "raster.hip"]
+           "raster.hip", "scatter.hip"]
 
 
 def hipcc():

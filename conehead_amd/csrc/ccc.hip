@@ -357,33 +357,6 @@ static size_t padded_elems(const ch_plan* p)
            (p->shape[2] + 2 * kZPad);
 }
 
-// Optional HIP-event timing of the gather launches (bench and profiling).
-struct LaunchTimer {
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    hipStream_t s;
-    bool on;
-    explicit LaunchTimer(hipStream_t st) : s(st), on(timing_enabled())
-    {
-        if (on) {
-            (void)hipEventCreate(&e0);
-            (void)hipEventCreate(&e1);
-            (void)hipEventRecord(e0, s);
-        }
-    }
-    void stop()
-    {
-        if (!on)
-            return;
-        (void)hipEventRecord(e1, s);
-        (void)hipEventSynchronize(e1);
-        float ms = -1.0f;
-        (void)hipEventElapsedTime(&ms, e0, e1);
-        record_kernel_ms(ms);
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-    }
-};
-
 // Per-stream workspaces: the padded TERMA copies and the batch job table.
 // Kernels on one stream run in order, so a stream's workspace is reused by
 // its next call without further synchronisation; the pinned job-table

@@ -74,6 +74,34 @@ struct ConvJob {
 void record_kernel_ms(float ms);
 bool timing_enabled();
 
+// Optional HIP-event timing of one launch (ch_set_timing): stop() waits for
+// the launch and records its duration for ch_last_kernel_ms().
+struct LaunchTimer {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipStream_t s;
+    bool on;
+    explicit LaunchTimer(hipStream_t st) : s(st), on(timing_enabled())
+    {
+        if (on) {
+            (void)hipEventCreate(&e0);
+            (void)hipEventCreate(&e1);
+            (void)hipEventRecord(e0, s);
+        }
+    }
+    void stop()
+    {
+        if (!on)
+            return;
+        (void)hipEventRecord(e1, s);
+        (void)hipEventSynchronize(e1);
+        float ms = -1.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        record_kernel_ms(ms);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
+};
+
 // Device scratch owned by the library, one buffer per (stream, slot),
 // grown on demand.  Kernels on one stream run in order, so a stream's
 // buffer can be reused by its next launch without synchronisation.

@@ -166,8 +166,23 @@ ch_status ch_plan_set_tile(ch_plan* plan, int32_t tx, int32_t ty, int32_t tz);
 ch_status ch_plan_get_tile(const ch_plan* plan, int32_t* tx, int32_t* ty,
                            int32_t* tz);
 /* Accumulation mode: 0 = Kahan per tap, 1 = plain per cone line with Kahan
- * across lines. */
+ * across lines, 2 = Kahan over blocks of 8 taps (default). */
 ch_status ch_plan_set_accum(ch_plan* plan, int32_t mode);
+
+/* Plain scatter convolution, the unbatched baseline: the loop of
+ * conehead/convolve_c.pyx:77-192 as one thread per (voxel, cone) -- voxels
+ * with T == 0 return, the others walk the cone's FP64 DDA ray from their
+ * centre and add T * k into d_dose with FP32 atomics (the structure of the
+ * reference GPU kernel cuda_dose, conehead/conehead.py:385-488).  No plan:
+ * d_dirs [n_cones][3] (discretised, as ch_plan_create takes them), d_rows
+ * [n_cones] and d_cum [rows][row_len] FP64 are device arrays.  d_dose is
+ * accumulated (+=); the summation order is nondeterministic. */
+ch_status ch_convolve_scatter(const float* d_terma, float* d_dose,
+                              const int32_t shape[3], const double vsize[3],
+                              int32_t n_cones, const double* d_dirs,
+                              const int32_t* d_rows, const double* d_cum,
+                              int32_t row_len, int32_t semantics,
+                              void* stream);
 
 /* ------------------------------------------------------------ TERMA ---- */
 
