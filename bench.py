@@ -165,6 +165,22 @@ def run_hetero(args, eng):
     dose = d_d.download(stream=s)
     terma_gpu = d_t.download(stream=s)
 
+    # the convolve_c drop-in on host float64 buffers: PCIe-inclusive rate
+    from conehead_amd.convolve_c import convolve_c
+    from conehead_amd.kernel2 import merged_kernel
+    from conehead_amd.workloads import base_kernel
+    kern = merged_kernel(base_kernel(), len(case.cones.phis))
+    t64 = terma_gpu.astype(np.float64)
+    d64 = np.zeros(shape)
+    convolve_c(t64, d64, case.spacing, case.cones.thetas, case.cones.phis,
+               kern)
+    reps = 5
+    c0 = time.perf_counter()
+    for _ in range(reps):
+        convolve_c(t64, d64, case.spacing, case.cones.thetas,
+                   case.cones.phis, kern)
+    dropin_ms = (time.perf_counter() - c0) * 1e3 / reps
+
     result_err, cpu = None, None
     if not args.no_cpu_baseline:
         threads = cpu_threads(args)
@@ -205,6 +221,7 @@ def run_hetero(args, eng):
         "gather_ms": g_ms,
         "terma_ms": float((step_ms - gather_ms).mean()),
         "error_vs_fp64_oracle": result_err,
+        "dropin_convolve_c_ms": dropin_ms,
         "roofline": {
             "bound": "hbm",
             "achieved": achieved,
@@ -227,6 +244,7 @@ def run_batched(args, eng, steps=5):
     be = BatchEngine(eng)
     req = DoseRequest.from_case(case, key="batched-phantom")
     s = Stream()
+    tile = tune_requests(eng, be, [req], s)
     be.release(be.collect(be.launch([req], s), download=False))  # warm-up
     times, gpu = [], []
     out = None
@@ -247,6 +265,7 @@ def run_batched(args, eng, steps=5):
     r = {"workload": "64 MLC beams (gantry U[0,360), seed 2024, 60 leaf "
                      "pairs x 5 mm, apertures 2-20 cm) on the 128^3 "
                      "heterogeneous phantom, 96 cones, one gather launch",
+         "gather_tile": list(list(tile.values())[0]),
          "ms_per_batch": ms, "gpu_ms_per_batch": float(np.median(gpu)),
          "beams_per_s": nb / (ms * 1e-3),
          "voxel_cone_updates_per_s":

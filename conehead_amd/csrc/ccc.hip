@@ -481,55 +481,22 @@ static ch_status convolve_one(const ch_plan* p, const float* d_terma,
 
 // ------------------------------------------------------- support box ----
 
-__device__ __forceinline__ int wave_min(int v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1)
-        v = min(v, __shfl_xor(v, o, 64));
-    return v;
-}
-
-__device__ __forceinline__ int wave_max(int v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1)
-        v = max(v, __shfl_xor(v, o, 64));
-    return v;
-}
-
 __global__ __launch_bounds__(256) void box_kernel(const float* __restrict__ T,
                                                   int X, int Y, int Z,
                                                   int32_t* box)
 {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     const int64_t n = (int64_t)X * Y * Z;
-    int mn0 = INT_MAX, mn1 = INT_MAX, mn2 = INT_MAX;
-    int mx0 = INT_MIN, mx1 = INT_MIN, mx2 = INT_MIN;
+    int32_t v[6] = {INT_MAX, INT_MAX, INT_MAX, INT_MIN, INT_MIN, INT_MIN};
     if (i < n && T[i] != 0.0f) {
         const int z = (int)(i % Z);
         const int64_t r = i / Z;
-        const int y = (int)(r % Y);
-        const int x = (int)(r / Y);
-        mn0 = mx0 = x;
-        mn1 = mx1 = y;
-        mn2 = mx2 = z;
+        v[0] = v[3] = (int)(r / Y);
+        v[1] = v[4] = (int)(r % Y);
+        v[2] = v[5] = z;
     }
-    mn0 = wave_min(mn0);
-    mn1 = wave_min(mn1);
-    mn2 = wave_min(mn2);
-    mx0 = wave_max(mx0);
-    mx1 = wave_max(mx1);
-    mx2 = wave_max(mx2);
-    // one partial box per wave; box_finalize reduces them (no atomics)
-    if ((threadIdx.x & 63) == 0) {
-        int32_t* p = box + (size_t)(i >> 6) * kBoxStride;
-        p[0] = mn0;
-        p[1] = mn1;
-        p[2] = mn2;
-        p[3] = mx0;
-        p[4] = mx1;
-        p[5] = mx2;
-    }
+    // one partial box per block; box_finalize reduces them (no atomics)
+    block_box_partial(v, box);
 }
 
 }  // namespace ch
@@ -548,7 +515,7 @@ ch_status ch_support_box(const float* d_terma, const int32_t h_shape[3],
     const unsigned blocks = (unsigned)((n + 255) / 256);
     void* parts = nullptr;
     ch_status st = stream_scratch(s, kScratchBoxPartials,
-                                  (size_t)blocks * 4 * kBoxStride *
+                                  (size_t)blocks * kBoxStride *
                                       sizeof(int32_t),
                                   &parts);
     if (st != CH_OK)
@@ -556,7 +523,7 @@ ch_status ch_support_box(const float* d_terma, const int32_t h_shape[3],
     hipLaunchKernelGGL(box_kernel, dim3(blocks), dim3(256), 0, s, d_terma,
                        h_shape[0], h_shape[1], h_shape[2], (int32_t*)parts);
     CH_HIP(hipGetLastError());
-    launch_box_finalize((const int32_t*)parts, (int)(blocks * 4), d_box, 0,
+    launch_box_finalize((const int32_t*)parts, (int)blocks, d_box, 0,
                         s);
     CH_HIP(hipGetLastError());
     return CH_OK;

@@ -106,7 +106,7 @@ struct LaunchTimer {
 // grown on demand.  Kernels on one stream run in order, so a stream's
 // buffer can be reused by its next launch without synchronisation.
 enum ScratchSlot {
-    kScratchBoxPartials = 0,  // per-wave partial support boxes
+    kScratchBoxPartials = 0,  // per-block partial support boxes
     kScratchStatus = 1,       // TERMA geometry flag (one int32)
     kScratchSlots = 2
 };
@@ -117,12 +117,43 @@ void* stream_scratch_peek(hipStream_t s, int slot);
 void release_stream_scratch(hipStream_t s);
 void release_gather_workspace(hipStream_t s);
 
-// Support box of T != 0 from per-wave partial boxes: kernels write
-// kBoxStride ints per wave (min x, y, z, max x, y, z) to `partials`, then
+// Support box of T != 0 from per-block partial boxes: kernels write
+// kBoxStride ints per block (min x, y, z, max x, y, z) to `partials`, then
 // box_finalize reduces n_parts of them into box[0..5] (lo > hi if empty).
 constexpr int kBoxStride = 8;
 void launch_box_finalize(const int32_t* partials, int n_parts, int32_t* box,
                          int widen, hipStream_t s);
+
+// Reduces each thread's candidate box (lo > hi when it has none) over its
+// block of 256 threads and writes the block's partial box to
+// parts[blockIdx.x * kBoxStride + 0..5].  Every thread of the block must
+// call it.
+__device__ __forceinline__ void block_box_partial(int32_t v[6],
+                                                  int32_t* parts)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            v[a] = min(v[a], __shfl_xor(v[a], o, 64));
+            v[3 + a] = max(v[3 + a], __shfl_xor(v[3 + a], o, 64));
+        }
+    __shared__ int32_t s[4][6];
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int a = 0; a < 6; ++a)
+            s[w][a] = v[a];
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int a = threadIdx.x;
+        int32_t r = s[0][a];
+#pragma unroll
+        for (int k = 1; k < 4; ++k)
+            r = a < 3 ? min(r, s[k][a]) : max(r, s[k][a]);
+        parts[(size_t)blockIdx.x * kBoxStride + a] = r;
+    }
+}
 
 }  // namespace ch
 

@@ -77,9 +77,9 @@ __global__ void copy_kernel(uint8_t* __restrict__ dst,
         dst[j] = src[j];
 }
 
-// Reduces per-wave partial boxes (one block, grid-stride, then across
-// the block's waves through LDS).
-__global__ __launch_bounds__(256) void box_finalize_kernel(
+// Reduces the partial boxes (one block of 1024 threads, grid-stride, then
+// across the block's waves through LDS).
+__global__ __launch_bounds__(1024) void box_finalize_kernel(
     const int32_t* __restrict__ parts, int n, int32_t* __restrict__ box,
     int widen)
 {
@@ -96,7 +96,7 @@ __global__ __launch_bounds__(256) void box_finalize_kernel(
             v[a] = min(v[a], __shfl_xor(v[a], o, 64));
             v[3 + a] = max(v[3 + a], __shfl_xor(v[3 + a], o, 64));
         }
-    __shared__ int s[4][6];
+    __shared__ int s[16][6];
     const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0)
         for (int a = 0; a < 6; ++a)
@@ -116,7 +116,7 @@ __global__ __launch_bounds__(256) void box_finalize_kernel(
 void launch_box_finalize(const int32_t* partials, int n_parts, int32_t* box,
                          int widen, hipStream_t s)
 {
-    hipLaunchKernelGGL(box_finalize_kernel, dim3(1), dim3(256), 0, s,
+    hipLaunchKernelGGL(box_finalize_kernel, dim3(1), dim3(1024), 0, s,
                        partials, n_parts, box, widen);
 }
 

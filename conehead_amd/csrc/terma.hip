@@ -74,22 +74,6 @@ __device__ __forceinline__ float transmission(const TermaParams& P, double px,
     return P.raster[i * 4000 + j];
 }
 
-__device__ __forceinline__ int wmin(int v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1)
-        v = min(v, __shfl_xor(v, o, 64));
-    return v;
-}
-
-__device__ __forceinline__ int wmax(int v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1)
-        v = max(v, __shfl_xor(v, o, 64));
-    return v;
-}
-
 // Voxel centre and unit direction toward the source (conehead.py:226-238),
 // FP64 without contraction.
 __device__ __forceinline__ void ray_to_source(const TermaParams& P, int x,
@@ -299,29 +283,14 @@ __global__ __launch_bounds__(256) void terma_kernel(TermaParams P)
             atomicOr(P.status, 1);
     }
     if (P.box) {
-        int mn0 = INT_MAX, mn1 = INT_MAX, mn2 = INT_MAX;
-        int mx0 = INT_MIN, mx1 = INT_MIN, mx2 = INT_MIN;
+        // one partial box per block; box_finalize reduces them (no atomics)
+        int32_t v[6] = {INT_MAX, INT_MAX, INT_MAX, INT_MIN, INT_MIN, INT_MIN};
         if (inside && T != 0.0f) {
-            mn0 = mx0 = x;
-            mn1 = mx1 = y;
-            mn2 = mx2 = z;
+            v[0] = v[3] = x;
+            v[1] = v[4] = y;
+            v[2] = v[5] = z;
         }
-        mn0 = wmin(mn0);
-        mn1 = wmin(mn1);
-        mn2 = wmin(mn2);
-        mx0 = wmax(mx0);
-        mx1 = wmax(mx1);
-        mx2 = wmax(mx2);
-        // one partial box per wave; box_finalize reduces them (no atomics)
-        if ((threadIdx.x & 63) == 0) {
-            int32_t* p = P.box + (size_t)(li >> 6) * kBoxStride;
-            p[0] = mn0;
-            p[1] = mn1;
-            p[2] = mn2;
-            p[3] = mx0;
-            p[4] = mx1;
-            p[5] = mx2;
-        }
+        block_box_partial(v, P.box);
     }
 }
 
@@ -428,7 +397,7 @@ ch_status ch_terma(int32_t n_beams, const ch_beam* h_beams,
         if (B.d_box) {
             void* parts = nullptr;
             ch_status st = stream_scratch(s, kScratchBoxPartials,
-                                          (size_t)blocks * 4 * kBoxStride *
+                                          (size_t)blocks * kBoxStride *
                                               sizeof(int32_t),
                                           &parts);
             if (st != CH_OK)
@@ -438,7 +407,7 @@ ch_status ch_terma(int32_t n_beams, const ch_beam* h_beams,
         hipLaunchKernelGGL(terma_kernel, dim3(blocks), dim3(256), 0, s, P);
         CH_HIP(hipGetLastError());
         if (B.d_box) {
-            launch_box_finalize(P.box, (int)(blocks * 4), B.d_box,
+            launch_box_finalize(P.box, (int)blocks, B.d_box,
                                 B.accumulate ? 1 : 0, s);
             CH_HIP(hipGetLastError());
         }

@@ -146,9 +146,11 @@ ch_status ch_convolve(const ch_plan* plan, const float* d_terma,
 ch_status ch_convolve_f64(const ch_plan* plan, const float* d_terma,
                           double* d_dose, const int32_t* d_box, void* stream);
 
-/* One persistent launch over many (terma, dose) jobs that may use different
- * plans (grid shapes, voxel sizes, cone sets): workgroups pull tiles from a
- * device work counter until every job is done. */
+/* One gather launch over many (terma, dose) jobs that may use different
+ * plans (grid shapes, voxel sizes, cone sets): the grid covers the tiles of
+ * every job and each wavefront finds its job by binary search over the
+ * jobs' first tiles.  Jobs whose plans use different tile shapes or
+ * accumulation modes go into one launch per (tile, mode). */
 typedef struct ch_conv_job {
     const ch_plan* plan;
     const float* d_terma;
@@ -229,7 +231,8 @@ typedef struct ch_beam {
     int32_t reserved;
 } ch_beam;
 
-/* TERMA for n beams on one dose grid in one launch: hit test, FP64-DDA
+/* TERMA for n beams on one dose grid, queued in order on `stream` (one
+ * fused launch per beam, so accumulating beams never race): hit test, FP64-DDA
  * radiological depth toward the source, off-axis distance, three-source
  * fluence, softening, horn and the polyenergetic TERMA sum.
  * Replaces the kernel chain cuda_hit_test, cuda_d_eff, cuda_oad,
@@ -283,8 +286,9 @@ ch_status ch_mlc_table(int32_t* d_table, int32_t n_pairs,
 ch_status ch_raster_mlc(float* d_raster, const int32_t* d_table,
                         void* stream);
 
-/* Kernel time of the last gather launch issued by this thread on `stream`,
- * measured with HIP events around the launch when timing is enabled. */
+/* Kernel time of the last gather or scatter launch, measured with HIP
+ * events around the launch when timing is enabled (the call then waits for
+ * the launch). */
 ch_status ch_set_timing(int32_t enabled);
 ch_status ch_last_kernel_ms(float* ms);
 
