@@ -9,7 +9,9 @@ import os
 import threading
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libconehead_amd.so")
+# CONEHEAD_AMD_LIB selects another build of the library (same C-ABI).
+LIB_PATH = os.environ.get("CONEHEAD_AMD_LIB",
+                          os.path.join(HERE, "libconehead_amd.so"))
 
 CH_OK = 0
 CH_ERR_ARG = -1

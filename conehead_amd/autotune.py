@@ -1,10 +1,12 @@
 """Tile autotuner for the collapsed-cone gather.
 
-A wavefront owns one 1 x TY x TZ output tile (TY * TZ = 64 voxels).  Per
-cone it walks the union of its voxels' clipped tap ranges, so taller tiles
-(larger TY) share one set of tap loads across more y-rows but widen that
-union (taps that read zero padding for part of the tile), and the L1 sector
-reuse of neighbouring taps changes with the tile's z-extent.  The best
+Two gathers compete.  In the direct one a wavefront owns a 1 x TY x TZ
+output tile (TY * TZ = 64 voxels) and walks the union of its voxels'
+clipped tap ranges with one global load per tap, so taller tiles share tap
+loads across more y-rows but widen that union.  In the staged one
+(4 x 16 x 16) a workgroup stages the TERMA box of each chunk of taps in LDS
+and reads the taps from there, trading global loads for LDS traffic and a
+wider union.  The best
 choice depends on the grid, the cone set and the TERMA support, so it is
 measured: every candidate is timed with HIP events on the real plan and a
 synthetic TERMA of the given support, and the winner is cached per plan
@@ -19,7 +21,7 @@ import numpy as np
 from . import _lib
 from .device import DeviceArray, Event, Stream
 
-CANDIDATES = [(1, 1, 64), (1, 2, 32), (1, 4, 16), (1, 8, 8)]
+CANDIDATES = [(1, 1, 64), (1, 2, 32), (1, 4, 16), (1, 8, 8), (4, 16, 16)]
 
 CACHE_PATH = os.environ.get(
     "CONEHEAD_AMD_TUNE_CACHE",

@@ -50,11 +50,14 @@ __device__ __forceinline__ ConvJob load_job(const c_job* p)
     j.k = p->k;
     j.ftab = p->ftab;
     j.meta = p->meta;
+    j.off = p->off;
+    j.lin2 = p->lin2;
     j.X = p->X;
     j.Y = p->Y;
     j.Z = p->Z;
     j.n_cones = p->n_cones;
     j.accumulate = p->accumulate;
+    j.ntx = p->ntx;
     j.nty = p->nty;
     j.ntz = p->ntz;
     j.tile_begin = p->tile_begin;
@@ -273,6 +276,207 @@ __global__ __launch_bounds__(256) void gather_batch(const ConvJob* jobs,
                                    threadIdx.x & 63);
 }
 
+// ------------------------------------------------------ staged gather ----
+//
+// One 256-thread workgroup owns a kSBX x kSBY x kSBZ output block: wave w
+// owns x = bx + w and lane (j, k) the outputs (y = by + j + 4 r, z = bz + k),
+// r = 0..3.  Per cone, the block's union tap range is cut into chunks of
+// s_ch taps.  The TERMA a chunk reads is the block shifted by the chunk's
+// offsets, a box of at most s_ext voxels.  It is staged into LDS one row
+// (s_ext[2] <= kSPitch floats along z) per range-checked buffer load that
+// writes LDS directly (x outside the grid reads 0; y and z stay inside the
+// padding), then every lane applies the chunk's taps from LDS.  The box
+// layout is fixed per cone (rows of kSPitch floats, planes of s_ext[1]
+// rows), so tap m reads LDS at (lane base + chunk base - lin2[m]).  With
+// rows of 48 floats the four y rows a wavefront reads per tap fall in
+// disjoint bank ranges.
+// Sums: plain FMA over a chunk's taps, Kahan-compensated across chunks.
+
+constexpr int kBufferFlags = 0x00020000;  // raw 32-bit buffer resource
+
+template <typename OutT>
+__device__ __forceinline__ void staged_block(const ConvJob& j, int blk,
+                                             float* __restrict__ lds)
+{
+    const int bz = (blk % j.ntz) * kSBZ;
+    const int rr = blk / j.ntz;
+    const int by = (rr % j.nty) * kSBY;
+    const int bx = (rr / j.nty) * kSBX;
+    const int xl = min(bx + kSBX - 1, j.X - 1);
+    const int yl = min(by + kSBY - 1, j.Y - 1);
+    const int zl = min(bz + kSBZ - 1, j.Z - 1);
+    const int tid = threadIdx.x;
+    const int li = tid >> 6, lj = (tid >> 4) & 3, lk = tid & 15;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int lane = tid & 63;
+
+    int lo0 = 0, lo1 = 0, lo2 = 0;
+    int hi0 = j.X - 1, hi1 = j.Y - 1, hi2 = j.Z - 1;
+    if (j.box) {
+        const c_i32* b = as_const(j.box);
+        lo0 = max(lo0, b[0]);
+        lo1 = max(lo1, b[1]);
+        lo2 = max(lo2, b[2]);
+        hi0 = min(hi0, b[3]);
+        hi1 = min(hi1, b[4]);
+        hi2 = min(hi2, b[5]);
+    }
+    const int Zp = j.Z + 2 * kZPad;
+    const int Yp = j.Y + 2 * kYPad;
+    const float* alloc = j.terma - ((size_t)kYPad * Zp + kZPad);
+    const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)alloc, (short)0, j.X * Yp * Zp * 4, kBufferFlags);
+
+    float sum[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    float comp[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (lo0 <= hi0 && lo1 <= hi1 && lo2 <= hi2) {
+        const c_meta* meta = as_const(j.meta);
+        const c_i32* ftab = as_const(j.ftab);
+        for (int c = 0; c < j.n_cones; ++c) {
+            const c_meta* m = meta + c;
+            int ax1, bx1, ax2, bx2, ay1, by1, ay2, by2, az1, bz1, az2, bz2;
+            urange(bx, lo0, hi0, m->step[0], ftab + m->f_begin[0],
+                   m->f_len[0], ax1, bx1);
+            urange(xl, lo0, hi0, m->step[0], ftab + m->f_begin[0],
+                   m->f_len[0], ax2, bx2);
+            urange(by, lo1, hi1, m->step[1], ftab + m->f_begin[1],
+                   m->f_len[1], ay1, by1);
+            urange(yl, lo1, hi1, m->step[1], ftab + m->f_begin[1],
+                   m->f_len[1], ay2, by2);
+            urange(bz, lo2, hi2, m->step[2], ftab + m->f_begin[2],
+                   m->f_len[2], az1, bz1);
+            urange(zl, lo2, hi2, m->step[2], ftab + m->f_begin[2],
+                   m->f_len[2], az2, bz2);
+            // union of the block's per-voxel ranges (monotone per axis)
+            const int ulo =
+                max(max(min(ax1, ax2), min(ay1, ay2)), min(az1, az2));
+            const int uhi =
+                min(min(max(bx1, bx2), max(by1, by2)), max(bz1, bz2));
+            if (ulo >= uhi)
+                continue;
+            const int ch = m->s_ch;
+            const int ex = m->s_ext[0], ey = m->s_ext[1], ez = m->s_ext[2];
+            const int px = ey * kSPitch;
+            const int lane_base = li * px + lj * kSPitch + lk;
+            const c_i32* off = as_const(j.off) + 3 * m->tap_begin;
+            const c_i32* l2 = as_const(j.lin2) + m->tap_begin;
+            const c_f32* kk = as_const(j.k) + m->tap_begin;
+            for (int m0 = ulo; m0 < uhi; m0 += ch) {
+                const int m1 = min(m0 + ch, uhi);
+                // offsets are monotone per axis: the chunk's largest offset
+                // sits at one of its ends
+                const int P0 = max(off[3 * m0], off[3 * (m1 - 1)]);
+                const int P1 = max(off[3 * m0 + 1], off[3 * (m1 - 1) + 1]);
+                const int P2 = max(off[3 * m0 + 2], off[3 * (m1 - 1) + 2]);
+                // element index of the box's first voxel in the padded T
+                const int origin =
+                    ((bx - P0) * Yp + (by - P1) + kYPad) * Zp + (bz - P2) +
+                    kZPad;
+                __syncthreads();  // every wave is done with the last chunk
+                // wave w stages the box planes x = w, w + 4, ...: one row of
+                // ez floats per instruction, straight into LDS (lanes >= ez
+                // idle; planes outside the grid read zeros)
+                if (lane < ez) {
+                    for (int xi = w; xi < ex; xi += 4) {
+                        const int g = origin + xi * Yp * Zp;
+                        float* row = lds + xi * px;
+                        for (int yi = 0; yi < ey; ++yi)
+                            __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                                rsrc,
+                                (__attribute__((address_space(3))) void*)(
+                                    row + yi * kSPitch),
+                                4, (g + yi * Zp + lane) * 4, 0, 0, 0);
+                    }
+                }
+                __builtin_amdgcn_s_waitcnt(0);  // the rows have landed
+                __syncthreads();
+                // byte offsets: lin2 holds 4 x the element offset
+                const int base =
+                    4 * (lane_base + P0 * px + P1 * kSPitch + P2);
+                const char* L = reinterpret_cast<const char*>(lds);
+                float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
+                int t = m0;
+                for (; t + 8 <= m1; t += 8) {
+                    float v[8][4], kv[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        const float* q = reinterpret_cast<const float*>(
+                            L + (base - l2[t + u]));
+                        v[u][0] = q[0];
+                        v[u][1] = q[4 * kSPitch];
+                        v[u][2] = q[8 * kSPitch];
+                        v[u][3] = q[12 * kSPitch];
+                        kv[u] = kk[t + u];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        a0 = fmaf(kv[u], v[u][0], a0);
+                        a1 = fmaf(kv[u], v[u][1], a1);
+                        a2 = fmaf(kv[u], v[u][2], a2);
+                        a3 = fmaf(kv[u], v[u][3], a3);
+                    }
+                }
+                for (; t < m1; ++t) {
+                    const float* q =
+                        reinterpret_cast<const float*>(L + (base - l2[t]));
+                    const float kv = kk[t];
+                    a0 = fmaf(kv, q[0], a0);
+                    a1 = fmaf(kv, q[4 * kSPitch], a1);
+                    a2 = fmaf(kv, q[8 * kSPitch], a2);
+                    a3 = fmaf(kv, q[12 * kSPitch], a3);
+                }
+                kahan_add(sum[0], comp[0], a0);
+                kahan_add(sum[1], comp[1], a1);
+                kahan_add(sum[2], comp[2], a2);
+                kahan_add(sum[3], comp[3], a3);
+            }
+        }
+    }
+    const int x = bx + li, z = bz + lk;
+    if (x < j.X && z < j.Z) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int y = by + lj + 4 * r;
+            if (y < j.Y) {
+                OutT* d = reinterpret_cast<OutT*>(j.dose) +
+                          ((size_t)x * j.Y + y) * j.Z + z;
+                const OutT prev = j.accumulate ? *d : (OutT)0;
+                *d = prev + (OutT)sum[r];
+            }
+        }
+    }
+}
+
+template <typename OutT>
+__global__ __launch_bounds__(256) void staged_single(ConvJob j, int n_blocks)
+{
+    __shared__ float lds[kSLds];
+    if ((int)blockIdx.x >= n_blocks)
+        return;
+    staged_block<OutT>(j, blockIdx.x, lds);
+}
+
+template <typename OutT>
+__global__ __launch_bounds__(256) void staged_batch(const ConvJob* jobs,
+                                                    int n_jobs, int n_blocks)
+{
+    __shared__ float lds[kSLds];
+    const int b = blockIdx.x;
+    if (b >= n_blocks)
+        return;
+    const c_job* J = as_const(jobs);
+    int lo = 0, hi = n_jobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (J[mid].tile_begin <= b)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const ConvJob jb = load_job(J + lo);
+    staged_block<OutT>(jb, b - jb.tile_begin, lds);
+}
+
 // T [X][Y][Z] -> padded [X][Y + 2 kYPad][Z + 2 kZPad] with zero borders.
 __global__ __launch_bounds__(256) void pad_kernel(const float* __restrict__ T,
                                                   float* __restrict__ P,
@@ -297,16 +501,24 @@ template <typename OutT>
 struct Kernels {
     void (*single)(ConvJob, int);
     void (*batch)(const ConvJob*, int, int);
+    int tiles_per_block;  // 4 wavefront tiles, or 1 staged block
 };
 
 template <typename OutT>
 static ch_status kernels_for(const ch_plan* p, Kernels<OutT>& k)
 {
-    const int ty = p->tile[1], tz = p->tile[2];
+    const int tx = p->tile[0], ty = p->tile[1], tz = p->tile[2];
+    if (tx == kSBX && ty == kSBY && tz == kSBZ) {
+        k.single = staged_single<OutT>;
+        k.batch = staged_batch<OutT>;
+        k.tiles_per_block = 1;
+        return CH_OK;
+    }
 #define CH_PICK(TY, TZ, A)                                                  \
-    if (ty == TY && tz == TZ && p->accum_mode == A) {                       \
+    if (tx == 1 && ty == TY && tz == TZ && p->accum_mode == A) {            \
         k.single = gather_single<TY, TZ, A, OutT>;                          \
         k.batch = gather_batch<TY, TZ, A, OutT>;                            \
+        k.tiles_per_block = 4;                                              \
         return CH_OK;                                                       \
     }
 #define CH_PICK3(TY, TZ) CH_PICK(TY, TZ, 0) CH_PICK(TY, TZ, 1) CH_PICK(TY, TZ, 2)
@@ -316,8 +528,8 @@ static ch_status kernels_for(const ch_plan* p, Kernels<OutT>& k)
     CH_PICK3(8, 8)
 #undef CH_PICK3
 #undef CH_PICK
-    set_error("no gather kernel for tile 1x%dx%d, accumulation %d", ty, tz,
-              p->accum_mode);
+    set_error("no gather kernel for tile %dx%dx%d, accumulation %d", tx, ty,
+              tz, p->accum_mode);
     return CH_ERR_ARG;
 }
 
@@ -335,11 +547,14 @@ static ConvJob make_job(const ch_plan* p, const float* padded, void* dose,
     j.k = p->d_k;
     j.ftab = p->d_ftab;
     j.meta = p->d_meta;
+    j.off = p->d_off;
+    j.lin2 = p->d_lin2;
     j.X = p->shape[0];
     j.Y = p->shape[1];
     j.Z = p->shape[2];
     j.n_cones = p->n_cones;
     j.accumulate = accumulate;
+    j.ntx = (j.X + p->tile[0] - 1) / p->tile[0];
     j.nty = (j.Y + p->tile[1] - 1) / p->tile[1];
     j.ntz = (j.Z + p->tile[2] - 1) / p->tile[2];
     j.tile_begin = 0;
@@ -348,7 +563,7 @@ static ConvJob make_job(const ch_plan* p, const float* padded, void* dose,
 
 static int64_t job_tiles(const ConvJob& j)
 {
-    return (int64_t)j.X * j.nty * j.ntz;
+    return (int64_t)j.ntx * j.nty * j.ntz;
 }
 
 static size_t padded_elems(const ch_plan* p)
@@ -470,7 +685,8 @@ static ch_status convolve_one(const ch_plan* p, const float* d_terma,
     ConvJob j = make_job(p, w->pad, d_dose, d_box, accumulate);
     const int64_t n_tiles = job_tiles(j);
     CH_REQUIRE(n_tiles < INT_MAX / 2, "too many tiles");
-    const unsigned blocks = (unsigned)((n_tiles + 3) / 4);
+    const unsigned blocks =
+        (unsigned)((n_tiles + k.tiles_per_block - 1) / k.tiles_per_block);
     LaunchTimer timer(s);
     hipLaunchKernelGGL(k.single, dim3(blocks), dim3(256), 0, s, j,
                        (int)n_tiles);
@@ -568,7 +784,9 @@ ch_status ch_convolve_batch(int32_t n_jobs, const ch_conv_job* h_jobs,
         const ch_plan* p0 = h_jobs[i0].plan;
         group.clear();
         for (int32_t i = i0; i < n_jobs; ++i)
-            if (!taken[i] && h_jobs[i].plan->tile[2] == p0->tile[2] &&
+            if (!taken[i] && h_jobs[i].plan->tile[0] == p0->tile[0] &&
+                h_jobs[i].plan->tile[1] == p0->tile[1] &&
+                h_jobs[i].plan->tile[2] == p0->tile[2] &&
                 h_jobs[i].plan->accum_mode == p0->accum_mode) {
                 taken[i] = 1;
                 group.push_back(i);
@@ -607,7 +825,8 @@ ch_status ch_convolve_batch(int32_t n_jobs, const ch_conv_job* h_jobs,
         CH_HIP(hipEventRecord(w->jobs_done, s));
         if (tiles == 0)
             continue;
-        const unsigned blocks = (unsigned)((tiles + 3) / 4);
+        const unsigned blocks =
+            (unsigned)((tiles + k.tiles_per_block - 1) / k.tiles_per_block);
         LaunchTimer timer(s);
         hipLaunchKernelGGL(k.batch, dim3(blocks), dim3(256), 0, s,
                            (const ConvJob*)w->d_jobs, ng, (int)tiles);

@@ -40,20 +40,39 @@ constexpr int32_t kTapPad = 8;
 //  taps [tap_begin, tap_begin + n_taps): source byte offset and increment.
 //  ftab[f_begin[a] + q], q in [0, f_len[a]): first tap index m whose offset
 //  has |o_{m,a}| >= q; the last entry (q = f_len[a]-1) equals n_taps.
+//  s_*: the staged gather's chunking of this cone (set by the host after
+//  the tap lists are built): s_ch taps per chunk, LDS box extents s_ext
+//  (block + the largest per-axis offset change over s_ch consecutive taps)
+//  and s_nstage staged elements per thread.
 struct ConeMeta {
     int32_t tap_begin;
     int32_t n_taps;
     int32_t f_begin[3];
     int32_t f_len[3];
     int32_t step[3];
-    int32_t pad;
+    int32_t s_ch;
+    int32_t s_ext[3];
+    int32_t s_nstage;
 };
 
 // Zero voxels on both ends of every z-row (kZPad) and every y-column
-// (kYPad) of the padded TERMA copy the gather reads; they must be at least
-// the z- and y-extent of a wavefront's output tile.
+// (kYPad) of the padded TERMA copy the gathers read.  They bound how far a
+// staged box may reach past the TERMA support along y and z (block extent
+// - 1 plus the chunk's offset change); x needs no padding because staged
+// loads are range-checked buffer loads.
 constexpr int32_t kZPad = 64;
-constexpr int32_t kYPad = 8;
+constexpr int32_t kYPad = 32;
+
+// Staged gather: one 256-thread workgroup per 4 x 16 x 16 output block, four
+// outputs per lane (y, y + 4, y + 8, y + 12); TERMA boxes of up to
+// kSLds floats are staged in LDS with rows of kSPitch floats.
+constexpr int kSBX = 4, kSBY = 16, kSBZ = 16;
+constexpr int kSPitch = 48;     // row pitch: conflict-free 4 x 16 lane reads
+constexpr int kSLds = 8192;     // LDS floats per workgroup (32 KiB)
+constexpr int kSStage = 16;     // staged elements per thread, at most
+constexpr int kSChunk = 32;     // taps per chunk, at most
+constexpr int kSMaxWY = kYPad - kSBY;       // y offset change per chunk
+constexpr int kSMaxWZ = kSPitch - kSBZ;     // z offset change per chunk
 
 // Flat job descriptor of the gather kernels.
 struct ConvJob {
@@ -64,10 +83,12 @@ struct ConvJob {
     const float* k;            // per tap FP32 increment
     const int32_t* ftab;
     const ConeMeta* meta;
+    const int32_t* off;        // per tap offset (x, y, z)
+    const int32_t* lin2;       // per tap byte offset in the cone's LDS box
     int32_t X, Y, Z;
     int32_t n_cones;
     int32_t accumulate;
-    int32_t nty, ntz;          // tiles along y and z
+    int32_t ntx, nty, ntz;     // tiles (or staged blocks) along x, y, z
     int32_t tile_begin;        // first global tile of this job
 };
 
@@ -172,6 +193,7 @@ struct ch_plan {
     float* d_k;
     double* d_k64;
     int32_t* d_off;
+    int32_t* d_lin2;
     int32_t* d_ftab;
     ch::ConeMeta* d_meta;
     std::vector<ch::ConeMeta> h_meta;

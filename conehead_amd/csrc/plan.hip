@@ -9,7 +9,9 @@
 // lies in the grid.  The convolution is therefore the translation-invariant
 // stencil dose(v) = sum_c sum_m k_{c,m} T(v - o_{c,m}); this file builds the
 // (o_{c,m}, k_{c,m}) lists once per (grid, voxel size, cone set, kernel).
+#include <cstdlib>
 #include <cstring>
+#include <vector>
 
 #include "dda_core.h"
 
@@ -93,7 +95,9 @@ __global__ __launch_bounds__(64) void build_taps_kernel(
     m.step[0] = d.s0;
     m.step[1] = d.s1;
     m.step[2] = d.s2;
-    m.pad = 0;
+    m.s_ch = 0;  // set on the host by stage_cones
+    m.s_ext[0] = m.s_ext[1] = m.s_ext[2] = 0;
+    m.s_nstage = 0;
     meta[c] = m;
 }
 
@@ -105,9 +109,59 @@ static void plan_free(ch_plan* p)
     hipFree(p->d_k);
     hipFree(p->d_k64);
     hipFree(p->d_off);
+    hipFree(p->d_lin2);
     hipFree(p->d_ftab);
     hipFree(p->d_meta);
     delete p;
+}
+
+// Chunking of every cone for the staged gather.  A chunk of L consecutive
+// taps moves the source box by at most w_a = max |o_{m+L-1,a} - o_{m,a}|
+// along axis a, so its TERMA box fits extents (kSBX, kSBY, kSBZ) + w.  Each
+// cone takes the longest L <= kSChunk whose box fits the LDS buffer, the
+// per-thread staging registers and the y/z padding; L = 1 always fits.
+// lin2[m] is tap m's byte offset in that box layout (rows of kSPitch
+// floats).
+static void stage_cones(std::vector<ConeMeta>& meta,
+                        const std::vector<int32_t>& off, int32_t tap_stride,
+                        std::vector<int32_t>& lin2)
+{
+    lin2.assign(meta.size() * (size_t)tap_stride, 0);
+    for (size_t c = 0; c < meta.size(); ++c) {
+        ConeMeta& m = meta[c];
+        const int32_t* o = off.data() + 3 * (size_t)m.tap_begin;
+        const int32_t n = m.n_taps;
+        int32_t w[3] = {0, 0, 0};
+        int32_t ch = kSChunk;
+        for (; ch >= 1; --ch) {
+            const int32_t L = n < ch ? n : ch;
+            w[0] = w[1] = w[2] = 0;
+            for (int32_t i = 0; L > 0 && i + L - 1 < n; ++i)
+                for (int a = 0; a < 3; ++a) {
+                    const int32_t d = std::abs(o[3 * (i + L - 1) + a] -
+                                               o[3 * i + a]);
+                    w[a] = d > w[a] ? d : w[a];
+                }
+            const int32_t ex = kSBX + w[0], ey = kSBY + w[1],
+                          ez = kSBZ + w[2];
+            if (w[1] <= kSMaxWY && w[2] <= kSMaxWZ &&
+                ex * ey * kSPitch <= kSLds && ex * ey * ez <= kSStage * 256)
+                break;
+        }
+        if (ch < 1) {  // unreachable: L = 1 gives w = 0
+            ch = 1;
+            w[0] = w[1] = w[2] = 0;
+        }
+        m.s_ch = ch;
+        m.s_ext[0] = kSBX + w[0];
+        m.s_ext[1] = kSBY + w[1];
+        m.s_ext[2] = kSBZ + w[2];
+        m.s_nstage = (m.s_ext[0] * m.s_ext[1] * m.s_ext[2] + 255) / 256;
+        const int32_t px = m.s_ext[1] * kSPitch;
+        int32_t* l2 = lin2.data() + m.tap_begin;
+        for (int32_t t = 0; t < n; ++t)
+            l2[t] = 4 * (o[3 * t] * px + o[3 * t + 1] * kSPitch + o[3 * t + 2]);
+    }
 }
 
 }  // namespace ch
@@ -164,6 +218,7 @@ ch_status ch_plan_create(const int32_t h_shape[3], const double h_vsize[3],
     p->d_k = nullptr;
     p->d_k64 = nullptr;
     p->d_off = nullptr;
+    p->d_lin2 = nullptr;
     p->d_ftab = nullptr;
     p->d_meta = nullptr;
 
@@ -182,6 +237,7 @@ ch_status ch_plan_create(const int32_t h_shape[3], const double h_vsize[3],
     CH_TRY(hipMalloc((void**)&p->d_k, nt * sizeof(float)));
     CH_TRY(hipMalloc((void**)&p->d_k64, nt * sizeof(double)));
     CH_TRY(hipMalloc((void**)&p->d_off, nt * 3 * sizeof(int32_t)));
+    CH_TRY(hipMalloc((void**)&p->d_lin2, nt * sizeof(int32_t)));
     CH_TRY(hipMalloc((void**)&p->d_ftab, nf * sizeof(int32_t)));
     CH_TRY(hipMalloc((void**)&p->d_meta, n_cones * sizeof(ConeMeta)));
     CH_TRY(hipMalloc((void**)&d_dirs, (size_t)n_cones * 3 * sizeof(double)));
@@ -208,6 +264,18 @@ ch_status ch_plan_create(const int32_t h_shape[3], const double h_vsize[3],
     CH_TRY(hipMemcpy(p->h_meta.data(), p->d_meta,
                      (size_t)n_cones * sizeof(ConeMeta),
                      hipMemcpyDeviceToHost));
+    if (e == hipSuccess) {
+        std::vector<int32_t> off(nt * 3), lin2;
+        CH_TRY(hipMemcpy(off.data(), p->d_off, nt * 3 * sizeof(int32_t),
+                         hipMemcpyDeviceToHost));
+        if (e == hipSuccess)
+            stage_cones(p->h_meta, off, p->tap_stride, lin2);
+        CH_TRY(hipMemcpy(p->d_lin2, lin2.data(), nt * sizeof(int32_t),
+                         hipMemcpyHostToDevice));
+        CH_TRY(hipMemcpy(p->d_meta, p->h_meta.data(),
+                         (size_t)n_cones * sizeof(ConeMeta),
+                         hipMemcpyHostToDevice));
+    }
 #undef CH_TRY
     hipFree(d_dirs);
     hipFree(d_rows);
@@ -274,11 +342,13 @@ ch_status ch_plan_cone_taps(const ch_plan* plan, int32_t cone,
 ch_status ch_plan_set_tile(ch_plan* plan, int32_t tx, int32_t ty, int32_t tz)
 {
     CH_REQUIRE(plan, "ch_plan_set_tile: null plan");
-    CH_REQUIRE(tx == 1 && ty * tz == 64 && ty >= 1 && ty <= kYPad &&
-                   tz <= kZPad && (ty & (ty - 1)) == 0,
-               "tile %dx%dx%d: a wavefront owns 1 x TY x TZ = 64 voxels, "
-               "TY in {1, 2, 4, 8}",
-               tx, ty, tz);
+    const bool staged = tx == kSBX && ty == kSBY && tz == kSBZ;
+    CH_REQUIRE(staged || (tx == 1 && ty * tz == 64 && ty >= 1 && ty <= 8 &&
+                          (ty & (ty - 1)) == 0),
+               "tile %dx%dx%d: either a wavefront owns 1 x TY x TZ = 64 "
+               "voxels, TY in {1, 2, 4, 8}, or a workgroup owns the staged "
+               "block %d x %d x %d",
+               tx, ty, tz, kSBX, kSBY, kSBZ);
     plan->tile[0] = tx;
     plan->tile[1] = ty;
     plan->tile[2] = tz;

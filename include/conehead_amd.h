@@ -162,8 +162,11 @@ typedef struct ch_conv_job {
 ch_status ch_convolve_batch(int32_t n_jobs, const ch_conv_job* h_jobs,
                             void* stream);
 
-/* Tile shape (voxels per wavefront along x, y, z; product 64) used by the
- * gather kernels for plans of this shape; set by the autotuner. */
+/* Tile shape used by the gather kernels for this plan; set by the
+ * autotuner.  1 x TY x TZ (TY in {1, 2, 4, 8}, TY * TZ = 64) selects the
+ * direct gather, one wavefront per tile; 4 x 16 x 16 selects the staged
+ * gather, one workgroup per block with the TERMA box of each chunk of taps
+ * staged through LDS. */
 ch_status ch_plan_set_tile(ch_plan* plan, int32_t tx, int32_t ty, int32_t tz);
 ch_status ch_plan_get_tile(const ch_plan* plan, int32_t* tx, int32_t* ty,
                            int32_t* tz);
