@@ -260,6 +260,16 @@ def run_batched(args, eng, steps=5):
             out = be.collect(h)[0]
         else:
             be.release(be.collect(h, download=False))
+    # the same batch with interleaved beam groups where boxes allow
+    be1 = BatchEngine(eng, group_beams=True)
+    be1.release(be1.collect(be1.launch([req], s), download=False))
+    t1 = []
+    for i in range(2):
+        t0 = time.perf_counter()
+        h = be1.launch([req], s)
+        Event().record(s).sync()
+        t1.append((time.perf_counter() - t0) * 1e3)
+        be1.release(be1.collect(h, download=False))
     ms = float(np.median(times))
     nb = len(case.beams)
     r = {"workload": "64 MLC beams (gantry U[0,360), seed 2024, 60 leaf "
@@ -268,6 +278,7 @@ def run_batched(args, eng, steps=5):
          "gather_tile": list(list(tile.values())[0]),
          "ms_per_batch": ms, "gpu_ms_per_batch": float(np.median(gpu)),
          "beams_per_s": nb / (ms * 1e-3),
+         "ms_per_batch_with_beam_groups": float(np.median(t1)),
          "voxel_cone_updates_per_s":
              case.n_voxels * case.cones.n_cones * nb / (ms * 1e-3)}
     if not args.no_cpu_baseline:

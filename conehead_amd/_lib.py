@@ -104,6 +104,22 @@ class ConvJob(ctypes.Structure):
     ]
 
 
+CH_MAX_GROUP = 4
+
+
+class ConvGroup(ctypes.Structure):
+    """Mirror of ch_conv_group."""
+
+    _fields_ = [
+        ("plan", c_vp),
+        ("n_beams", ctypes.c_int32),
+        ("accumulate", ctypes.c_int32),
+        ("d_terma", c_vp * CH_MAX_GROUP),
+        ("d_dose", c_vp * CH_MAX_GROUP),
+        ("d_box", c_vp * CH_MAX_GROUP),
+    ]
+
+
 # name -> (restype, argtypes); every function declared in the header.
 SIGNATURES = {
     "ch_last_error": (ctypes.c_char_p, []),
@@ -149,6 +165,8 @@ SIGNATURES = {
     "ch_convolve_f64": (ctypes.c_int32, [c_vp, c_vp, c_vp, c_vp, c_vp]),
     "ch_convolve_batch": (ctypes.c_int32, [ctypes.c_int32,
                                            ctypes.POINTER(ConvJob), c_vp]),
+    "ch_convolve_groups": (ctypes.c_int32, [ctypes.c_int32,
+                                            ctypes.POINTER(ConvGroup), c_vp]),
     "ch_convolve_scatter": (ctypes.c_int32, [
         c_vp, c_vp, c_i32p, c_f64p, ctypes.c_int32, c_vp, c_vp, c_vp,
         ctypes.c_int32, ctypes.c_int32, c_vp]),

@@ -477,6 +477,233 @@ __global__ __launch_bounds__(256) void staged_batch(const ConvJob* jobs,
     staged_block<OutT>(jb, b - jb.tile_begin, lds);
 }
 
+// --------------------------------------------------- beam-group gather ----
+//
+// Up to kGroup beams on one plan share every tap: their padded TERMA grids
+// are interleaved ([X][Yp][Zp][kGroup] floats), so one aligned 16-byte load
+// per lane and tap feeds all of them, and the tap walk (ranges, offsets,
+// increments) is paid once per group.  Tap ranges come from the union of
+// the beams' support boxes; each beam's sum follows the direct gather's
+// accumulation scheme.
+
+struct GroupJob {
+    const float* terma;        // interleaved padded TERMA, interior (0,0,0)
+    float* dose[kGroup];
+    const int32_t* box[kGroup];
+    const int32_t* lin;
+    const float* k;
+    const int32_t* ftab;
+    const ConeMeta* meta;
+    int32_t X, Y, Z;
+    int32_t n_cones;
+    int32_t n_beams;
+    int32_t accumulate;
+    int32_t nty, ntz;
+    int32_t tile_begin;
+};
+typedef __attribute__((address_space(4))) const GroupJob c_gjob;
+
+__device__ __forceinline__ GroupJob load_gjob(const c_gjob* p)
+{
+    GroupJob j;
+    j.terma = p->terma;
+#pragma unroll
+    for (int b = 0; b < kGroup; ++b) {
+        j.dose[b] = p->dose[b];
+        j.box[b] = p->box[b];
+    }
+    j.lin = p->lin;
+    j.k = p->k;
+    j.ftab = p->ftab;
+    j.meta = p->meta;
+    j.X = p->X;
+    j.Y = p->Y;
+    j.Z = p->Z;
+    j.n_cones = p->n_cones;
+    j.n_beams = p->n_beams;
+    j.accumulate = p->accumulate;
+    j.nty = p->nty;
+    j.ntz = p->ntz;
+    j.tile_begin = p->tile_begin;
+    return j;
+}
+
+template <int TY, int TZ, int ACC>
+__device__ __forceinline__ void gather_tile_group(const GroupJob& j, int tile,
+                                                  int lane)
+{
+    static_assert(TY * TZ == 64 && TY <= kYPad && TZ <= kZPad, "tile");
+    const int tz = tile % j.ntz;
+    const int r = tile / j.ntz;
+    const int ty = r % j.nty;
+    const int x = r / j.nty;
+    const int y0 = ty * TY, z0 = tz * TZ;
+    const int yl = min(y0 + TY - 1, j.Y - 1);
+    const int zl = min(z0 + TZ - 1, j.Z - 1);
+    const int y = y0 + lane / TZ;
+    const int z = z0 + lane % TZ;
+    const int Zp = j.Z + 2 * kZPad;
+    const int Yp = j.Y + 2 * kYPad;
+
+    // union of the beams' support boxes
+    int lo0 = INT_MAX, lo1 = INT_MAX, lo2 = INT_MAX;
+    int hi0 = INT_MIN, hi1 = INT_MIN, hi2 = INT_MIN;
+#pragma unroll
+    for (int b = 0; b < kGroup; ++b) {
+        if (b >= j.n_beams)
+            continue;
+        int l0 = 0, l1 = 0, l2 = 0, h0 = j.X - 1, h1 = j.Y - 1, h2 = j.Z - 1;
+        if (j.box[b]) {
+            const c_i32* bb = as_const(j.box[b]);
+            l0 = max(l0, bb[0]);
+            l1 = max(l1, bb[1]);
+            l2 = max(l2, bb[2]);
+            h0 = min(h0, bb[3]);
+            h1 = min(h1, bb[4]);
+            h2 = min(h2, bb[5]);
+        }
+        if (l0 <= h0 && l1 <= h1 && l2 <= h2) {
+            lo0 = min(lo0, l0);
+            lo1 = min(lo1, l1);
+            lo2 = min(lo2, l2);
+            hi0 = max(hi0, h0);
+            hi1 = max(hi1, h1);
+            hi2 = max(hi2, h2);
+        }
+    }
+    // byte offset of this lane's voxel (16 bytes per voxel) in the padded
+    // interleaved allocation
+    const uint32_t base16 =
+        (uint32_t)(((x * Yp + y + kYPad) * Zp + z + kZPad)) * 16u;
+    const char* __restrict__ Tb = reinterpret_cast<const char*>(j.terma) -
+                                  (size_t)(kYPad * Zp + kZPad) * 16;
+    const c_meta* meta = as_const(j.meta);
+    const c_i32* ftab = as_const(j.ftab);
+    const c_i32* lin = as_const(j.lin);
+    const c_f32* kk = as_const(j.k);
+
+    float sum[kGroup] = {}, comp[kGroup] = {};
+    if (lo0 <= hi0 && lo1 <= hi1 && lo2 <= hi2) {
+        for (int c = 0; c < j.n_cones; ++c) {
+            const c_meta* m = meta + c;
+            int ax, bx, ay1, by1, ay2, by2, az1, bz1, az2, bz2;
+            urange(x, lo0, hi0, m->step[0], ftab + m->f_begin[0],
+                   m->f_len[0], ax, bx);
+            urange(y0, lo1, hi1, m->step[1], ftab + m->f_begin[1],
+                   m->f_len[1], ay1, by1);
+            urange(z0, lo2, hi2, m->step[2], ftab + m->f_begin[2],
+                   m->f_len[2], az1, bz1);
+            if (TY > 1)
+                urange(yl, lo1, hi1, m->step[1], ftab + m->f_begin[1],
+                       m->f_len[1], ay2, by2);
+            else {
+                ay2 = ay1;
+                by2 = by1;
+            }
+            urange(zl, lo2, hi2, m->step[2], ftab + m->f_begin[2],
+                   m->f_len[2], az2, bz2);
+            const int ulo = max(max(ax, min(ay1, ay2)), min(az1, az2));
+            const int uhi = min(min(bx, max(by1, by2)), max(bz1, bz2));
+            if (ulo >= uhi)
+                continue;
+            const c_i32* ml = lin + m->tap_begin;
+            const c_f32* mk = kk + m->tap_begin;
+            float acc[kGroup] = {}, ac[kGroup] = {};
+            // whole chunks without a per-tap condition, then one guarded
+            // remainder whose taps past uhi read offset 0 with increment 0
+            const int tend = ulo + (uhi - ulo) / kUnroll * kUnroll;
+            for (int t = ulo; t < uhi; t += kUnroll) {
+                const bool whole = t < tend;
+                float4 v[kUnroll];
+                float kc[kUnroll];
+#pragma unroll
+                for (int u = 0; u < kUnroll; ++u) {
+                    const bool in = whole || t + u < uhi;
+                    const uint32_t o = (uint32_t)ml[t + u] << 2;
+                    v[u] = *reinterpret_cast<const float4*>(
+                        Tb + (base16 - (in ? o : 0u)));
+                    kc[u] = in ? mk[t + u] : 0.0f;
+                }
+#pragma unroll
+                for (int b = 0; b < kGroup; ++b) {
+                    float vb[kUnroll];
+#pragma unroll
+                    for (int u = 0; u < kUnroll; ++u)
+                        vb[u] = b == 0 ? v[u].x
+                                       : (b == 1 ? v[u].y
+                                                 : (b == 2 ? v[u].z : v[u].w));
+                    accumulate_chunk<ACC>(acc[b], ac[b], kc, vb);
+                }
+            }
+#pragma unroll
+            for (int b = 0; b < kGroup; ++b)
+                kahan_add(sum[b], comp[b], acc[b]);
+        }
+    }
+    if (y <= yl && z <= zl) {
+        const size_t o = ((size_t)x * j.Y + y) * j.Z + z;
+#pragma unroll
+        for (int b = 0; b < kGroup; ++b)
+            if (b < j.n_beams) {
+                float* d = j.dose[b] + o;
+                const float prev = j.accumulate ? *d : 0.0f;
+                *d = prev + sum[b];
+            }
+    }
+}
+
+template <int TY, int TZ, int ACC>
+__global__ __launch_bounds__(256) void gather_group(const GroupJob* jobs,
+                                                    int n_jobs, int n_tiles)
+{
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int tile = blockIdx.x * 4 + wave;
+    if (tile >= n_tiles)
+        return;
+    const c_gjob* J = (const c_gjob*)as_const(jobs);
+    int lo = 0, hi = n_jobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (J[mid].tile_begin <= tile)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const GroupJob jb = load_gjob(J + lo);
+    gather_tile_group<TY, TZ, ACC>(jb, tile - jb.tile_begin,
+                                   threadIdx.x & 63);
+}
+
+// Up to kGroup TERMA grids -> one padded interleaved grid
+// [X][Y + 2 kYPad][Z + 2 kZPad][kGroup] with zero borders and zero slots
+// for absent beams.
+__global__ __launch_bounds__(256) void pad_group_kernel(
+    const float* __restrict__ T0, const float* __restrict__ T1,
+    const float* __restrict__ T2, const float* __restrict__ T3,
+    float4* __restrict__ P, int X, int Y, int Z)
+{
+    const int Zp = Z + 2 * kZPad, Yp = Y + 2 * kYPad;
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= (int64_t)X * Yp * Zp)
+        return;
+    const int zz = (int)(i % Zp) - kZPad;
+    const int64_t r = i / Zp;
+    const int yy = (int)(r % Yp) - kYPad;
+    const int64_t x = r / Yp;
+    float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (zz >= 0 && zz < Z && yy >= 0 && yy < Y) {
+        const int64_t s = (x * Y + yy) * Z + zz;
+        v.x = T0[s];
+        if (T1)
+            v.y = T1[s];
+        if (T2)
+            v.z = T2[s];
+        if (T3)
+            v.w = T3[s];
+    }
+    P[i] = v;
+}
+
 // T [X][Y][Z] -> padded [X][Y + 2 kYPad][Z + 2 kZPad] with zero borders.
 __global__ __launch_bounds__(256) void pad_kernel(const float* __restrict__ T,
                                                   float* __restrict__ P,
@@ -572,23 +799,24 @@ static size_t padded_elems(const ch_plan* p)
            (p->shape[2] + 2 * kZPad);
 }
 
-// Per-stream workspaces: the padded TERMA copies and the batch job table.
-// Kernels on one stream run in order, so a stream's workspace is reused by
-// its next call without further synchronisation; the pinned job-table
-// staging buffer is reused only after the event of its last copy.
+// Per-stream workspaces: the padded TERMA copies and the batch job table
+// (a byte buffer holding ConvJob or GroupJob records).  Kernels on one
+// stream run in order, so a stream's workspace is reused by its next call
+// without further synchronisation; the pinned job-table staging buffer is
+// reused only after the event of its last copy.
 struct Workspace {
     hipStream_t stream;
     float* pad;
     size_t pad_elems;
-    ConvJob* d_jobs;
-    ConvJob* h_jobs;
-    int job_cap;
+    void* d_tab;
+    void* h_tab;
+    size_t tab_bytes;
     hipEvent_t jobs_done;
 };
 static std::vector<Workspace*> g_ws;
 static std::mutex g_ws_mu;
 
-static ch_status workspace(hipStream_t s, size_t pad_elems, int n_jobs,
+static ch_status workspace(hipStream_t s, size_t pad_elems, size_t tab_bytes,
                            Workspace** out)
 {
     Workspace* w = nullptr;
@@ -600,9 +828,9 @@ static ch_status workspace(hipStream_t s, size_t pad_elems, int n_jobs,
         w->stream = s;
         w->pad = nullptr;
         w->pad_elems = 0;
-        w->d_jobs = nullptr;
-        w->h_jobs = nullptr;
-        w->job_cap = 0;
+        w->d_tab = nullptr;
+        w->h_tab = nullptr;
+        w->tab_bytes = 0;
         CH_HIP(hipEventCreateWithFlags(&w->jobs_done, hipEventDisableTiming));
         g_ws.push_back(w);
     }
@@ -615,19 +843,19 @@ static ch_status workspace(hipStream_t s, size_t pad_elems, int n_jobs,
         CH_HIP(hipMalloc((void**)&w->pad, pad_elems * sizeof(float)));
         w->pad_elems = pad_elems;
     }
-    if (n_jobs > 0) {
+    if (tab_bytes > 0) {
         CH_HIP(hipEventSynchronize(w->jobs_done));
-        if (w->job_cap < n_jobs) {
+        if (w->tab_bytes < tab_bytes) {
             CH_HIP(hipStreamSynchronize(s));
-            CH_HIP(hipFree(w->d_jobs));
-            CH_HIP(hipHostFree(w->h_jobs));
-            w->d_jobs = nullptr;
-            w->h_jobs = nullptr;
-            const int cap = std::max(n_jobs, std::max(64, 2 * w->job_cap));
-            CH_HIP(hipMalloc((void**)&w->d_jobs, cap * sizeof(ConvJob)));
-            CH_HIP(hipHostMalloc((void**)&w->h_jobs, cap * sizeof(ConvJob),
-                                 hipHostMallocDefault));
-            w->job_cap = cap;
+            CH_HIP(hipFree(w->d_tab));
+            CH_HIP(hipHostFree(w->h_tab));
+            w->d_tab = nullptr;
+            w->h_tab = nullptr;
+            w->tab_bytes = 0;
+            const size_t cap = std::max(tab_bytes, (size_t)16384);
+            CH_HIP(hipMalloc(&w->d_tab, cap));
+            CH_HIP(hipHostMalloc(&w->h_tab, cap, hipHostMallocDefault));
+            w->tab_bytes = cap;
         }
     }
     *out = w;
@@ -643,8 +871,8 @@ void release_gather_workspace(hipStream_t s)
             continue;
         (void)hipStreamSynchronize(s);
         (void)hipFree(w->pad);
-        (void)hipFree(w->d_jobs);
-        (void)hipHostFree(w->h_jobs);
+        (void)hipFree(w->d_tab);
+        (void)hipHostFree(w->h_tab);
         (void)hipEventDestroy(w->jobs_done);
         delete w;
         g_ws.erase(g_ws.begin() + i);
@@ -800,9 +1028,10 @@ ch_status ch_convolve_batch(int32_t n_jobs, const ch_conv_job* h_jobs,
         for (int32_t i : group)
             pad_total += padded_elems(h_jobs[i].plan);
         Workspace* w = nullptr;
-        st = workspace(s, pad_total, ng, &w);
+        st = workspace(s, pad_total, (size_t)ng * sizeof(ConvJob), &w);
         if (st != CH_OK)
             return st;
+        ConvJob* tab = (ConvJob*)w->h_tab;
         int64_t tiles = 0;
         size_t pad_off = 0;
         for (int32_t g = 0; g < ng; ++g) {
@@ -817,9 +1046,9 @@ ch_status ch_convolve_batch(int32_t n_jobs, const ch_conv_job* h_jobs,
             j.tile_begin = (int32_t)tiles;
             tiles += job_tiles(j);
             CH_REQUIRE(tiles < INT_MAX / 2, "batch has too many tiles");
-            w->h_jobs[g] = j;
+            tab[g] = j;
         }
-        CH_HIP(hipMemcpyAsync(w->d_jobs, w->h_jobs,
+        CH_HIP(hipMemcpyAsync(w->d_tab, w->h_tab,
                               (size_t)ng * sizeof(ConvJob),
                               hipMemcpyHostToDevice, s));
         CH_HIP(hipEventRecord(w->jobs_done, s));
@@ -829,7 +1058,125 @@ ch_status ch_convolve_batch(int32_t n_jobs, const ch_conv_job* h_jobs,
             (unsigned)((tiles + k.tiles_per_block - 1) / k.tiles_per_block);
         LaunchTimer timer(s);
         hipLaunchKernelGGL(k.batch, dim3(blocks), dim3(256), 0, s,
-                           (const ConvJob*)w->d_jobs, ng, (int)tiles);
+                           (const ConvJob*)w->d_tab, ng, (int)tiles);
+        CH_HIP(hipGetLastError());
+        timer.stop();
+    }
+    return CH_OK;
+}
+
+ch_status ch_convolve_groups(int32_t n_groups, const ch_conv_group* h_groups,
+                             void* stream)
+{
+    CH_REQUIRE(n_groups >= 0, "n_groups < 0");
+    if (n_groups == 0)
+        return CH_OK;
+    CH_REQUIRE(h_groups, "ch_convolve_groups: null groups");
+    for (int32_t i = 0; i < n_groups; ++i) {
+        const ch_conv_group& g = h_groups[i];
+        CH_REQUIRE(g.plan, "group %d: null plan", i);
+        CH_REQUIRE(g.n_beams >= 1 && g.n_beams <= CH_MAX_GROUP,
+                   "group %d: %d beams (1..%d)", i, g.n_beams, CH_MAX_GROUP);
+        for (int b = 0; b < g.n_beams; ++b)
+            CH_REQUIRE(g.d_terma[b] && g.d_dose[b],
+                       "group %d beam %d: null buffer", i, b);
+        CH_REQUIRE(padded_elems(g.plan) < (1ull << 28),
+                   "group %d: grid too large for a beam group", i);
+    }
+    hipStream_t s = (hipStream_t)stream;
+    std::lock_guard<std::mutex> lock(g_ws_mu);
+    // one launch per (direct tile, accumulation mode); plans tuned for the
+    // staged gather use the 1 x 4 x 16 tile here
+    auto tile_of = [](const ch_plan* p, int a) {
+        if (p->tile[0] != 1)
+            return a == 1 ? 4 : 16;
+        return a == 1 ? p->tile[1] : p->tile[2];
+    };
+    std::vector<char> taken(n_groups, 0);
+    std::vector<int32_t> sel;
+    for (int32_t i0 = 0; i0 < n_groups; ++i0) {
+        if (taken[i0])
+            continue;
+        const ch_plan* p0 = h_groups[i0].plan;
+        const int ty = tile_of(p0, 1), tz = tile_of(p0, 2);
+        sel.clear();
+        for (int32_t i = i0; i < n_groups; ++i)
+            if (!taken[i] && tile_of(h_groups[i].plan, 1) == ty &&
+                tile_of(h_groups[i].plan, 2) == tz &&
+                h_groups[i].plan->accum_mode == p0->accum_mode) {
+                taken[i] = 1;
+                sel.push_back(i);
+            }
+        void (*kern)(const GroupJob*, int, int) = nullptr;
+#define CH_GPICK(TY, TZ, A)                                                 \
+    if (ty == TY && tz == TZ && p0->accum_mode == A)                        \
+        kern = gather_group<TY, TZ, A>;
+#define CH_GPICK3(TY, TZ) CH_GPICK(TY, TZ, 0) CH_GPICK(TY, TZ, 1) CH_GPICK(TY, TZ, 2)
+        CH_GPICK3(1, 64)
+        CH_GPICK3(2, 32)
+        CH_GPICK3(4, 16)
+        CH_GPICK3(8, 8)
+#undef CH_GPICK3
+#undef CH_GPICK
+        CH_REQUIRE(kern, "no group gather for tile 1x%dx%d", ty, tz);
+        const int32_t ng = (int32_t)sel.size();
+        size_t pad_total = 0;
+        for (int32_t i : sel)
+            pad_total += (size_t)kGroup * padded_elems(h_groups[i].plan);
+        Workspace* w = nullptr;
+        ch_status st =
+            workspace(s, pad_total, (size_t)ng * sizeof(GroupJob), &w);
+        if (st != CH_OK)
+            return st;
+        GroupJob* tab = (GroupJob*)w->h_tab;
+        int64_t tiles = 0;
+        size_t pad_off = 0;
+        for (int32_t g = 0; g < ng; ++g) {
+            const ch_conv_group& hg = h_groups[sel[g]];
+            const ch_plan* p = hg.plan;
+            float* P = w->pad + pad_off;
+            const int64_t n = (int64_t)padded_elems(p);
+            hipLaunchKernelGGL(
+                pad_group_kernel, dim3((unsigned)((n + 255) / 256)),
+                dim3(256), 0, s, hg.d_terma[0],
+                hg.n_beams > 1 ? hg.d_terma[1] : nullptr,
+                hg.n_beams > 2 ? hg.d_terma[2] : nullptr,
+                hg.n_beams > 3 ? hg.d_terma[3] : nullptr, (float4*)P,
+                p->shape[0], p->shape[1], p->shape[2]);
+            CH_HIP(hipGetLastError());
+            pad_off += (size_t)kGroup * padded_elems(p);
+            GroupJob j;
+            const int Zp = p->shape[2] + 2 * kZPad;
+            j.terma = P + ((size_t)kYPad * Zp + kZPad) * kGroup;
+            for (int b = 0; b < kGroup; ++b) {
+                j.dose[b] = b < hg.n_beams ? hg.d_dose[b] : nullptr;
+                j.box[b] = b < hg.n_beams ? hg.d_box[b] : nullptr;
+            }
+            j.lin = p->d_lin;
+            j.k = p->d_k;
+            j.ftab = p->d_ftab;
+            j.meta = p->d_meta;
+            j.X = p->shape[0];
+            j.Y = p->shape[1];
+            j.Z = p->shape[2];
+            j.n_cones = p->n_cones;
+            j.n_beams = hg.n_beams;
+            j.accumulate = hg.accumulate;
+            j.nty = (j.Y + ty - 1) / ty;
+            j.ntz = (j.Z + tz - 1) / tz;
+            j.tile_begin = (int32_t)tiles;
+            tiles += (int64_t)j.X * j.nty * j.ntz;
+            CH_REQUIRE(tiles < INT_MAX / 2, "groups have too many tiles");
+            tab[g] = j;
+        }
+        CH_HIP(hipMemcpyAsync(w->d_tab, w->h_tab,
+                              (size_t)ng * sizeof(GroupJob),
+                              hipMemcpyHostToDevice, s));
+        CH_HIP(hipEventRecord(w->jobs_done, s));
+        LaunchTimer timer(s);
+        hipLaunchKernelGGL(kern, dim3((unsigned)((tiles + 3) / 4)),
+                           dim3(256), 0, s, (const GroupJob*)w->d_tab, ng,
+                           (int)tiles);
         CH_HIP(hipGetLastError());
         timer.stop();
     }

@@ -71,6 +71,7 @@ constexpr int kSPitch = 48;     // row pitch: conflict-free 4 x 16 lane reads
 constexpr int kSLds = 8192;     // LDS floats per workgroup (32 KiB)
 constexpr int kSStage = 16;     // staged elements per thread, at most
 constexpr int kSChunk = 32;     // taps per chunk, at most
+constexpr int kGroup = CH_MAX_GROUP;  // beams per interleaved group
 constexpr int kSMaxWY = kYPad - kSBY;       // y offset change per chunk
 constexpr int kSMaxWZ = kSPitch - kSBZ;     // z offset change per chunk
 

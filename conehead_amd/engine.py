@@ -133,6 +133,26 @@ def convolve_batch(jobs, stream=None):
     _lib.check(_lib.lib().ch_convolve_batch(n, arr, _h(stream)))
 
 
+def convolve_groups(groups, stream=None):
+    """Beam groups on shared plans: each item is (plan, [(terma, dose, box),
+    ...] with 1..4 beams, accumulate)."""
+    n = len(groups)
+    if n == 0:
+        return
+    arr = (_lib.ConvGroup * n)()
+    for i, (plan, beams, acc) in enumerate(groups):
+        if not 1 <= len(beams) <= _lib.CH_MAX_GROUP:
+            raise ValueError("a group holds 1..%d beams" % _lib.CH_MAX_GROUP)
+        arr[i].plan = plan.handle
+        arr[i].n_beams = len(beams)
+        arr[i].accumulate = 1 if acc else 0
+        for b, (t, d, box) in enumerate(beams):
+            arr[i].d_terma[b] = t.ptr
+            arr[i].d_dose[b] = d.ptr
+            arr[i].d_box[b] = box.ptr if box is not None else None
+    _lib.check(_lib.lib().ch_convolve_groups(n, arr, _h(stream)))
+
+
 class ApertureRef:
     """A ch_aperture plus the device buffers it points into."""
 

@@ -162,6 +162,25 @@ typedef struct ch_conv_job {
 ch_status ch_convolve_batch(int32_t n_jobs, const ch_conv_job* h_jobs,
                             void* stream);
 
+/* Up to CH_MAX_GROUP beams on the same plan convolved together: their
+ * TERMA grids are interleaved so each tap is one 16-byte load for all of
+ * them, and the tap walk is shared.  Tap ranges use the union of the
+ * beams' support boxes (d_box[b] nullable: whole grid).  Each beam's dose
+ * is written to d_dose[b] (accumulate != 0 adds).  The result equals
+ * ch_convolve's to FP32 rounding (the union box can shift the blocks of
+ * the compensated sum); it is deterministic for a given group. */
+#define CH_MAX_GROUP 4
+typedef struct ch_conv_group {
+    const ch_plan* plan;
+    int32_t n_beams;
+    int32_t accumulate;
+    const float* d_terma[CH_MAX_GROUP];
+    float* d_dose[CH_MAX_GROUP];
+    const int32_t* d_box[CH_MAX_GROUP];
+} ch_conv_group;
+ch_status ch_convolve_groups(int32_t n_groups, const ch_conv_group* h_groups,
+                             void* stream);
+
 /* Tile shape used by the gather kernels for this plan; set by the
  * autotuner.  1 x TY x TZ (TY in {1, 2, 4, 8}, TY * TZ = 64) selects the
  * direct gather, one wavefront per tile; 4 x 16 x 16 selects the staged

@@ -203,6 +203,47 @@ def test_staged_batch_mixed_tiles_bitwise(engine, kern):
         assert np.array_equal(d2.download(), s)
 
 
+def test_beam_groups_match_single(engine, kern):
+    """Interleaved beam groups (ch_convolve_groups): every beam matches its
+    own ch_convolve to FP32 rounding, a one-beam group matches bit for bit,
+    groups of different plans and tiles share a call, accumulate adds."""
+    from conehead_amd import engine as E
+    from conehead_amd.device import DeviceArray
+    rng = np.random.default_rng(21)
+    groups, want = [], []
+    for shape, nphi, tile, nb in [((40, 36, 44), 6, (1, 4, 16), 4),
+                                  ((23, 50, 31), 4, (1, 2, 32), 3),
+                                  ((30, 30, 30), 4, STAGED, 2),
+                                  ((17, 19, 70), 8, (1, 4, 16), 1)]:
+        cs, _ = cones(kern, 6, nphi)
+        plan = engine.plan(shape, (0.3, 0.35, 0.25), cs, tile=tile)
+        beams = []
+        for b in range(nb):
+            T = beam_like_terma(shape, rng, frac=0.3 + 0.15 * b)
+            T = np.roll(T, 3 * b, axis=b % 3).astype(np.float32)
+            d_t = DeviceArray.from_host(T)
+            d_box = DeviceArray((6,), np.int32)
+            E.support_box(d_t, shape, d_box)
+            single = DeviceArray(shape, np.float32)
+            E.convolve(plan, d_t, single, d_box)
+            want.append((single.download(), nb == 1))
+            beams.append((d_t, DeviceArray(shape, np.float32), d_box))
+        groups.append((plan, beams, False))
+    E.convolve_groups(groups)
+    got = [d.download() for _, beams, _ in groups for (_, d, _) in beams]
+    for g, (w, exact) in zip(got, want):
+        if exact:
+            assert np.array_equal(g, w)
+        assert rel_err(g, w) < 1e-6
+    # accumulate: dose += the same convolution again
+    plan, beams, _ = groups[0]
+    E.convolve_groups([(plan, beams, True)])
+    for (_, d, _), (w, _) in zip(beams, want[:len(beams)]):
+        assert rel_err(d.download(), 2.0 * w) < 1e-6
+    with pytest.raises(Exception):
+        E.convolve_groups([(plan, beams + beams, False)])
+
+
 def test_accumulate_and_empty_terma(engine, kern):
     from conehead_amd import engine as E
     from conehead_amd.device import DeviceArray

@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 from conehead_amd import workloads as W
-from conehead_amd.batch import DoseRequest
+from conehead_amd.batch import DoseRequest, group_beams
 from conehead_amd.serve import AdmissionQueue, poisson_arrivals
 
 
@@ -30,9 +30,10 @@ def test_request_accounting():
     X, Y, Z = c.shape
     assert r.n_jobs == len(c.beams)
     assert DoseRequest.from_case(c, per_beam=False).n_jobs == 1
-    padded = X * (Y + 16) * (Z + 128)
+    padded = X * (Y + 64) * (Z + 128)
+    slots = -(-r.n_jobs // 4) * 4
     assert r.device_bytes() == 4 * (X * Y * Z + r.n_jobs *
-                                    (2 * X * Y * Z + padded + 8))
+                                    (2 * X * Y * Z + 8) + slots * padded)
     assert r.voxel_cone_updates() == X * Y * Z * c.cones.n_cones * \
         len(c.beams)
 
@@ -79,3 +80,21 @@ def test_poisson_arrivals():
     gaps = np.diff(np.concatenate([[0.0], t]))
     assert abs(gaps.mean() - 1 / 200.0) < 0.05 / 200.0
     assert abs(gaps.std() / gaps.mean() - 1.0) < 0.05   # exponential CV=1
+
+
+def test_group_beams_cost_model():
+    same = [(0, 0, 0, 9, 9, 9)] * 4
+    assert group_beams(same) == [[0, 1, 2, 3]]
+    # two far-apart clusters of three: each cluster groups, nothing mixes
+    a = [(0, 0, 0, 9, 9, 9), (0, 0, 1, 9, 9, 9), (1, 0, 0, 9, 9, 9)]
+    b = [(50, 50, 50, 59, 59, 59)] * 3
+    parts = group_beams(a + b)
+    assert sorted(map(sorted, parts)) == [[0, 1, 2], [3, 4, 5]]
+    # disjoint boxes stay single; two beams never form a group
+    far = [(0, 0, 0, 4, 4, 4), (20, 20, 20, 24, 24, 24),
+           (40, 40, 40, 44, 44, 44)]
+    assert sorted(group_beams(far)) == [[0], [1], [2]]
+    assert sorted(group_beams(same[:2])) == [[0], [1]]
+    parts = group_beams([(0, 0, 0, 9, 9, 9)] * 6)
+    assert sorted(len(p) for p in parts) in ([1, 1, 4], [3, 3])
+    assert sorted(i for p in parts for i in p) == list(range(6))
