@@ -165,6 +165,21 @@ def run_hetero(args, eng):
     dose = d_d.download(stream=s)
     terma_gpu = d_t.download(stream=s)
 
+    # the same step captured once as a HIP graph and replayed K times
+    with E.Graph.capture(s) as cap:
+        E.terma([beam], shape, case.origin, case.spacing, stream=s,
+                check=False)
+        E.convolve(plan, d_t, d_d, d_box, stream=s)
+    graph = cap.graph
+    graph.launch(s)
+    s.sync()
+    t0 = time.perf_counter()
+    for _ in range(K):
+        graph.launch(s)
+    s.sync()
+    graph_ms = (time.perf_counter() - t0) * 1e3 / K
+    graph.close()
+
     # the convolve_c drop-in on host float64 buffers: PCIe-inclusive rate
     from conehead_amd.convolve_c import convolve_c
     from conehead_amd.kernel2 import merged_kernel
@@ -218,6 +233,7 @@ def run_hetero(args, eng):
         "latency_ms": {"p50": float(np.percentile(step_ms, 50)),
                        "p99": float(np.percentile(step_ms, 99))},
         "gpu_ms_per_step": gpu_total_ms / K,
+        "graph_ms_per_step": graph_ms,
         "gather_ms": g_ms,
         "terma_ms": float((step_ms - gather_ms).mean()),
         "error_vs_fp64_oracle": result_err,

@@ -165,6 +165,10 @@ SIGNATURES = {
     "ch_convolve_f64": (ctypes.c_int32, [c_vp, c_vp, c_vp, c_vp, c_vp]),
     "ch_convolve_batch": (ctypes.c_int32, [ctypes.c_int32,
                                            ctypes.POINTER(ConvJob), c_vp]),
+    "ch_graph_begin": (ctypes.c_int32, [c_vp]),
+    "ch_graph_end": (ctypes.c_int32, [c_vp, ctypes.POINTER(c_vp)]),
+    "ch_graph_launch": (ctypes.c_int32, [c_vp, c_vp]),
+    "ch_graph_destroy": (ctypes.c_int32, [c_vp]),
     "ch_convolve_groups": (ctypes.c_int32, [ctypes.c_int32,
                                             ctypes.POINTER(ConvGroup), c_vp]),
     "ch_convolve_scatter": (ctypes.c_int32, [

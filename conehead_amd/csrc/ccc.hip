@@ -834,6 +834,15 @@ static ch_status workspace(hipStream_t s, size_t pad_elems, size_t tab_bytes,
         CH_HIP(hipEventCreateWithFlags(&w->jobs_done, hipEventDisableTiming));
         g_ws.push_back(w);
     }
+    if (capturing(s) && (w->pad_elems < pad_elems || tab_bytes > 0)) {
+        set_error(tab_bytes > 0
+                      ? "batched gathers stage a host job table and cannot "
+                        "be captured; capture ch_convolve calls"
+                      : "the gather workspace must grow while the stream is "
+                        "being captured: run the pipeline once before "
+                        "capturing it");
+        return CH_ERR_ARG;
+    }
     if (w->pad_elems < pad_elems) {
         // the old buffer may still be read by queued kernels of this stream
         CH_HIP(hipStreamSynchronize(s));

@@ -95,6 +95,9 @@ struct ConvJob {
 
 void record_kernel_ms(float ms);
 bool timing_enabled();
+// True while `s` is being captured into a graph (no allocation, no
+// synchronisation may happen then).
+bool capturing(hipStream_t s);
 
 // Optional HIP-event timing of one launch (ch_set_timing): stop() waits for
 // the launch and records its duration for ch_last_kernel_ms().
@@ -102,7 +105,8 @@ struct LaunchTimer {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hipStream_t s;
     bool on;
-    explicit LaunchTimer(hipStream_t st) : s(st), on(timing_enabled())
+    explicit LaunchTimer(hipStream_t st)
+        : s(st), on(timing_enabled() && !capturing(st))
     {
         if (on) {
             (void)hipEventCreate(&e0);

@@ -47,6 +47,14 @@ int32_t num_cus()
 }
 
 void record_kernel_ms(float ms) { g_last_kernel_ms = ms; }
+
+bool capturing(hipStream_t s)
+{
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &st) != hipSuccess)
+        return false;
+    return st == hipStreamCaptureStatusActive;
+}
 bool timing_enabled() { return g_timing; }
 
 // Grid-stride zero fill in 16-byte units plus a byte tail.
@@ -146,6 +154,11 @@ ch_status stream_scratch(hipStream_t s, int slot, size_t bytes, void** out)
         e = &g_scratch.back();
     }
     if (e->bytes[slot] < bytes) {
+        if (capturing(s)) {
+            set_error("stream scratch must grow while the stream is being "
+                      "captured: run the pipeline once before capturing it");
+            return CH_ERR_ARG;
+        }
         // queued kernels of this stream may still use the old buffer
         CH_HIP(hipStreamSynchronize(s));
         CH_HIP(hipFree(e->ptr[slot]));
@@ -332,6 +345,57 @@ ch_status ch_stream_destroy(void* stream)
         release_stream_scratch(s);
         CH_HIP(hipStreamDestroy(s));
     }
+    return CH_OK;
+}
+
+ch_status ch_graph_begin(void* stream)
+{
+    CH_REQUIRE(stream, "ch_graph_begin: capture needs a created stream");
+    CH_HIP(hipStreamBeginCapture((hipStream_t)stream,
+                                 hipStreamCaptureModeThreadLocal));
+    return CH_OK;
+}
+
+struct ch_graph {
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+};
+
+ch_status ch_graph_end(void* stream, ch_graph** graph_out)
+{
+    CH_REQUIRE(stream && graph_out, "ch_graph_end: null argument");
+    *graph_out = nullptr;
+    hipGraph_t g = nullptr;
+    // always ends the capture, also when a call inside it failed
+    CH_HIP(hipStreamEndCapture((hipStream_t)stream, &g));
+    CH_REQUIRE(g, "ch_graph_end: the capture produced no graph");
+    hipGraphExec_t x = nullptr;
+    const hipError_t e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return hip_fail(e, "hipGraphInstantiate", __FILE__, __LINE__);
+    }
+    ch_graph* out = new ch_graph();
+    out->graph = g;
+    out->exec = x;
+    *graph_out = out;
+    return CH_OK;
+}
+
+ch_status ch_graph_launch(const ch_graph* graph, void* stream)
+{
+    CH_REQUIRE(graph, "ch_graph_launch: null graph");
+    CH_HIP(hipGraphLaunch(graph->exec, (hipStream_t)stream));
+    return CH_OK;
+}
+
+ch_status ch_graph_destroy(ch_graph* graph)
+{
+    if (!graph)
+        return CH_OK;
+    (void)hipGraphExecDestroy(graph->exec);
+    (void)hipGraphDestroy(graph->graph);
+    delete graph;
     return CH_OK;
 }
 

@@ -153,6 +153,52 @@ def convolve_groups(groups, stream=None):
     _lib.check(_lib.lib().ch_convolve_groups(n, arr, _h(stream)))
 
 
+class Graph:
+    """A captured pipeline (ch_graph_begin ... ch_graph_end), replayed with
+    launch().  Use as:  with Graph.capture(stream) as g: <calls on stream>."""
+
+    def __init__(self, handle):
+        self.handle = handle
+
+    @classmethod
+    def capture(cls, stream):
+        return _Capture(stream)
+
+    def launch(self, stream):
+        _lib.check(_lib.lib().ch_graph_launch(self.handle, _h(stream)))
+
+    def close(self):
+        if self.handle:
+            _lib.lib().ch_graph_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class _Capture:
+    def __init__(self, stream):
+        self.stream = stream
+        self.graph = None
+
+    def __enter__(self):
+        _lib.check(_lib.lib().ch_graph_begin(_h(self.stream)))
+        return self
+
+    def __exit__(self, exc_type, exc, tb):
+        h = ctypes.c_void_p()
+        st = _lib.lib().ch_graph_end(_h(self.stream), ctypes.byref(h))
+        if exc_type is None:
+            _lib.check(st)
+            self.graph = Graph(h.value)
+        elif h.value:
+            _lib.lib().ch_graph_destroy(h.value)
+        return False
+
+
 class ApertureRef:
     """A ch_aperture plus the device buffers it points into."""
 

@@ -208,6 +208,25 @@ ch_status ch_convolve_scatter(const float* d_terma, float* d_dose,
                               int32_t row_len, int32_t semantics,
                               void* stream);
 
+/* ------------------------------------------------------------ graphs --- */
+
+/* Capture of a pipeline of calls on `stream` into a HIP graph, replayed by
+ * one launch.  Between ch_graph_begin and ch_graph_end the calls made on
+ * `stream` (ch_terma, ch_support_box, ch_convolve, ch_convolve_f64,
+ * ch_zero, ch_convolve_scatter) are recorded, not run: every one of them
+ * is a kernel launch (no memset, no host copy), with the device pointers
+ * and sizes it was given.  Run the same pipeline once before capturing it,
+ * so its stream scratch and gather workspace exist; a call that would have
+ * to allocate or synchronise fails with CH_ERR_ARG instead, and the batch
+ * entry points (host job tables) always do.  ch_graph_end ends the capture
+ * in every case.  No counterpart in the reference (its flow re-launches
+ * every kernel from Python, conehead/conehead.py:536-682). */
+typedef struct ch_graph ch_graph;
+ch_status ch_graph_begin(void* stream);
+ch_status ch_graph_end(void* stream, ch_graph** graph_out);
+ch_status ch_graph_launch(const ch_graph* graph, void* stream);
+ch_status ch_graph_destroy(ch_graph* graph);
+
 /* ------------------------------------------------------------ TERMA ---- */
 
 #define CH_MAX_ENERGIES 32
