@@ -83,6 +83,64 @@ __device__ __forceinline__ float mlc_value(const int32_t* __restrict__ t,
     return v;
 }
 
+// Transmission of every cell of the block [i0, i1] x [j0, j1] (raster
+// rows and columns, inside the raster) when it is one constant, else -1.
+// A leaf table is constant on rectangles that avoid the bank edge, leaf
+// edge rows, tips, tapers and jaw edges: the open gap of one leaf pair, or
+// anything outside the bank or the jaws (closed).
+__device__ __forceinline__ float mlc_block_value(const int32_t* __restrict__ t,
+                                                 int i0, int i1, int j0,
+                                                 int j1)
+{
+    const int n_pairs = t[0], W = t[1];
+    const int r0 = i0 - t[2], r1 = i1 - t[2];
+    if (r1 < 0 || r0 >= W)
+        return 0.0f;  // outside the bank rows
+    if (j1 < 2000 + t[3] || j0 >= 2000 + t[4] || r1 < t[5] || r0 >= t[6])
+        return 0.0f;  // behind a jaw
+    if (r0 < 0 || r1 >= W || r0 < t[5] || r1 >= t[6] || j0 < 2000 + t[3] ||
+        j1 >= 2000 + t[4])
+        return -1.0f;  // straddles a bank or jaw edge
+    const int32_t* bound = t + kMlcHeader;
+    const int32_t* ea = bound + n_pairs + 1;
+    const int32_t* eb = ea + n_pairs;
+    const int shift = abs(bound[0]);
+    if (bound[0] + shift > r0)
+        return -1.0f;
+    int lo = 0, hi = n_pairs - 1;  // the leaf pair holding row r0
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (bound[mid] + shift <= r0)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const int rmin = bound[lo] + shift;
+    const int w = abs(bound[lo + 1] - bound[lo]);
+    if (r1 >= rmin + w)
+        return -1.0f;  // spans two leaf pairs
+    // open gap between the tips: value 1 (mlc_value leaves v = 1 there)
+    if (j0 >= 2000 + ea[lo] && j1 < 2000 + eb[lo])
+        return 1.0f;
+    return -1.0f;
+}
+
+__device__ __forceinline__ float rects_block_value(const int32_t* __restrict__ t,
+                                                   int n, int i0, int i1,
+                                                   int j0, int j1)
+{
+    bool any = false;
+    for (int k = 0; k < n; ++k) {
+        const int a = t[4 * k], b = t[4 * k + 1], c = t[4 * k + 2],
+                  d = t[4 * k + 3];
+        if (i0 >= a && i1 < b && j0 >= c && j1 < d)
+            return 1.0f;  // inside one rectangle
+        if (i1 >= a && i0 < b && j1 >= c && j0 < d)
+            any = true;   // touches this rectangle
+    }
+    return any ? -1.0f : 0.0f;
+}
+
 // Union of half-open rectangles [r0, r1) x [c0, c1) of ones
 // (Block.set_square, conehead/block.py:73-77), layout n x 4.
 __device__ __forceinline__ float rects_value(const int32_t* __restrict__ t,

@@ -126,14 +126,23 @@ __global__ __launch_bounds__(256) void terma_rays_kernel(
     count[r] = m;
 }
 
+// Thread layout: a workgroup covers a 4 x 4 x 16 brick of voxels, each
+// wavefront a 4 x 4 x 4 cube, so a wavefront's voxels are either all well
+// inside, all outside or on the edge of the field (the corner shortcut of
+// the hit test pays for whole wavefronts) and their rays toward the source
+// have similar lengths.
 __global__ __launch_bounds__(256) void terma_kernel(TermaParams P)
 {
     const int64_t nv = (int64_t)P.X * P.Y * P.Z;
-    const int64_t li = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    const bool inside = li < nv;
-    const int z = inside ? (int)(li % P.Z) : 0;
-    const int y = inside ? (int)((li / P.Z) % P.Y) : 0;
-    const int x = inside ? (int)(li / ((int64_t)P.Z * P.Y)) : 0;
+    const int nbz = (P.Z + 15) / 16, nby = (P.Y + 3) / 4;
+    const int bz = blockIdx.x % nbz;
+    const int br = blockIdx.x / nbz;
+    const int l = threadIdx.x & 63;
+    const int x = (br / nby) * 4 + (l >> 4);
+    const int y = (br % nby) * 4 + ((l >> 2) & 3);
+    const int z = bz * 16 + (threadIdx.x >> 6) * 4 + (l & 3);
+    const bool inside = x < P.X && y < P.Y && z < P.Z;
+    const int64_t li = ((int64_t)x * P.Y + y) * P.Z + z;
     float T = 0.0f;
     if (inside) {
         double pos[3];
@@ -181,7 +190,70 @@ __global__ __launch_bounds__(256) void terma_kernel(TermaParams P)
                        cj + m < P.ap_bbox[2] || cj - m > P.ap_bbox[3];
             }
         }
-        for (int ix = 0; ix < ns && !skip; ++ix)
+        // Corner shortcut: the sub-samples form a lattice whose points are
+        // convex combinations of its 8 corners, and the projection through
+        // the source onto the isocentre plane keeps convex combinations
+        // while n.u keeps one sign (n.u is affine in the point).  So when
+        // the 8 corner images (computed exactly as below) have one sign of
+        // n.u and their cell box, widened by 1e-6 cell for rounding, has one
+        // constant transmission, every sample has it and the blocked
+        // fraction is that value added n^3 times, as the loop would.
+        bool done = skip;
+        if (!skip && P.ap_kind != CH_APERTURE_RASTER) {
+            double lo_i = 1e300, hi_i = -1e300, lo_j = 1e300, hi_j = -1e300;
+            int sgn = 0;
+            bool ok = true;
+            for (int k = 0; k < 8 && ok; ++k) {
+                const int ix = (k & 1) ? ns - 1 : 0;
+                const int iy = (k & 2) ? ns - 1 : 0;
+                const int iz = (k & 4) ? ns - 1 : 0;
+                const double ps0 = pos[0] - P.spacing[0] / 2 + off[0] / 2 + off[0] * ix;
+                const double ps1 = pos[1] - P.spacing[1] / 2 + off[1] / 2 + off[1] * iy;
+                const double ps2 = pos[2] - P.spacing[2] / 2 + off[2] / 2 + off[2] * iz;
+                const double u0 = P.src[0] - ps0, u1 = P.src[1] - ps1,
+                             u2 = P.src[2] - ps2;
+                const double ndotu = dot3(P.vy, u0, u1, u2);
+                const int sg = ndotu > 0 ? 1 : -1;
+                if (fabs(ndotu) < 1e-6 || (sgn != 0 && sg != sgn)) {
+                    ok = false;
+                    break;
+                }
+                sgn = sg;
+                const double si = -ndotn_src / ndotu;
+                const double pp0 = P.src[0] + si * u0 + 0.0;
+                const double pp1 = P.src[1] + si * u1 + 0.0;
+                const double pp2 = P.src[2] + si * u2 + 0.0;
+                const double ci = dot3(P.tr0, pp0, pp1, pp2) * 100.0;
+                const double cj = dot3(P.tr2, pp0, pp1, pp2) * 100.0;
+                lo_i = fmin(lo_i, ci);
+                hi_i = fmax(hi_i, ci);
+                lo_j = fmin(lo_j, cj);
+                hi_j = fmax(hi_j, cj);
+            }
+            // cells floor(c) + 1999 of the widened box, inside the raster
+            // (no wrap-around, no out-of-raster zeros)
+            const double a_i = floor(lo_i - 1e-6) + 1999.0;
+            const double b_i = floor(hi_i + 1e-6) + 1999.0;
+            const double a_j = floor(lo_j - 1e-6) + 1999.0;
+            const double b_j = floor(hi_j + 1e-6) + 1999.0;
+            if (ok && a_i >= 0.0 && b_i <= 3998.0 && a_j >= 0.0 &&
+                b_j <= 3998.0) {
+                const int i0 = (int)a_i, i1 = (int)b_i, j0 = (int)a_j,
+                          j1 = (int)b_j;
+                const float v =
+                    P.ap_kind == CH_APERTURE_MLC
+                        ? mlc_block_value(P.ap_table, i0, i1, j0, j1)
+                        : rects_block_value(P.ap_table, P.ap_n, i0, i1, j0,
+                                            j1);
+                if (v >= 0.0f) {
+                    const double inc = (double)v / n3;
+                    for (int k = 0; k < ns * ns * ns; ++k)
+                        blocked = blocked + inc;
+                    done = true;
+                }
+            }
+        }
+        for (int ix = 0; ix < ns && !done; ++ix)
             for (int iy = 0; iy < ns; ++iy)
                 for (int iz = 0; iz < ns; ++iz) {
                     const double ps0 = pos[0] - P.spacing[0] / 2 + off[0] / 2 + off[0] * ix;
@@ -392,7 +464,9 @@ ch_status ch_terma(int32_t n_beams, const ch_beam* h_beams,
         P.accumulate = B.accumulate;
         P.debug = d_debug ? d_debug + (size_t)b * 4 * nv : nullptr;
         P.status = (int32_t*)status;
-        const unsigned blocks = (unsigned)((nv + 255) / 256);
+        const unsigned blocks = (unsigned)(((h_shape[0] + 3) / 4) *
+                                           ((h_shape[1] + 3) / 4) *
+                                           ((h_shape[2] + 15) / 16));
         P.box = nullptr;
         if (B.d_box) {
             void* parts = nullptr;
