@@ -26,7 +26,7 @@ from . import engine as E
 from .device import DeviceArray, Event, Stream, _h
 
 PAD_Y = 32   # kYPad of the padded TERMA copy (ch_internal.h)
-PAD_Z = 64   # kZPad
+PAD_Z = 80   # kZPad
 
 
 class DoseRequest:

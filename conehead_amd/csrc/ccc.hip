@@ -123,14 +123,43 @@ __device__ __forceinline__ void accumulate_chunk(float& acc, float& ac,
     }
 }
 
+// Tap range [a, b) of one axis, per lane (vector loads; the cone differs
+// between lanes).  Same clamping as urange.
+__device__ __forceinline__ void vrange(int v, int lo, int hi, int step,
+                                       const int32_t* __restrict__ f,
+                                       int f_len, int& a, int& b)
+{
+    int L, U;
+    if (step > 0) {
+        L = v - hi;
+        U = v - lo;
+    } else {
+        L = lo - v;
+        U = hi - v;
+    }
+    const int last = f_len - 1;
+    L = L < 0 ? 0 : (L > last ? last : L);
+    U = U + 1 < 0 ? 0 : (U + 1 > last ? last : U + 1);
+    a = f[L];
+    b = f[U];
+}
+
+constexpr int kBufferFlags = 0x00020000;  // raw 32-bit buffer resource
+
 // One wavefront owns a 1 x TY x TZ output tile (TY * TZ = 64 lanes, z
-// fastest).  Returns the tile's T reads relative to the padded allocation
-// through base4 (bytes); every tap applies to every lane.
+// fastest).  The tile's union tap range of 64 cones at a time is computed
+// in parallel, one lane per cone, and read back with readlane, so the
+// scalar unit only walks taps.  Taps are applied in whole chunks of
+// kUnroll: a tap past the union end reads a zero (its source lies outside
+// the support box along some axis for every lane, within the y/z padding,
+// or outside the grid along x, where the range-checked buffer load returns
+// 0), so no chunk needs a mask.
 template <int TY, int TZ, int ACC, typename OutT>
 __device__ __forceinline__ void gather_tile(const ConvJob& j, int tile,
                                             int lane)
 {
-    static_assert(TY * TZ == 64 && TY <= kYPad && TZ <= kZPad, "tile");
+    static_assert(TY * TZ == 64 && TY <= kYPad && TZ + kUnroll <= kZPad,
+                  "tile");
     const int tz = tile % j.ntz;
     const int r = tile / j.ntz;
     const int ty = r % j.nty;
@@ -157,83 +186,62 @@ __device__ __forceinline__ void gather_tile(const ConvJob& j, int tile,
     // byte offset of this lane's own voxel in the padded T allocation
     const uint32_t base4 =
         (uint32_t)(((x * Yp + y + kYPad) * Zp + z + kZPad) * 4);
-    const char* __restrict__ Tb = reinterpret_cast<const char*>(j.terma) -
-                                  (size_t)(kYPad * Zp + kZPad) * 4;
-    const c_meta* meta = as_const(j.meta);
-    const c_i32* ftab = as_const(j.ftab);
+    const float* alloc = j.terma - ((size_t)kYPad * Zp + kZPad);
+    const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)alloc, (short)0, j.X * Yp * Zp * 4, kBufferFlags);
     const c_i32* lin = as_const(j.lin);
     const c_f32* kk = as_const(j.k);
 
     float sum = 0.0f, comp = 0.0f;
     if (lo0 <= hi0 && lo1 <= hi1 && lo2 <= hi2) {
-        for (int c = 0; c < j.n_cones; ++c) {
-            const c_meta* m = meta + c;
-            int ax, bx, ay1, by1, ay2, by2, az1, bz1, az2, bz2;
-            urange(x, lo0, hi0, m->step[0], ftab + m->f_begin[0],
-                   m->f_len[0], ax, bx);
-            urange(y0, lo1, hi1, m->step[1], ftab + m->f_begin[1],
-                   m->f_len[1], ay1, by1);
-            urange(z0, lo2, hi2, m->step[2], ftab + m->f_begin[2],
-                   m->f_len[2], az1, bz1);
-            if (TY > 1)
-                urange(yl, lo1, hi1, m->step[1], ftab + m->f_begin[1],
+        for (int cb = 0; cb < j.n_cones; cb += 64) {
+            // lane l: union range and tap list of cone cb + l
+            int r_lo = 0, r_hi = 0, r_tb = 0;
+            const int c = cb + lane;
+            if (c < j.n_cones) {
+                const ConeMeta* m = j.meta + c;
+                const int32_t* f = j.ftab;
+                int ax, bx, ay1, by1, ay2, by2, az1, bz1, az2, bz2;
+                vrange(x, lo0, hi0, m->step[0], f + m->f_begin[0],
+                       m->f_len[0], ax, bx);
+                vrange(y0, lo1, hi1, m->step[1], f + m->f_begin[1],
+                       m->f_len[1], ay1, by1);
+                vrange(yl, lo1, hi1, m->step[1], f + m->f_begin[1],
                        m->f_len[1], ay2, by2);
-            else {
-                ay2 = ay1;
-                by2 = by1;
+                vrange(z0, lo2, hi2, m->step[2], f + m->f_begin[2],
+                       m->f_len[2], az1, bz1);
+                vrange(zl, lo2, hi2, m->step[2], f + m->f_begin[2],
+                       m->f_len[2], az2, bz2);
+                // union of the tile's per-voxel ranges (each axis is
+                // monotone, so its extremes sit at the tile's end points)
+                r_lo = max(max(ax, min(ay1, ay2)), min(az1, az2));
+                r_hi = min(min(bx, max(by1, by2)), max(bz1, bz2));
+                r_tb = m->tap_begin;
             }
-            urange(zl, lo2, hi2, m->step[2], ftab + m->f_begin[2],
-                   m->f_len[2], az2, bz2);
-            // union of the tile's per-voxel ranges (each axis is monotone,
-            // so its extremes sit at the tile's end points)
-            const int ulo = max(max(ax, min(ay1, ay2)), min(az1, az2));
-            const int uhi = min(min(bx, max(by1, by2)), max(bz1, bz2));
-            if (ulo >= uhi)
-                continue;
-            const c_i32* ml = lin + m->tap_begin;
-            const c_f32* mk = kk + m->tap_begin;
-            float acc = 0.0f, ac = 0.0f;
-            int o[kUnroll];
-            float kv[kUnroll];
+            const int nc = min(64, j.n_cones - cb);
+            for (int i = 0; i < nc; ++i) {
+                const int ulo = __builtin_amdgcn_readlane(r_lo, i);
+                const int uhi = __builtin_amdgcn_readlane(r_hi, i);
+                if (ulo >= uhi)
+                    continue;
+                const int tb = __builtin_amdgcn_readlane(r_tb, i);
+                const c_i32* ml = lin + tb;
+                const c_f32* mk = kk + tb;
+                float acc = 0.0f, ac = 0.0f;
+                for (int t = ulo; t < uhi; t += kUnroll) {
+                    float v[kUnroll], kc[kUnroll];
 #pragma unroll
-            for (int u = 0; u < kUnroll; ++u) {
-                o[u] = ml[ulo + u];
-                kv[u] = mk[ulo + u];
-            }
-            // whole chunks: no per-tap condition at all
-            const int tend = ulo + (uhi - ulo) / kUnroll * kUnroll;
-            int t = ulo;
-            for (; t < tend; t += kUnroll) {
-                float v[kUnroll], kc[kUnroll];
-#pragma unroll
-                for (int u = 0; u < kUnroll; ++u) {
-                    v[u] = *reinterpret_cast<const float*>(
-                        Tb + (base4 - (uint32_t)o[u]));
-                    kc[u] = kv[u];
+                    for (int u = 0; u < kUnroll; ++u) {
+                        kc[u] = mk[t + u];
+                        v[u] = __int_as_float(
+                            __builtin_amdgcn_raw_buffer_load_b32(
+                                rsrc, (int)(base4 - (uint32_t)ml[t + u]), 0,
+                                0));
+                    }
+                    accumulate_chunk<ACC>(acc, ac, kc, v);
                 }
-                // next chunk's taps while the loads are in flight (the
-                // padding keeps t + 2 kUnroll - 1 readable)
-#pragma unroll
-                for (int u = 0; u < kUnroll; ++u) {
-                    o[u] = ml[t + kUnroll + u];
-                    kv[u] = mk[t + kUnroll + u];
-                }
-                accumulate_chunk<ACC>(acc, ac, kc, v);
+                kahan_add(sum, comp, acc);
             }
-            // last partial chunk: taps past uhi are real taps of the cone
-            // and are masked (uniformly) to offset 0, increment 0
-            if (t < uhi) {
-                float v[kUnroll], kc[kUnroll];
-#pragma unroll
-                for (int u = 0; u < kUnroll; ++u) {
-                    const bool in = t + u < uhi;
-                    v[u] = *reinterpret_cast<const float*>(
-                        Tb + (base4 - (uint32_t)(in ? o[u] : 0)));
-                    kc[u] = in ? kv[u] : 0.0f;
-                }
-                accumulate_chunk<ACC>(acc, ac, kc, v);
-            }
-            kahan_add(sum, comp, acc);
         }
     }
     if (y <= yl && z <= zl) {
@@ -291,8 +299,6 @@ __global__ __launch_bounds__(256) void gather_batch(const ConvJob* jobs,
 // rows of 48 floats the four y rows a wavefront reads per tap fall in
 // disjoint bank ranges.
 // Sums: plain FMA over a chunk's taps, Kahan-compensated across chunks.
-
-constexpr int kBufferFlags = 0x00020000;  // raw 32-bit buffer resource
 
 template <typename OutT>
 __device__ __forceinline__ void staged_block(const ConvJob& j, int blk,

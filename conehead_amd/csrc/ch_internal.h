@@ -57,10 +57,11 @@ struct ConeMeta {
 
 // Zero voxels on both ends of every z-row (kZPad) and every y-column
 // (kYPad) of the padded TERMA copy the gathers read.  They bound how far a
-// staged box may reach past the TERMA support along y and z (block extent
-// - 1 plus the chunk's offset change); x needs no padding because staged
-// loads are range-checked buffer loads.
-constexpr int32_t kZPad = 64;
+// read may reach past the TERMA support along y and z: a tile's extent - 1
+// plus the unrolled chunk's overrun past the union range (direct gather),
+// or plus a chunk's offset change (staged gather); x needs no padding
+// because the gathers read T with range-checked buffer loads.
+constexpr int32_t kZPad = 80;
 constexpr int32_t kYPad = 32;
 
 // Staged gather: one 256-thread workgroup per 4 x 16 x 16 output block, four

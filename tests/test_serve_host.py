@@ -30,7 +30,7 @@ def test_request_accounting():
     X, Y, Z = c.shape
     assert r.n_jobs == len(c.beams)
     assert DoseRequest.from_case(c, per_beam=False).n_jobs == 1
-    padded = X * (Y + 64) * (Z + 128)
+    padded = X * (Y + 64) * (Z + 160)
     slots = -(-r.n_jobs // 4) * 4
     assert r.device_bytes() == 4 * (X * Y * Z + r.n_jobs *
                                     (2 * X * Y * Z + 8) + slots * padded)
