@@ -101,3 +101,21 @@ def test_device_square_raster_matches_host(engine):
     blk.set_square(2.0)
     d = engine.raster(blk).download()
     assert np.array_equal(d, blk.block_values)
+
+
+def test_dicom_plan_mlc(oracle, engine):
+    """The MLC and jaws of the reference's sample RT plan
+    (RP.3DCRT.dcm, conehead/block.py:87-215): the analytic leaf table, the
+    materialised raster and the oracle agree."""
+    import os
+    from conehead_amd import dicom
+    plan = dicom.read_file(os.path.join(os.path.dirname(__file__), "golden",
+                                        "RP.3DCRT.dcm"))
+    blk = Block()
+    blk._set_from_plan(plan)
+    n, s = 40, 0.4
+    shape = (n, n, n)
+    origin = np.array([-(n - 1) / 2 * s, s / 2, -(n - 1) / 2 * s])
+    dens = hetero_densities(n, s)
+    T, ref = compare(oracle, engine, shape, [s] * 3, origin, dens, 0.0, blk)
+    assert T.max() > 0 and (T == 0).any()
