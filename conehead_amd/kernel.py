@@ -59,3 +59,83 @@ class KernelPoly:
         # as given (conehead/kernel.py:62-63); callers pass a normalised one.
         self.kernel_diff = kernel_diff / kernel_diff.sum()
         self.kernel_cum = _cumulative_table(radii, kernel_diff)
+
+
+# Incident energies (MeV) of the EDKnrc listings, as settings keys.
+ENERGIES = ("0.5", "1.0", "1.5", "2.0", "2.5", "3.0", "3.5", "4.0", "4.5",
+            "5.0", "5.5", "6.0")
+
+
+def listing_files(root, reference_1mev=True):
+    """Listing path per energy key under `root` (the kernels/ directory).
+    With reference_1mev the "1.0" entry reads the 1.5 MeV listing, as
+    conehead/conehead.py:688 does; False reads the 1.0 MeV one."""
+    import os
+    out = {}
+    for e in ENERGIES:
+        d = "1.5" if (e == "1.0" and reference_1mev) else e
+        out[e] = os.path.join(root, "%sMeV" % d, "%sMeV.egslst" % d)
+    return out
+
+
+def polyenergetic(kernels, energy_weights):
+    """Weighted sum of monoenergetic kernels, renormalised, as a KernelPoly
+    (conehead/conehead.py:700-704).  kernels: energy key -> object with
+    angles, radii, kernel_diff; energy_weights: settings['energy_weights']."""
+    first = kernels[next(iter(kernels))]
+    diff = np.zeros_like(first.kernel_diff)
+    for e, w in energy_weights.items():
+        diff += kernels[e].kernel_diff * w
+    diff = diff / diff.sum()
+    return KernelPoly(first.angles, first.radii, diff)
+
+
+class EdkSet:
+    """The twelve monoenergetic EDKnrc tables in one compact file
+    (conehead_amd/data/edk_6mv.npz): per energy the normalised 48 x 24
+    differential table of the listing, with the angle and radius grids.
+    .mono(key) gives a KernelMono-like object; .poly(weights) the
+    polyenergetic kernel of conehead/conehead.py:686-704."""
+
+    def __init__(self, energies, angles, radii, diffs):
+        self.energies = tuple(energies)
+        self.angles = np.asarray(angles, np.float32)
+        self.radii = np.asarray(radii, np.float32)
+        self.diffs = np.asarray(diffs, np.float32)
+
+    @classmethod
+    def from_listings(cls, root):
+        ks = [KernelMono(p) for p in listing_files(root, False).values()]
+        return cls(ENERGIES, ks[0].angles, ks[0].radii,
+                   np.stack([k.kernel_diff for k in ks]))
+
+    @classmethod
+    def load(cls, path=None):
+        import os
+        if path is None:
+            path = os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                                "data", "edk_6mv.npz")
+        z = np.load(path, allow_pickle=False)
+        return cls([str(e) for e in z["energies"]], z["angles"],
+                   z["radii"], z["diffs"])
+
+    def save(self, path):
+        np.savez_compressed(path, energies=np.array(self.energies),
+                            angles=self.angles, radii=self.radii,
+                            diffs=self.diffs)
+
+    def mono(self, key):
+        k = KernelMono.__new__(KernelMono)
+        k.energy = float(key)
+        k.angles = self.angles
+        k.radii = self.radii
+        k.kernel_diff = self.diffs[self.energies.index(key)]
+        k.kernel_cum = _cumulative_table(self.radii, k.kernel_diff)
+        return k
+
+    def poly(self, energy_weights, reference_1mev=True):
+        kernels = {}
+        for e in energy_weights:
+            src = "1.5" if (e == "1.0" and reference_1mev) else e
+            kernels[e] = self.mono(src)
+        return polyenergetic(kernels, energy_weights)

@@ -90,3 +90,31 @@ def test_scatter_rejects_bad_arguments():
         scatter(d, d, (2, 2, 2), (1, 1, 1), t, r, t, 100, 1)  # short rows
     with pytest.raises(_lib.CHError):
         scatter(d, d, (2, 2, 2), (0, 1, 1), t, r, t, 5996, 1)
+
+
+def test_edk_polyenergetic_cuda_dose(engine):
+    """The reference GPU pipeline's convolution (cuda_dose semantics,
+    conehead/conehead.py:385-488) with the polyenergetic EDKnrc kernel of
+    :686-704: the gather and the independent scatter agree."""
+    from conehead_amd.cones import default_thetas
+    from conehead_amd.kernel import EdkSet
+    poly = EdkSet.load().poly(W.SETTINGS["energy_weights"])
+    cs = ConeSet(default_thetas(4), poly.angles.astype(np.float64),
+                 poly.kernel_cum, semantics="cuda_dose")
+    assert cs.n_cones == 4 * 48
+    shape, vs = (20, 24, 18), (0.5, 0.5, 0.5)
+    rng = np.random.default_rng(8)
+    terma = np.zeros(shape, np.float32)
+    terma[6:14, 2:20, 5:13] = rng.random((8, 18, 8)).astype(np.float32)
+    plan = engine.plan(shape, vs, cs)
+    d_t = DeviceArray.from_host(terma)
+    g = DeviceArray(shape, zero=True)
+    E.convolve(plan, d_t, g)
+    sc = DeviceArray(shape, zero=True)
+    scatter(d_t, sc, shape, vs, DeviceArray.from_host(cs.dirs),
+            DeviceArray.from_host(cs.rows.astype(np.int32)),
+            DeviceArray.from_host(np.ascontiguousarray(cs.cum)),
+            cs.cum.shape[1], cs.n_cones, "cuda_dose")
+    gd, sd = g.download(), sc.download()
+    assert gd.max() > 0
+    assert np.abs(gd - sd).max() / gd.max() < 1e-5
