@@ -165,6 +165,9 @@ SIGNATURES = {
     "ch_convolve_f64": (ctypes.c_int32, [c_vp, c_vp, c_vp, c_vp, c_vp]),
     "ch_convolve_batch": (ctypes.c_int32, [ctypes.c_int32,
                                            ctypes.POINTER(ConvJob), c_vp]),
+    "ch_resample_linear": (ctypes.c_int32, [
+        c_vp, c_i32p, c_f64p, c_f64p, c_vp, c_i32p, c_f64p, c_f64p,
+        ctypes.c_int32, c_vp]),
     "ch_graph_begin": (ctypes.c_int32, [c_vp]),
     "ch_graph_end": (ctypes.c_int32, [c_vp, ctypes.POINTER(c_vp)]),
     "ch_graph_launch": (ctypes.c_int32, [c_vp, c_vp]),

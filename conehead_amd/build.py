@@ -23,9 +23,10 @@ COMMON = [
     "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument",
 ]
 # Files whose arithmetic must round every operation like the FP64 oracle.
-EXACT = {"dda.hip", "plan.hip", "terma.hip", "raster.hip", "scatter.hip"}
+EXACT = {"dda.hip", "plan.hip", "terma.hip", "raster.hip", "scatter.hip",
+         "resample.hip"}
 SOURCES = ["runtime.hip", "dda.hip", "plan.hip", "ccc.hip", "terma.hip",
-           "raster.hip", "scatter.hip"]
+           "raster.hip", "scatter.hip", "resample.hip"]
 
 
 def hipcc():

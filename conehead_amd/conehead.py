@@ -60,15 +60,33 @@ class Conehead:
         return self._engine
 
     def calculate(self, source, block, phantom, settings):
-        """TERMA and dose of one beam; returns the FP32 dose grid."""
+        """TERMA and dose of one beam; returns the FP32 dose grid.
+
+        The dose grid is the phantom's own (conehead/conehead.py:536) unless
+        settings['dose_grid'] = (size, origin, spacing) names another one;
+        the densities are then resampled on the GPU (ch_resample_linear)
+        with the reference's interpolation and layout."""
         eng = self.engine
-        self.dose_grid = DoseGrid(phantom.size, phantom.origin,
-                                  phantom.spacing)
-        shape = tuple(int(s) for s in phantom.size)
-        origin = np.asarray(phantom.origin, dtype=np.float64)
-        spacing = np.asarray(phantom.spacing, dtype=np.float64)
-        dens = dose_grid_densities(phantom,
-                                   settings.get("density_xy_swap", True))
+        swap = settings.get("density_xy_swap", True)
+        grid = settings.get("dose_grid")
+        s = Stream()
+        if grid is None:
+            self.dose_grid = DoseGrid(phantom.size, phantom.origin,
+                                      phantom.spacing)
+            dens = dose_grid_densities(phantom, swap)
+            d_rho = DeviceArray.from_host(dens, stream=s)
+        else:
+            from .density import resample
+            self.dose_grid = DoseGrid(*grid)
+            d_src = DeviceArray.from_host(phantom.densities, stream=s)
+            d_rho = resample(d_src, phantom.densities.shape, phantom.origin,
+                             phantom.spacing, self.dose_grid.size,
+                             self.dose_grid.origin, self.dose_grid.spacing,
+                             swap_xy=swap, stream=s)
+            dens = d_rho.download(stream=s)
+        shape = tuple(int(v) for v in self.dose_grid.size)
+        origin = np.asarray(self.dose_grid.origin, dtype=np.float64)
+        spacing = np.asarray(self.dose_grid.spacing, dtype=np.float64)
         if dens.shape != shape:
             raise ValueError("densities %s do not match size %s" %
                              (dens.shape, shape))
@@ -88,8 +106,6 @@ class Conehead:
         thetas = default_thetas(12) if thetas is None else thetas
         cones = ConeSet(thetas, phis, kernel, semantics=semantics)
 
-        s = Stream()
-        d_rho = DeviceArray.from_host(dens, stream=s)
         d_terma = DeviceArray(shape, np.float32)
         d_box = DeviceArray((6,), np.int32)
         d_raster = eng.aperture(block, s)

@@ -208,6 +208,24 @@ ch_status ch_convolve_scatter(const float* d_terma, float* d_dose,
                               int32_t row_len, int32_t semantics,
                               void* stream);
 
+/* --------------------------------------------------------- densities --- */
+
+/* Linear resampling of phantom densities onto a dose grid, as the
+ * reference's scipy RegularGridInterpolator (linear, fill 0 outside) on the
+ * axes linspace(origin, origin + n * spacing, n) of both grids
+ * (conehead/conehead.py:525-533, :561-568).  xy_swap != 0 lays the result
+ * out as np.meshgrid(indexing='xy') does: d_dst[i][j][k] is the value at
+ * (x_j, y_i, z_k), which needs dst_shape[0] == dst_shape[1] (the
+ * reference's own grids are then square in x and y); 0 gives the
+ * [x][y][z] layout. */
+ch_status ch_resample_linear(const float* d_src, const int32_t src_shape[3],
+                             const double src_origin[3],
+                             const double src_spacing[3], float* d_dst,
+                             const int32_t dst_shape[3],
+                             const double dst_origin[3],
+                             const double dst_spacing[3], int32_t xy_swap,
+                             void* stream);
+
 /* ------------------------------------------------------------ graphs --- */
 
 /* Capture of a pipeline of calls on `stream` into a HIP graph, replayed by
