@@ -22,6 +22,7 @@ CH_ERR_GEOMETRY = -5
 
 CH_SEM_CONVOLVE_C = 0
 CH_SEM_CUDA_DOSE = 1
+CH_SEM_RADIOLOGICAL = 2
 CH_MAX_ENERGIES = 32
 
 c_i32p = ctypes.POINTER(ctypes.c_int32)
@@ -165,6 +166,8 @@ SIGNATURES = {
     "ch_convolve_f64": (ctypes.c_int32, [c_vp, c_vp, c_vp, c_vp, c_vp]),
     "ch_convolve_batch": (ctypes.c_int32, [ctypes.c_int32,
                                            ctypes.POINTER(ConvJob), c_vp]),
+    "ch_convolve_rad": (ctypes.c_int32, [
+        c_vp, c_vp, c_vp, c_vp, c_vp, ctypes.c_int32, c_vp]),
     "ch_resample_linear": (ctypes.c_int32, [
         c_vp, c_i32p, c_f64p, c_f64p, c_vp, c_i32p, c_f64p, c_f64p,
         ctypes.c_int32, c_vp]),

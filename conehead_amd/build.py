@@ -24,9 +24,9 @@ COMMON = [
 ]
 # Files whose arithmetic must round every operation like the FP64 oracle.
 EXACT = {"dda.hip", "plan.hip", "terma.hip", "raster.hip", "scatter.hip",
-         "resample.hip"}
+         "resample.hip", "physics.hip"}
 SOURCES = ["runtime.hip", "dda.hip", "plan.hip", "ccc.hip", "terma.hip",
-           "raster.hip", "scatter.hip", "resample.hip"]
+           "raster.hip", "scatter.hip", "resample.hip", "physics.hip"]
 
 
 def hipcc():

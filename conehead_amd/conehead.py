@@ -11,9 +11,13 @@ Settings keys are the reference's (example.py:27-58).  Optional keys:
   thetas, phis, kernel -- the cone set handed to the convolution (default:
       12 azimuths and all 48 polar keys of PolyenergeticKernel, the cone set
       of the reference's convolve_c call, conehead/conehead.py:780-794);
-  semantics -- "convolve_c" (default) or "cuda_dose" (y-polar cones and the
+  semantics -- "convolve_c" (default), "cuda_dose" (y-polar cones and the
       index arithmetic of conehead/conehead.py:385-488; `kernel` is then a
-      [n_phi, >=5996] cumulative table and `phis` its angles);
+      [n_phi, >=5996] cumulative table and `phis` its angles) or "physics"
+      (the cuda_dose cones aligned with the beam and the kernel looked up at
+      the radiological distance through the densities; without `kernel` the
+      EDKnrc polyenergetic kernel of settings['energy_weights'] merged into
+      settings.get('n_phi', 12) polar groups);
   density_xy_swap -- True (default) reproduces the reference's density
       resampling, whose meshgrid(indexing='xy') evaluation at the grid nodes
       swaps the x and y axes (conehead/conehead.py:562-567); False uses the
@@ -22,9 +26,10 @@ Settings keys are the reference's (example.py:27-58).  Optional keys:
 import numpy as np
 
 from . import engine as _engine
-from .cones import ConeSet, default_thetas, kernel_phis
+from .cones import ConeSet, beam_frame, default_thetas, kernel_phis
 from .device import DeviceArray, Stream
 from .dosegrid import DoseGrid
+from .kernel import EdkSet, merge_polar
 from .kernel2 import PolyenergeticKernel
 
 
@@ -93,18 +98,27 @@ class Conehead:
 
         semantics = settings.get("semantics", "convolve_c")
         kernel = settings.get("kernel")
+        physics = semantics == "physics"
         if semantics == "convolve_c":
             kernel = kernel if kernel is not None else PolyenergeticKernel()
             phis = settings.get("phis")
             phis = kernel_phis(kernel) if phis is None else phis
+        elif physics and kernel is None:
+            # EDKnrc polyenergetic kernel of the beam's spectrum
+            # (conehead/conehead.py:686-704), merged into polar groups
+            poly = EdkSet.load().poly(settings["energy_weights"])
+            phis, kernel = merge_polar(poly.angles, poly.kernel_cum,
+                                       int(settings.get("n_phi", 12)))
         else:
             if kernel is None or settings.get("phis") is None:
-                raise ValueError("cuda_dose semantics needs settings['kernel']"
-                                 " (cumulative table) and settings['phis']")
+                raise ValueError("%s semantics needs settings['kernel'] "
+                                 "(cumulative table) and settings['phis']"
+                                 % semantics)
             phis = settings["phis"]
         thetas = settings.get("thetas")
         thetas = default_thetas(12) if thetas is None else thetas
-        cones = ConeSet(thetas, phis, kernel, semantics=semantics)
+        cones = ConeSet(thetas, phis, kernel, semantics=semantics,
+                        frame=beam_frame(source) if physics else None)
 
         d_terma = DeviceArray(shape, np.float32)
         d_box = DeviceArray((6,), np.int32)
@@ -114,7 +128,11 @@ class Conehead:
         _engine.terma([beam], shape, origin, spacing, stream=s)
         plan = eng.plan(shape, spacing, cones)
         d_dose = DeviceArray(shape, np.float32)
-        _engine.convolve(plan, d_terma, d_dose, d_box, stream=s)
+        if physics:
+            _engine.convolve_rad(plan, d_terma, d_rho, d_dose, d_box,
+                                 stream=s)
+        else:
+            _engine.convolve(plan, d_terma, d_dose, d_box, stream=s)
         dose = d_dose.download(stream=s)
         self.dose_grid_terma = d_terma.download(stream=s)
         self.dose_grid_densities = dens

@@ -60,18 +60,52 @@ def kernel_rows(kernel, phis):
     return np.ascontiguousarray(np.stack([r[:n] for r in rows]))
 
 
+def beam_frame(source):
+    """Rows v_x, v_y, v_z of a Source (conehead/source.py:77-111) as FP64:
+    v_y is the beam direction, v_x and v_z span the isocentre plane."""
+    return np.array([source.v_x, source.v_y, source.v_z], dtype=np.float64)
+
+
+def beam_aligned_direction(theta, phi, frame):
+    """Physics-mode cone direction: the cuda_dose direction taken in the
+    beam frame (polar axis = beam direction v_y) and expressed in grid
+    coordinates, d = a v_x + b v_y + c v_z.  With the gantry and collimator
+    at 0 the frame is the identity and d is the cuda_dose direction bit for
+    bit."""
+    a, b, c = cuda_dose_direction(theta, phi)
+    f = np.asarray(frame, dtype=np.float64)
+    return np.array([a * f[0, k] + b * f[1, k] + c * f[2, k]
+                     for k in range(3)], dtype=np.float64)
+
+
 class ConeSet:
     """n_theta x n_phi cones ordered theta-major as convolve_c loops them
-    (conehead/convolve_c.pyx:88-89); cone (i, j) uses kernel row j."""
+    (conehead/convolve_c.pyx:88-89); cone (i, j) uses kernel row j.
 
-    def __init__(self, thetas, phis, kernel, semantics="convolve_c"):
+    semantics: "convolve_c" (z-polar, discretised directions, kernel dict),
+    "cuda_dose" (y-polar, table rows) or "physics" (the cuda_dose cones in
+    the beam frame `frame`, see beam_frame, with the kernel looked up at
+    the radiological distance)."""
+
+    def __init__(self, thetas, phis, kernel, semantics="convolve_c",
+                 frame=None):
         self.thetas = np.asarray(thetas, dtype=np.float64).ravel()
         self.phis = np.asarray(phis, dtype=np.float64).ravel()
-        if semantics not in ("convolve_c", "cuda_dose"):
-            raise ValueError("semantics must be convolve_c or cuda_dose")
+        if semantics not in ("convolve_c", "cuda_dose", "physics"):
+            raise ValueError(
+                "semantics must be convolve_c, cuda_dose or physics")
+        if frame is not None and semantics != "physics":
+            raise ValueError("a beam frame applies to physics cones only")
         self.semantics = semantics
-        fn = convolve_c_direction if semantics == "convolve_c" else \
-            cuda_dose_direction
+        self.frame = np.eye(3) if frame is None else \
+            np.asarray(frame, dtype=np.float64).reshape(3, 3)
+        if semantics == "convolve_c":
+            fn = convolve_c_direction
+        elif semantics == "cuda_dose":
+            fn = cuda_dose_direction
+        else:
+            def fn(t, p):
+                return beam_aligned_direction(t, p, self.frame)
         nt, nph = len(self.thetas), len(self.phis)
         self.dirs = np.zeros((nt * nph, 3), np.float64)
         self.rows = np.zeros(nt * nph, np.int32)

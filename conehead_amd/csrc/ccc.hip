@@ -95,7 +95,6 @@ __device__ __forceinline__ void urange(int v, int lo, int hi, int step,
     b = f[U];
 }
 
-constexpr int kUnroll = 8;  // taps per step; tap lists carry kTapPad zeros
 
 // Sum of one chunk of kUnroll taps into a cone line's accumulator:
 //   ACC 0  Kahan-compensated per tap,
@@ -123,28 +122,6 @@ __device__ __forceinline__ void accumulate_chunk(float& acc, float& ac,
     }
 }
 
-// Tap range [a, b) of one axis, per lane (vector loads; the cone differs
-// between lanes).  Same clamping as urange.
-__device__ __forceinline__ void vrange(int v, int lo, int hi, int step,
-                                       const int32_t* __restrict__ f,
-                                       int f_len, int& a, int& b)
-{
-    int L, U;
-    if (step > 0) {
-        L = v - hi;
-        U = v - lo;
-    } else {
-        L = lo - v;
-        U = hi - v;
-    }
-    const int last = f_len - 1;
-    L = L < 0 ? 0 : (L > last ? last : L);
-    U = U + 1 < 0 ? 0 : (U + 1 > last ? last : U + 1);
-    a = f[L];
-    b = f[U];
-}
-
-constexpr int kBufferFlags = 0x00020000;  // raw 32-bit buffer resource
 
 // One wavefront owns a 1 x TY x TZ output tile (TY * TZ = 64 lanes, z
 // fastest).  The tile's union tap range of 64 cones at a time is computed
@@ -740,6 +717,11 @@ struct Kernels {
 template <typename OutT>
 static ch_status kernels_for(const ch_plan* p, Kernels<OutT>& k)
 {
+    if (p->semantics == CH_SEM_RADIOLOGICAL) {
+        set_error("a CH_SEM_RADIOLOGICAL plan is used through "
+                  "ch_convolve_rad");
+        return CH_ERR_ARG;
+    }
     const int tx = p->tile[0], ty = p->tile[1], tz = p->tile[2];
     if (tx == kSBX && ty == kSBY && tz == kSBZ) {
         k.single = staged_single<OutT>;
@@ -903,6 +885,26 @@ static ch_status pad_into(const float* T, float* P, const ch_plan* p,
                        dim3(256), 0, s, T, P, p->shape[0], p->shape[1],
                        p->shape[2]);
     CH_HIP(hipGetLastError());
+    return CH_OK;
+}
+
+ch_status pad_pair(const ch_plan* p, const float* a, const float* b,
+                   hipStream_t s, const float** pa, const float** pb)
+{
+    std::lock_guard<std::mutex> lock(g_ws_mu);
+    Workspace* w = nullptr;
+    const size_t n = padded_elems(p);
+    ch_status st = workspace(s, 2 * n, 0, &w);
+    if (st != CH_OK)
+        return st;
+    st = pad_into(a, w->pad, p, s);
+    if (st == CH_OK)
+        st = pad_into(b, w->pad + n, p, s);
+    if (st != CH_OK)
+        return st;
+    const size_t interior = (size_t)kYPad * (p->shape[2] + 2 * kZPad) + kZPad;
+    *pa = w->pad + interior;
+    *pb = w->pad + n + interior;
     return CH_OK;
 }
 
@@ -1090,6 +1092,9 @@ ch_status ch_convolve_groups(int32_t n_groups, const ch_conv_group* h_groups,
     for (int32_t i = 0; i < n_groups; ++i) {
         const ch_conv_group& g = h_groups[i];
         CH_REQUIRE(g.plan, "group %d: null plan", i);
+        CH_REQUIRE(g.plan->semantics != CH_SEM_RADIOLOGICAL,
+                   "group %d: a CH_SEM_RADIOLOGICAL plan is used through "
+                   "ch_convolve_rad", i);
         CH_REQUIRE(g.n_beams >= 1 && g.n_beams <= CH_MAX_GROUP,
                    "group %d: %d beams (1..%d)", i, g.n_beams, CH_MAX_GROUP);
         for (int b = 0; b < g.n_beams; ++b)

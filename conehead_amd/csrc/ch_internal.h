@@ -35,6 +35,7 @@ constexpr int32_t kMaxIndex = 5995;    // convolve_c.pyx:144
 constexpr int32_t kRowLenMin = kMaxIndex + 1;
 // Zero taps appended to every cone's tap list (>= the gather's unroll).
 constexpr int32_t kTapPad = 8;
+constexpr int kUnroll = 8;  // taps per gather step (<= kTapPad)
 
 // Per-cone metadata of a plan (device and host copies).
 //  taps [tap_begin, tap_begin + n_taps): source byte offset and increment.
@@ -143,6 +144,11 @@ void* stream_scratch_peek(hipStream_t s, int slot);
 // Free a stream's scratch and gather workspace (before it is destroyed).
 void release_stream_scratch(hipStream_t s);
 void release_gather_workspace(hipStream_t s);
+// Pads two grids of plan p's shape into the stream's gather workspace
+// (the layout the gathers read) and returns the interior voxel (0, 0, 0)
+// of each copy.
+ch_status pad_pair(const ch_plan* p, const float* a, const float* b,
+                   hipStream_t s, const float** pa, const float** pb);
 
 // Support box of T != 0 from per-block partial boxes: kernels write
 // kBoxStride ints per block (min x, y, z, max x, y, z) to `partials`, then
@@ -182,6 +188,32 @@ __device__ __forceinline__ void block_box_partial(int32_t v[6],
     }
 }
 
+// Tap range [a, b) of one axis for coordinate v and box [lo, hi]: the taps
+// whose source coordinate v - o lies in the box, from the axis' first-step
+// table f (per lane: the cone may differ between lanes).  Out-of-range
+// arguments clamp to the ends of the table, which yields empty ranges and
+// keeps a and b monotone in v.
+__device__ __forceinline__ void vrange(int v, int lo, int hi, int step,
+                                       const int32_t* __restrict__ f,
+                                       int f_len, int& a, int& b)
+{
+    int L, U;
+    if (step > 0) {
+        L = v - hi;
+        U = v - lo;
+    } else {
+        L = lo - v;
+        U = hi - v;
+    }
+    const int last = f_len - 1;
+    L = L < 0 ? 0 : (L > last ? last : L);
+    U = U + 1 < 0 ? 0 : (U + 1 > last ? last : U + 1);
+    a = f[L];
+    b = f[U];
+}
+
+constexpr int kBufferFlags = 0x00020000;  // raw 32-bit buffer resource
+
 }  // namespace ch
 
 struct ch_plan {
@@ -203,4 +235,11 @@ struct ch_plan {
     int32_t* d_ftab;
     ch::ConeMeta* d_meta;
     std::vector<ch::ConeMeta> h_meta;
+    // CH_SEM_RADIOLOGICAL plans only: the kernel rows (n_rows x row_len
+    // FP64), the cones ordered by row and each row's range in that order.
+    double* d_cum;
+    int32_t* d_perm;
+    int32_t* d_row_begin;
+    int32_t n_rows;
+    int32_t row_len;
 };

@@ -19,7 +19,10 @@ namespace ch {
 
 // One lane per cone: walk the DDA from the origin of an unbounded grid while
 // every offset component could still land inside the grid, stop at the first
-// skipped kernel index (t >= 60.46 cm), record taps and first-step tables.
+// skipped kernel index (t >= 60.46 cm, convolve_c) or after the first
+// clamped one (cuda_dose), record taps and first-step tables.  Radiological
+// plans keep the whole path and store each voxel's path length instead of
+// an increment.
 __global__ __launch_bounds__(64) void build_taps_kernel(
     int32_t n_cones, const double* __restrict__ dirs,
     const int32_t* __restrict__ rows, const double* __restrict__ cum,
@@ -45,23 +48,32 @@ __global__ __launch_bounds__(64) void build_taps_kernel(
     ftab[fb1] = 0;
     ftab[fb2] = 0;
     int32_t n = 0, prev = -1;
+    double t_prev = 0.0;
     // strides of the padded TERMA copy the gather reads
     const int32_t sy = Z + 2 * kZPad, sx = (Y + 2 * kYPad) * sy;
     while (n0 < X && n1 < Y && n2 < Z && n < tap_stride - kTapPad) {
         const int a = d.axis();
         double t = a == 0 ? d.t0 : (a == 1 ? d.t1 : d.t2);
         bool clamped = false;
-        const int32_t idx = kernel_index(t, semantics, &clamped);
-        if (idx < 0)
-            break;
-        const double k = prev < 0 ? row[idx] : row[idx] - row[prev];
+        double k;
+        if (semantics == CH_SEM_RADIOLOGICAL) {
+            // the path length inside the m-th voxel; the increments depend
+            // on the densities and are formed by ch_convolve_rad
+            k = t - t_prev;
+            t_prev = t;
+        } else {
+            const int32_t idx = kernel_index(t, semantics, &clamped);
+            if (idx < 0)
+                break;
+            k = prev < 0 ? row[idx] : row[idx] - row[prev];
+            prev = idx;
+        }
         lin[tb + n] = 4 * (c0 * sx + c1 * sy + c2);  // byte offset
         k64[tb + n] = k;
         k32[tb + n] = (float)k;
         off[3 * (tb + n) + 0] = c0;
         off[3 * (tb + n) + 1] = c1;
         off[3 * (tb + n) + 2] = c2;
-        prev = idx;
         ++n;
         if (clamped)
             break;  // later taps would add cum[5995] - cum[5995] = 0
@@ -112,6 +124,9 @@ static void plan_free(ch_plan* p)
     hipFree(p->d_lin2);
     hipFree(p->d_ftab);
     hipFree(p->d_meta);
+    hipFree(p->d_cum);
+    hipFree(p->d_perm);
+    hipFree(p->d_row_begin);
     delete p;
 }
 
@@ -191,7 +206,8 @@ ch_status ch_plan_create(const int32_t h_shape[3], const double h_vsize[3],
                "ch_plan_create: kernel rows have %d values, need >= %d",
                row_len, kRowLenMin);
     CH_REQUIRE(semantics == CH_SEM_CONVOLVE_C ||
-                   semantics == CH_SEM_CUDA_DOSE,
+                   semantics == CH_SEM_CUDA_DOSE ||
+                   semantics == CH_SEM_RADIOLOGICAL,
                "ch_plan_create: unknown semantics %d", semantics);
     for (int a = 0; a < 3; ++a)
         CH_REQUIRE(h_vsize[a] > 0, "ch_plan_create: voxel size <= 0");
@@ -221,6 +237,11 @@ ch_status ch_plan_create(const int32_t h_shape[3], const double h_vsize[3],
     p->d_lin2 = nullptr;
     p->d_ftab = nullptr;
     p->d_meta = nullptr;
+    p->d_cum = nullptr;
+    p->d_perm = nullptr;
+    p->d_row_begin = nullptr;
+    p->n_rows = n_rows;
+    p->row_len = row_len;
 
     const size_t nt = (size_t)n_cones * p->tap_stride;
     const size_t nf = (size_t)n_cones * p->f_stride;
@@ -275,6 +296,29 @@ ch_status ch_plan_create(const int32_t h_shape[3], const double h_vsize[3],
         CH_TRY(hipMemcpy(p->d_meta, p->h_meta.data(),
                          (size_t)n_cones * sizeof(ConeMeta),
                          hipMemcpyHostToDevice));
+    }
+    if (semantics == CH_SEM_RADIOLOGICAL) {
+        // cones ordered by kernel row, so a workgroup stages each row once
+        std::vector<int32_t> begin(n_rows + 1, 0), perm(n_cones);
+        for (int32_t c = 0; c < n_cones; ++c)
+            ++begin[h_cone_row[c] + 1];
+        for (int32_t r = 0; r < n_rows; ++r)
+            begin[r + 1] += begin[r];
+        std::vector<int32_t> fill(begin.begin(), begin.end() - 1);
+        for (int32_t c = 0; c < n_cones; ++c)
+            perm[fill[h_cone_row[c]]++] = c;
+        CH_TRY(hipMalloc((void**)&p->d_perm, n_cones * sizeof(int32_t)));
+        CH_TRY(hipMalloc((void**)&p->d_row_begin,
+                         (n_rows + 1) * sizeof(int32_t)));
+        CH_TRY(hipMemcpy(p->d_perm, perm.data(), n_cones * sizeof(int32_t),
+                         hipMemcpyHostToDevice));
+        CH_TRY(hipMemcpy(p->d_row_begin, begin.data(),
+                         (n_rows + 1) * sizeof(int32_t),
+                         hipMemcpyHostToDevice));
+        if (e == hipSuccess) {
+            p->d_cum = d_cum;  // kept: the increments are formed per launch
+            d_cum = nullptr;
+        }
     }
 #undef CH_TRY
     hipFree(d_dirs);

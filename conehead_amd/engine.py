@@ -20,7 +20,8 @@ SETTINGS_KEYS = ("sPri", "sAnn", "zAnn", "rInner", "rOuter", "zExp", "sExp",
                  "kExp", "softRatio", "softLimit", "hornRatio")
 
 SEMANTICS = {"convolve_c": _lib.CH_SEM_CONVOLVE_C,
-             "cuda_dose": _lib.CH_SEM_CUDA_DOSE}
+             "cuda_dose": _lib.CH_SEM_CUDA_DOSE,
+             "physics": _lib.CH_SEM_RADIOLOGICAL}
 
 
 class Plan:
@@ -108,6 +109,17 @@ def convolve(plan, d_terma, d_dose, d_box=None, accumulate=False,
     _lib.check(_lib.lib().ch_convolve(plan.handle, d_terma.ptr, d_dose.ptr,
                                       d_box.ptr if d_box is not None else None,
                                       1 if accumulate else 0, _h(stream)))
+
+
+def convolve_rad(plan, d_terma, d_density, d_dose, d_box=None,
+                 accumulate=False, stream=None):
+    """Physics mode: dose (+)= convolution of terma with the kernel looked
+    up at the radiological distance through d_density (FP32, same grid).
+    The plan must come from a "physics" cone set."""
+    _lib.check(_lib.lib().ch_convolve_rad(
+        plan.handle, d_terma.ptr, d_density.ptr, d_dose.ptr,
+        d_box.ptr if d_box is not None else None, 1 if accumulate else 0,
+        _h(stream)))
 
 
 def convolve_f64(plan, d_terma, d_dose64, d_box=None, stream=None):

@@ -139,3 +139,27 @@ class EdkSet:
             src = "1.5" if (e == "1.0" and reference_1mev) else e
             kernels[e] = self.mono(src)
         return polyenergetic(kernels, energy_weights)
+
+
+def merge_polar(angles, kernel_cum, n_phi):
+    """Merges the EDKnrc polar bins (angles = upper bin edges, ascending,
+    equal width) into n_phi equal groups for a coarser cone set: a group's
+    table is the sum of its members' cumulative tables (so it keeps their
+    energy) and its angle is the centre of its angular range.  Returns
+    (angles[n_phi], cum[n_phi, row_len])."""
+    angles = np.asarray(angles, dtype=np.float64)
+    cum = np.asarray(kernel_cum, dtype=np.float64)
+    n = len(angles)
+    if n_phi < 1 or n % n_phi:
+        raise ValueError("%d bins do not split into %d groups" % (n, n_phi))
+    w = n // n_phi
+    width = angles[1] - angles[0] if n > 1 else angles[0]
+    out_a = np.empty(n_phi)
+    out_c = np.empty((n_phi, cum.shape[1]))
+    for g in range(n_phi):
+        lo = angles[g * w] - width
+        hi = angles[(g + 1) * w - 1]
+        out_a[g] = 0.5 * (lo + hi)
+        out_c[g] = cum[g * w:(g + 1) * w].sum(axis=0)
+    return out_a, np.ascontiguousarray(out_c)
+

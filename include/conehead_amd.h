@@ -96,6 +96,10 @@ typedef struct ch_plan ch_plan;
                                (conehead/convolve_c.pyx:138-145)       */
 #define CH_SEM_CUDA_DOSE 1  /* |floor(100 t - 5)|, clamp at 5995
                                (conehead/conehead.py:453-463)          */
+#define CH_SEM_RADIOLOGICAL 2 /* physics mode: the cuda_dose index taken
+                               at the radiological distance R instead of
+                               t; plans of this kind are used through
+                               ch_convolve_rad only                    */
 
 /* Builds the per-cone tap tables on the device: for every cone, the FP64
  * DDA from the origin (offsets o_m, kernel increments
@@ -207,6 +211,23 @@ ch_status ch_convolve_scatter(const float* d_terma, float* d_dose,
                               const int32_t* d_rows, const double* d_cum,
                               int32_t row_len, int32_t semantics,
                               void* stream);
+
+/* Physics mode (no counterpart in the reference, which lists kernel
+ * tilting as a milestone, README.md:22): the cone set is aligned with the
+ * beam by the caller (the directions given to ch_plan_create are rotated
+ * into the beam frame) and each increment is looked up at the
+ * radiological distance from the receiving voxel v:
+ *   R_m = sum_{j<=m} rho(v - o_j) (t_j - t_{j-1}),  t_{-1} = 0,
+ *   dose[v] (+)= sum_c sum_m T(v - o_m) (cum[i(R_m)] - cum[i(R_{m-1})]),
+ * i(R) = min(|floor(100 R - 5)|, 5995) as cuda_dose indexes t
+ * (conehead/conehead.py:453-463).  R and the increments are FP64; the sum
+ * is FP32 with compensation.  `plan` must be built with
+ * CH_SEM_RADIOLOGICAL; d_density is the FP32 density grid of the same
+ * shape; d_box (nullable) is the TERMA support box. */
+ch_status ch_convolve_rad(const ch_plan* plan, const float* d_terma,
+                          const float* d_density, float* d_dose,
+                          const int32_t* d_box, int32_t accumulate,
+                          void* stream);
 
 /* --------------------------------------------------------- densities --- */
 

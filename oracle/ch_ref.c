@@ -521,3 +521,151 @@ int32_t ch_ref_terma(const ch_ref_beam* b, const int32_t shape[3],
     }
     return err ? -1 : 0;
 }
+
+/* ------------------------------------------------------- physics mode */
+
+/* Kernel index of cuda_dose (conehead/conehead.py:453-463): the radius r
+ * looks up |floor(100 r - 5)|, clamped at 5995 (no truncation). */
+int32_t ch_ref_kernel_index_cuda(double r)
+{
+    double f = floor(r * 100.0 - 5.0);
+    if (!(f < 2147483647.0 && f > -2147483648.0))
+        return CH_REF_MAX_INDEX;
+    int32_t idx = abs((int32_t)f);
+    return idx > CH_REF_MAX_INDEX ? CH_REF_MAX_INDEX : idx;
+}
+
+/* The DDA path of one cone from the origin voxel of an unbounded grid, kept
+ * while every offset component can still land inside `shape`: offsets and
+ * the t-values at which each voxel is left (same arithmetic as
+ * ch_ref_dda_3d).  Returns the number of voxels, or -1 on overflow. */
+int32_t ch_ref_cone_path(const double dir[3], const int32_t shape[3],
+                         const double dims[3], int32_t* off_out,
+                         double* t_out, int32_t cap)
+{
+    int32_t step[3], cur[3] = {0, 0, 0};
+    double d[3], t[3], dt[3];
+    for (int a = 0; a < 3; ++a) {
+        step[a] = dir[a] < 0 ? -1 : 1;
+        d[a] = dir[a] < 0 ? -1 * dir[a] : dir[a];
+        if (d[a] == 0.0) {
+            t[a] = CH_REF_BIG;
+            dt[a] = CH_REF_BIG;
+        } else {
+            t[a] = (dims[a] / 2) / d[a];
+            dt[a] = dims[a] / d[a];
+        }
+    }
+    int32_t n = 0;
+    while (abs(cur[0]) < shape[0] && abs(cur[1]) < shape[1] &&
+           abs(cur[2]) < shape[2]) {
+        int a;
+        if (t[0] < t[1])
+            a = (t[0] < t[2]) ? 0 : 2;
+        else
+            a = (t[1] < t[2]) ? 1 : 2;
+        if (n >= cap)
+            return -1;
+        off_out[3 * n + 0] = cur[0];
+        off_out[3 * n + 1] = cur[1];
+        off_out[3 * n + 2] = cur[2];
+        t_out[n] = t[a];
+        ++n;
+        t[a] += dt[a];
+        cur[a] = cur[a] + step[a];
+    }
+    return n;
+}
+
+static void rad_range(const double* terma, const float* density, double* dose,
+                      const int32_t shape[3], int32_t n_cones,
+                      const int32_t* path_n, const int32_t* path_off,
+                      const double* path_t, int32_t cap,
+                      const int32_t* cone_row, int32_t row_len,
+                      const double* cum, int32_t radiological, int32_t x0,
+                      int32_t x1, int64_t* updates)
+{
+    const int64_t sy = shape[2], sx = (int64_t)shape[1] * shape[2];
+    for (int32_t x = x0; x < x1; ++x)
+        for (int32_t y = 0; y < shape[1]; ++y)
+            for (int32_t z = 0; z < shape[2]; ++z) {
+                double acc = 0.0;
+                for (int32_t c = 0; c < n_cones; ++c) {
+                    const int32_t* off = path_off + (int64_t)3 * cap * c;
+                    const double* tv = path_t + (int64_t)cap * c;
+                    const double* row = cum + (int64_t)cone_row[c] * row_len;
+                    double r = 0.0, t_prev = 0.0, c_prev = 0.0;
+                    for (int32_t m = 0; m < path_n[c]; ++m) {
+                        /* the voxel m steps back from v along the cone */
+                        const int32_t px = x - off[3 * m],
+                                      py = y - off[3 * m + 1],
+                                      pz = z - off[3 * m + 2];
+                        if (px < 0 || px >= shape[0] || py < 0 ||
+                            py >= shape[1] || pz < 0 || pz >= shape[2])
+                            break;  /* the path only leaves the grid */
+                        const int64_t p = px * sx + py * sy + pz;
+                        if (radiological) {
+                            const double seg = tv[m] - t_prev;
+                            r = r + (double)density[p] * seg;
+                            t_prev = tv[m];
+                        } else {
+                            r = tv[m];
+                        }
+                        const double cv = row[ch_ref_kernel_index_cuda(r)];
+                        const double k = cv - c_prev;
+                        c_prev = cv;
+                        const double T = terma[p];
+                        if (T != 0.0) {
+                            acc += T * k;
+                            ++*updates;
+                        }
+                    }
+                }
+                dose[x * sx + y * sy + z] += acc;
+            }
+}
+
+int64_t ch_ref_convolve_rad(const double* terma, const float* density,
+                            double* dose, const int32_t shape[3],
+                            const double dims[3], int32_t n_cones,
+                            const double* dirs, const int32_t* cone_row,
+                            int32_t row_len, const double* cum,
+                            int32_t radiological, int32_t n_threads)
+{
+    if (row_len < CH_REF_MAX_INDEX + 1 || shape[0] <= 0 || shape[1] <= 0 ||
+        shape[2] <= 0 || n_cones < 0 || (radiological && !density))
+        return -1;
+    const int32_t cap = shape[0] + shape[1] + shape[2] + 1;
+    int32_t* path_n = (int32_t*)malloc(sizeof(int32_t) * (n_cones + 1));
+    int32_t* path_off =
+        (int32_t*)malloc(sizeof(int32_t) * 3 * (size_t)cap * (n_cones + 1));
+    double* path_t =
+        (double*)malloc(sizeof(double) * (size_t)cap * (n_cones + 1));
+    for (int32_t c = 0; c < n_cones; ++c) {
+        path_n[c] = ch_ref_cone_path(dirs + 3 * c, shape, dims,
+                                     path_off + (int64_t)3 * cap * c,
+                                     path_t + (int64_t)cap * c, cap);
+        if (path_n[c] < 0) {
+            free(path_n);
+            free(path_off);
+            free(path_t);
+            return -2;
+        }
+    }
+    int64_t total = 0;
+    if (n_threads < 1)
+        n_threads = 1;
+#pragma omp parallel for num_threads(n_threads) schedule(dynamic) \
+    reduction(+ : total)
+    for (int32_t x = 0; x < shape[0]; ++x) {
+        int64_t u = 0;
+        rad_range(terma, density, dose, shape, n_cones, path_n, path_off,
+                  path_t, cap, cone_row, row_len, cum, radiological, x, x + 1,
+                  &u);
+        total += u;
+    }
+    free(path_n);
+    free(path_off);
+    free(path_t);
+    return total;
+}

@@ -74,6 +74,30 @@ int32_t ch_ref_cone_taps(const double dir[3], const int32_t shape[3],
                          const double* cum_row, int32_t* off_out,
                          double* k_out, int32_t cap);
 
+/* Physics mode: cones aligned with the beam (the caller rotates the
+ * directions) and the kernel looked up at the radiological distance.
+ * The reference lists both as milestones ("Kernel tilting", README.md:22)
+ * and has no implementation; this is the build's own definition, stated in
+ * gather form for output voxel v and cone c with path offsets o_m and exit
+ * t-values t_m (ch_ref_cone_path):
+ *   R_m = sum_{j<=m} rho(v - o_j) (t_j - t_{j-1}),  t_{-1} = 0,
+ *   dose[v] += sum_c sum_m T(v - o_m) (cum[i(R_m)] - cum[i(R_{m-1})]),
+ * with i() the cuda_dose index (ch_ref_kernel_index_cuda) and the walk ending
+ * where v - o_m leaves the grid.  radiological = 0 uses R_m = t_m, which is
+ * the cuda_dose convolution (conehead/conehead.py:385-488) in gather form.
+ * density is float32 (the engine's densities); returns the number of
+ * T != 0 updates, or a negative code. */
+int32_t ch_ref_kernel_index_cuda(double r);
+int32_t ch_ref_cone_path(const double dir[3], const int32_t shape[3],
+                         const double dims[3], int32_t* off_out,
+                         double* t_out, int32_t cap);
+int64_t ch_ref_convolve_rad(const double* terma, const float* density,
+                            double* dose, const int32_t shape[3],
+                            const double dims[3], int32_t n_cones,
+                            const double* dirs, const int32_t* cone_row,
+                            int32_t row_len, const double* cum,
+                            int32_t radiological, int32_t n_threads);
+
 /* Beam description for the TERMA chain. */
 typedef struct ch_ref_beam {
     double source_position[3];

@@ -86,6 +86,16 @@ def lib():
         L.ch_ref_cone_taps.restype = ctypes.c_int32
         L.ch_ref_cone_taps.argtypes = [_f64p, _i32p, _f64p, ctypes.c_int32,
                                        _f64p, _i32p, _f64p, ctypes.c_int32]
+        L.ch_ref_kernel_index_cuda.restype = ctypes.c_int32
+        L.ch_ref_kernel_index_cuda.argtypes = [ctypes.c_double]
+        L.ch_ref_cone_path.restype = ctypes.c_int32
+        L.ch_ref_cone_path.argtypes = [_f64p, _i32p, _f64p, _i32p, _f64p,
+                                       ctypes.c_int32]
+        L.ch_ref_convolve_rad.restype = ctypes.c_int64
+        L.ch_ref_convolve_rad.argtypes = [_f64p, _f32p, _f64p, _i32p, _f64p,
+                                          ctypes.c_int32, _f64p, _i32p,
+                                          ctypes.c_int32, _f64p,
+                                          ctypes.c_int32, ctypes.c_int32]
         L.ch_ref_terma.restype = ctypes.c_int32
         L.ch_ref_terma.argtypes = [ctypes.POINTER(RefBeam), _i32p, _f64p,
                                    _f64p, _f32p, _f32p, _f64p, _f64p, _f64p,
@@ -220,6 +230,56 @@ def cone_taps(direction, shape, dims, cum_row):
     if n < 0:
         raise RuntimeError("oracle tap overflow")
     return off[:n].copy(), k[:n].copy()
+
+
+def kernel_index_cuda(r):
+    """cuda_dose kernel index |floor(100 r - 5)| clamped at 5995."""
+    return int(lib().ch_ref_kernel_index_cuda(float(r)))
+
+
+def cone_path(direction, shape, dims):
+    """DDA path of one cone from the origin of an unbounded grid while it
+    can land inside `shape`: offsets int32[M,3] and exit t-values
+    float64[M]."""
+    d = _c64(direction, (3,))
+    shp = _c32i(shape)
+    cap = int(shp.sum()) + 2
+    off = np.zeros((cap, 3), np.int32)
+    t = np.zeros(cap, np.float64)
+    n = lib().ch_ref_cone_path(_p(d, _f64p), _p(shp, _i32p),
+                               _p(_c64(dims, (3,)), _f64p), _p(off, _i32p),
+                               _p(t, _f64p), cap)
+    if n < 0:
+        raise RuntimeError("oracle path overflow")
+    return off[:n].copy(), t[:n].copy()
+
+
+def convolve_rad(terma, density, dose, dims, dirs, cone_row, cum,
+                 radiological=True, n_threads=1):
+    """Physics-mode gather (radiological kernel distance, cuda_dose index);
+    radiological=False gives the cuda_dose convolution in gather form.
+    `dose` (float64, C-contiguous) is accumulated in place; `density` is
+    float32 on the same grid."""
+    terma = _c64(terma)
+    assert dose.dtype == np.float64 and dose.flags.c_contiguous
+    assert dose.shape == terma.shape
+    shp = _c32i(terma.shape)
+    dens = np.ascontiguousarray(
+        np.zeros(terma.shape, np.float32) if density is None else density,
+        dtype=np.float32)
+    assert dens.shape == terma.shape
+    dirs = _c64(dirs).reshape(-1, 3)
+    rows = _c32i(cone_row)
+    cum = _c64(cum)
+    r = lib().ch_ref_convolve_rad(_p(terma, _f64p), _p(dens, _f32p),
+                                  _p(dose, _f64p), _p(shp, _i32p),
+                                  _p(_c64(dims, (3,)), _f64p), len(dirs),
+                                  _p(dirs, _f64p), _p(rows, _i32p),
+                                  cum.shape[1], _p(cum, _f64p),
+                                  int(bool(radiological)), int(n_threads))
+    if r < 0:
+        raise RuntimeError("oracle physics convolution failed (%d)" % r)
+    return int(r)
 
 
 def make_beam(source_position, v_y, transform, sad, settings, energy,

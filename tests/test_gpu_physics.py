@@ -1,0 +1,148 @@
+"""Physics mode on the GPU (ch_convolve_rad): beam-aligned cones and the
+kernel looked up at the radiological distance, against the oracle's FP64
+restatement (ch_ref_convolve_rad).
+
+Tolerance: max |D_gpu - D_ref| / max(D_ref) <= 1e-5.  R, the kernel index
+and the increments round like the oracle's (FP64, no contraction); the
+remaining difference is the FP32 compensated sum (measured ~1e-7)."""
+import numpy as np
+import pytest
+
+from conehead_amd import engine as E
+from conehead_amd.cones import ConeSet, beam_frame, default_thetas
+from conehead_amd.device import DeviceArray
+from conehead_amd.kernel import EdkSet, merge_polar
+from conehead_amd.source import Source
+from conehead_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def rel_err(d, ref):
+    return float(np.abs(d - ref).max() / np.abs(ref).max())
+
+
+@pytest.fixture(scope="module")
+def edk():
+    poly = EdkSet.load().poly(W.SETTINGS["energy_weights"])
+    return merge_polar(poly.angles, poly.kernel_cum, 6)
+
+
+def physics_cones(edk, gantry, n_theta=6):
+    src = Source("varian_clinac_6MV")
+    src.gantry = gantry
+    phis, cum = edk
+    return ConeSet(default_thetas(n_theta), phis, cum, semantics="physics",
+                   frame=beam_frame(src))
+
+
+def heterogeneous(shape, seed):
+    rng = np.random.default_rng(seed)
+    rho = rng.uniform(0.2, 1.9, shape).astype(np.float32)
+    T = np.zeros(shape, np.float32)
+    sl = tuple(slice(s // 4, 3 * s // 4) for s in shape)
+    T[sl] = rng.random(T[sl].shape).astype(np.float32)
+    return T, rho
+
+
+def gpu_rad(engine, shape, vs, cs, T, rho, box=True, accumulate=False,
+            init=None):
+    plan = engine.plan(shape, vs, cs)
+    d_t = DeviceArray.from_host(T)
+    d_r = DeviceArray.from_host(rho)
+    d_d = DeviceArray.from_host(init) if init is not None else \
+        DeviceArray(shape, zero=True)
+    d_box = None
+    if box:
+        d_box = DeviceArray((6,), np.int32)
+        E.support_box(d_t, shape, d_box)
+    E.convolve_rad(plan, d_t, d_r, d_d, d_box, accumulate=accumulate)
+    return d_d.download()
+
+
+@pytest.mark.parametrize("gantry,shape", [(0.0, (20, 24, 18)),
+                                          (37.0, (21, 19, 33)),
+                                          (250.0, (13, 40, 17))])
+def test_matches_oracle(oracle, engine, edk, gantry, shape):
+    vs = (0.5, 0.4, 0.6)
+    cs = physics_cones(edk, gantry)
+    T, rho = heterogeneous(shape, int(gantry) + 1)
+    ref = np.zeros(shape)
+    oracle.convolve_rad(T, rho, ref, vs, cs.dirs, cs.rows, cs.cum,
+                        radiological=True, n_threads=8)
+    assert ref.max() > 0
+    for box in (True, False):
+        d = gpu_rad(engine, shape, vs, cs, T, rho, box=box)
+        assert rel_err(d, ref) < TOL, (box, rel_err(d, ref))
+
+
+def test_density_one_is_the_cuda_dose_gather(engine, edk):
+    """rho = 1 and the gantry at 0: the same dose as the geometric cuda_dose
+    gather (same cones, same increments up to the R = t rounding)."""
+    shape, vs = (18, 22, 20), (0.5, 0.5, 0.5)
+    phis, cum = edk
+    cs = physics_cones(edk, 0.0)
+    geo = ConeSet(default_thetas(6), phis, cum, semantics="cuda_dose")
+    assert np.array_equal(cs.dirs, geo.dirs)
+    T, _ = heterogeneous(shape, 4)
+    rho = np.ones(shape, np.float32)
+    d = gpu_rad(engine, shape, vs, cs, T, rho)
+    plan = engine.plan(shape, vs, geo)
+    d_t = DeviceArray.from_host(T)
+    d_g = DeviceArray(shape, zero=True)
+    E.convolve(plan, d_t, d_g)
+    assert rel_err(d, d_g.download()) < TOL
+
+
+def test_accumulate_and_empty(engine, edk):
+    shape, vs = (9, 10, 11), (0.5, 0.5, 0.5)
+    cs = physics_cones(edk, 90.0)
+    T, rho = heterogeneous(shape, 6)
+    base = gpu_rad(engine, shape, vs, cs, T, rho)
+    init = np.full(shape, 0.25, np.float32)
+    acc = gpu_rad(engine, shape, vs, cs, T, rho, accumulate=True, init=init)
+    np.testing.assert_allclose(acc, base + 0.25, rtol=1e-6, atol=1e-6)
+    zero = gpu_rad(engine, shape, vs, cs, np.zeros(shape, np.float32), rho)
+    assert np.all(zero == 0.0)
+
+
+def test_geometric_plans_are_refused(engine, edk):
+    from conehead_amd import _lib
+    shape, vs = (8, 8, 8), (0.5, 0.5, 0.5)
+    cs = physics_cones(edk, 0.0)
+    plan = engine.plan(shape, vs, cs)
+    d = DeviceArray(shape, zero=True)
+    with pytest.raises(_lib.CHError):
+        E.convolve(plan, d, d)
+    phis, cum = edk
+    geo = engine.plan(shape, vs, ConeSet([0.0], phis, cum,
+                                         semantics="cuda_dose"))
+    with pytest.raises(_lib.CHError):
+        E.convolve_rad(geo, d, d, d)
+
+
+def test_gantry_90_is_gantry_0_rotated(engine):
+    """A cubic water phantom: turning the gantry by 90 degrees turns the
+    TERMA and the cone set with it, so the physics dose turns too (the 90
+    degree rotation maps the grid onto itself)."""
+    from conehead_amd.block import Block
+    from conehead_amd.conehead import Conehead
+    from conehead_amd.phantom import Phantom
+    n, s = 24, 0.5
+    origin = np.array([-(n - 1) / 2 * s] * 3)
+    phantom = Phantom(np.ones((n, n, n), np.float32), origin, [s] * 3)
+    blk = Block()
+    blk.set_square(6)
+    settings = dict(W.SETTINGS, semantics="physics", n_phi=6,
+                    thetas=default_thetas(8))
+    doses = {}
+    for g in (0.0, 90.0):
+        src = Source("varian_clinac_6MV")
+        src.gantry = g
+        doses[g] = Conehead(engine).calculate(src, blk, phantom, settings)
+    # gantry 0: beam along +y; gantry 90: beam along -x.  The grid index
+    # (i, j) at gantry 90 sees what (n-1-j, i) sees at gantry 0.
+    rot = np.transpose(doses[0.0], (1, 0, 2))[::-1, :, :]
+    assert rel_err(doses[90.0], rot) < 1e-3, rel_err(doses[90.0], rot)

@@ -178,3 +178,32 @@ def test_cone_set_order_and_missing_key():
                                   convolve_c_direction(120.0, 67.5))
     with pytest.raises(KeyError):
         ConeSet([0.0], [12.0], k)
+
+
+def test_beam_aligned_cones():
+    """Physics cones: the cuda_dose directions in the beam frame.  Gantry 0
+    is the identity; at any gantry a cone's angle to the beam direction v_y
+    is its polar angle phi."""
+    from conehead_amd.cones import (ConeSet, beam_frame,
+                                    beam_aligned_direction,
+                                    cuda_dose_direction)
+    from conehead_amd.source import Source
+    src = Source("varian_clinac_6MV")
+    for th in (0.0, 30.0, 250.0):
+        for ph in (3.75, 90.0, 172.5):
+            assert np.array_equal(
+                beam_aligned_direction(th, ph, beam_frame(src)),
+                cuda_dose_direction(th, ph))
+    for g in (37.0, 90.0, 181.0):
+        src.gantry = g
+        f = beam_frame(src)
+        vy = f[1] / np.linalg.norm(f[1])
+        cs = ConeSet([0.0, 120.0], [7.5, 67.5], np.zeros((2, 5996)),
+                     semantics="physics", frame=f)
+        for c, ph in enumerate([7.5, 67.5] * 2):
+            d = cs.dirs[c]
+            ang = np.degrees(np.arccos(np.dot(d, vy) / np.linalg.norm(d)))
+            assert ang == pytest.approx(ph, abs=1e-9)
+    with pytest.raises(ValueError):
+        ConeSet([0.0], [7.5], np.zeros((1, 5996)), semantics="cuda_dose",
+                frame=np.eye(3))
