@@ -196,6 +196,27 @@ def run_hetero(args, eng):
                    case.cones.phis, kern)
     dropin_ms = (time.perf_counter() - c0) * 1e3 / reps
 
+    # physics mode on the same beam: beam-aligned cones, kernel looked up
+    # at the radiological distance (EDKnrc kernel, 12 x 12 cones)
+    from conehead_amd.cones import ConeSet, beam_frame
+    from conehead_amd.kernel import EdkSet, merge_polar
+    poly = EdkSet.load().poly(st["energy_weights"])
+    phis, cum = merge_polar(poly.angles, poly.kernel_cum,
+                            len(case.cones.phis))
+    pcs = ConeSet(case.cones.thetas, phis, cum, semantics="physics",
+                  frame=beam_frame(src))
+    pplan = eng.plan(shape, case.spacing, pcs)
+    d_p = DeviceArray(shape, np.float32)
+    E.convolve_rad(pplan, d_t, d_rho, d_p, d_box, stream=s)
+    p0, p1 = Event(), Event()
+    s.sync()
+    p0.record(s)
+    for _ in range(reps):
+        E.convolve_rad(pplan, d_t, d_rho, d_p, d_box, stream=s)
+    p1.record(s)
+    s.sync()
+    physics_ms = p0.elapsed_ms(p1) / reps
+
     result_err, cpu = None, None
     if not args.no_cpu_baseline:
         threads = cpu_threads(args)
@@ -238,6 +259,7 @@ def run_hetero(args, eng):
         "terma_ms": float((step_ms - gather_ms).mean()),
         "error_vs_fp64_oracle": result_err,
         "dropin_convolve_c_ms": dropin_ms,
+        "physics_gather_ms": physics_ms,
         "roofline": {
             "bound": "hbm",
             "achieved": achieved,

@@ -146,3 +146,38 @@ def test_gantry_90_is_gantry_0_rotated(engine):
     # (i, j) at gantry 90 sees what (n-1-j, i) sees at gantry 0.
     rot = np.transpose(doses[0.0], (1, 0, 2))[::-1, :, :]
     assert rel_err(doses[90.0], rot) < 1e-3, rel_err(doses[90.0], rot)
+
+
+def test_water_pdd_against_tank(engine):
+    """Validation, not parity: the physics-mode central-axis depth dose of
+    a 10 x 10 cm field in water (SSD 100, 12 x 12 cones, EDKnrc kernel)
+    against the measured tank depth dose the reference carries
+    (conehead/conehead.py:1047-1061, tests/golden/tank_10x10.json).
+    Beyond the build-up region the two agree within the bound below."""
+    import json
+    import os
+    from conehead_amd.block import Block
+    from conehead_amd.conehead import Conehead
+    from conehead_amd.phantom import Phantom
+    tank = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                       "tank_10x10.json")))["pdd"]
+    nx, ny, s = 40, 64, 0.5
+    origin = np.array([-(nx - 1) / 2 * s, s / 2, -(nx - 1) / 2 * s])
+    phantom = Phantom(np.ones((nx, ny, nx), np.float32), origin, [s] * 3)
+    blk = Block()
+    blk.set_square(10)
+    settings = dict(W.SETTINGS, semantics="physics", density_xy_swap=False)
+    dose = Conehead(engine).calculate(Source("varian_clinac_6MV"), blk,
+                                      phantom, settings)
+    cax = dose[nx // 2 - 1:nx // 2 + 1, :, nx // 2 - 1:nx // 2 + 1].mean(
+        axis=(0, 2))
+    depth = origin[1] + s * np.arange(ny)
+    calc = 100.0 * cax / cax.max()
+    meas = np.interp(depth, tank["depth_cm"], tank["dose_pct"])
+    sel = (depth >= 2.0) & (depth <= 25.0)
+    dev = np.abs(calc[sel] - meas[sel]).max()
+    print("max |calc - tank| over 2-25 cm: %.2f %% of Dmax" % dev)
+    print("depth  calc  tank")
+    for d, c, m in zip(depth[::4], calc[::4], meas[::4]):
+        print("%5.2f %6.2f %6.2f" % (d, c, m))
+    assert dev < 2.5
