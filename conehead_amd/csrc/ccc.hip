@@ -49,6 +49,7 @@ __device__ __forceinline__ ConvJob load_job(const c_job* p)
     j.lin = p->lin;
     j.k = p->k;
     j.ftab = p->ftab;
+    j.ftT = p->ftT;
     j.meta = p->meta;
     j.off = p->off;
     j.lin2 = p->lin2;
@@ -177,18 +178,20 @@ __device__ __forceinline__ void gather_tile(const ConvJob& j, int tile,
             const int c = cb + lane;
             if (c < j.n_cones) {
                 const ConeMeta* m = j.meta + c;
-                const int32_t* f = j.ftab;
+                const int n = j.n_cones;
+                const int32_t* f0 = j.ftT;
+                const int32_t* f1 = j.ftT + ftab_t_base(1, j.X, j.Y, n);
+                const int32_t* f2 = j.ftT + ftab_t_base(2, j.X, j.Y, n);
                 int ax, bx, ay1, by1, ay2, by2, az1, bz1, az2, bz2;
-                vrange(x, lo0, hi0, m->step[0], f + m->f_begin[0],
-                       m->f_len[0], ax, bx);
-                vrange(y0, lo1, hi1, m->step[1], f + m->f_begin[1],
-                       m->f_len[1], ay1, by1);
-                vrange(yl, lo1, hi1, m->step[1], f + m->f_begin[1],
-                       m->f_len[1], ay2, by2);
-                vrange(z0, lo2, hi2, m->step[2], f + m->f_begin[2],
-                       m->f_len[2], az1, bz1);
-                vrange(zl, lo2, hi2, m->step[2], f + m->f_begin[2],
-                       m->f_len[2], az2, bz2);
+                vrange_t(x, lo0, hi0, m->step[0], f0, j.X + 2, n, c, ax, bx);
+                vrange_t(y0, lo1, hi1, m->step[1], f1, j.Y + 2, n, c, ay1,
+                         by1);
+                vrange_t(yl, lo1, hi1, m->step[1], f1, j.Y + 2, n, c, ay2,
+                         by2);
+                vrange_t(z0, lo2, hi2, m->step[2], f2, j.Z + 2, n, c, az1,
+                         bz1);
+                vrange_t(zl, lo2, hi2, m->step[2], f2, j.Z + 2, n, c, az2,
+                         bz2);
                 // union of the tile's per-voxel ranges (each axis is
                 // monotone, so its extremes sit at the tile's end points)
                 r_lo = max(max(ax, min(ay1, ay2)), min(az1, az2));
@@ -761,6 +764,7 @@ static ConvJob make_job(const ch_plan* p, const float* padded, void* dose,
     j.lin = p->d_lin;
     j.k = p->d_k;
     j.ftab = p->d_ftab;
+    j.ftT = p->d_ftT;
     j.meta = p->d_meta;
     j.off = p->d_off;
     j.lin2 = p->d_lin2;

@@ -85,6 +85,7 @@ struct ConvJob {
     const int32_t* lin;        // per tap source byte offset in padded TERMA
     const float* k;            // per tap FP32 increment
     const int32_t* ftab;
+    const int32_t* ftT;        // first-step tables, cone-minor (ftab_t)
     const ConeMeta* meta;
     const int32_t* off;        // per tap offset (x, y, z)
     const int32_t* lin2;       // per tap byte offset in the cone's LDS box
@@ -214,6 +215,38 @@ __device__ __forceinline__ void vrange(int v, int lo, int hi, int step,
 
 constexpr int kBufferFlags = 0x00020000;  // raw 32-bit buffer resource
 
+// Start of axis a's block in the cone-minor first-step tables of a grid
+// (X, Y, Z) with n cones: blocks of (dim + 2) rows of n entries.
+__host__ __device__ __forceinline__ int64_t ftab_t_base(int a, int X, int Y,
+                                                        int n)
+{
+    return a == 0 ? 0
+                  : (a == 1 ? (int64_t)(X + 2) * n
+                            : (int64_t)(X + 2 + Y + 2) * n);
+}
+
+// vrange on the cone-minor tables: ft points at the axis block, rows = the
+// axis' dimension + 2, n = number of cones, c = this lane's cone.
+__device__ __forceinline__ void vrange_t(int v, int lo, int hi, int step,
+                                         const int32_t* __restrict__ ft,
+                                         int rows, int n, int c, int& a,
+                                         int& b)
+{
+    int L, U;
+    if (step > 0) {
+        L = v - hi;
+        U = v - lo;
+    } else {
+        L = lo - v;
+        U = hi - v;
+    }
+    const int last = rows - 1;
+    L = L < 0 ? 0 : (L > last ? last : L);
+    U = U + 1 < 0 ? 0 : (U + 1 > last ? last : U + 1);
+    a = ft[(int64_t)L * n + c];
+    b = ft[(int64_t)U * n + c];
+}
+
 }  // namespace ch
 
 struct ch_plan {
@@ -233,6 +266,11 @@ struct ch_plan {
     int32_t* d_off;
     int32_t* d_lin2;
     int32_t* d_ftab;
+    // The first-step tables again, cone-minor: axis a's entry q of cone c
+    // at ftab_t_base(a) + q * n_cones + c for q < dims[a] + 2, entries past
+    // a cone's own table repeating its last one (so lanes that look up the
+    // same q for different cones read consecutive words).
+    int32_t* d_ftT;
     ch::ConeMeta* d_meta;
     std::vector<ch::ConeMeta> h_meta;
     // CH_SEM_RADIOLOGICAL plans only: the kernel rows (n_rows x row_len

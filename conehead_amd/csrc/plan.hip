@@ -123,6 +123,7 @@ static void plan_free(ch_plan* p)
     hipFree(p->d_off);
     hipFree(p->d_lin2);
     hipFree(p->d_ftab);
+    hipFree(p->d_ftT);
     hipFree(p->d_meta);
     hipFree(p->d_cum);
     hipFree(p->d_perm);
@@ -236,6 +237,7 @@ ch_status ch_plan_create(const int32_t h_shape[3], const double h_vsize[3],
     p->d_off = nullptr;
     p->d_lin2 = nullptr;
     p->d_ftab = nullptr;
+    p->d_ftT = nullptr;
     p->d_meta = nullptr;
     p->d_cum = nullptr;
     p->d_perm = nullptr;
@@ -295,6 +297,29 @@ ch_status ch_plan_create(const int32_t h_shape[3], const double h_vsize[3],
                          hipMemcpyHostToDevice));
         CH_TRY(hipMemcpy(p->d_meta, p->h_meta.data(),
                          (size_t)n_cones * sizeof(ConeMeta),
+                         hipMemcpyHostToDevice));
+    }
+    if (e == hipSuccess) {
+        // cone-minor copy of the first-step tables
+        std::vector<int32_t> ft(nf);
+        CH_TRY(hipMemcpy(ft.data(), p->d_ftab, nf * sizeof(int32_t),
+                         hipMemcpyDeviceToHost));
+        const int32_t X = h_shape[0], Y = h_shape[1], Z = h_shape[2];
+        const size_t nt_t = (size_t)(X + 2 + Y + 2 + Z + 2) * n_cones;
+        std::vector<int32_t> ftT(nt_t);
+        const int32_t dims[3] = {X, Y, Z};
+        for (int a = 0; a < 3; ++a) {
+            const int64_t base = ftab_t_base(a, X, Y, n_cones);
+            for (int32_t c = 0; c < n_cones; ++c) {
+                const ConeMeta& m = p->h_meta[c];
+                const int32_t* f = ft.data() + m.f_begin[a];
+                for (int32_t q = 0; q < dims[a] + 2; ++q)
+                    ftT[base + (int64_t)q * n_cones + c] =
+                        f[q < m.f_len[a] ? q : m.f_len[a] - 1];
+            }
+        }
+        CH_TRY(hipMalloc((void**)&p->d_ftT, nt_t * sizeof(int32_t)));
+        CH_TRY(hipMemcpy(p->d_ftT, ftT.data(), nt_t * sizeof(int32_t),
                          hipMemcpyHostToDevice));
     }
     if (semantics == CH_SEM_RADIOLOGICAL) {
