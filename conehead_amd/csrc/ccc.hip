@@ -50,6 +50,7 @@ __device__ __forceinline__ ConvJob load_job(const c_job* p)
     j.k = p->k;
     j.ftab = p->ftab;
     j.ftT = p->ftT;
+    j.slabs = p->slabs;
     j.meta = p->meta;
     j.off = p->off;
     j.lin2 = p->lin2;
@@ -150,17 +151,6 @@ __device__ __forceinline__ void gather_tile(const ConvJob& j, int tile,
     const int Zp = j.Z + 2 * kZPad;
     const int Yp = j.Y + 2 * kYPad;
 
-    int lo0 = 0, lo1 = 0, lo2 = 0;
-    int hi0 = j.X - 1, hi1 = j.Y - 1, hi2 = j.Z - 1;
-    if (j.box) {
-        const c_i32* b = as_const(j.box);
-        lo0 = max(lo0, b[0]);
-        lo1 = max(lo1, b[1]);
-        lo2 = max(lo2, b[2]);
-        hi0 = min(hi0, b[3]);
-        hi1 = min(hi1, b[4]);
-        hi2 = min(hi2, b[5]);
-    }
     // byte offset of this lane's own voxel in the padded T allocation
     const uint32_t base4 =
         (uint32_t)(((x * Yp + y + kYPad) * Zp + z + kZPad) * 4);
@@ -169,19 +159,36 @@ __device__ __forceinline__ void gather_tile(const ConvJob& j, int tile,
         (void*)alloc, (short)0, j.X * Yp * Zp * 4, kBufferFlags);
     const c_i32* lin = as_const(j.lin);
     const c_f32* kk = as_const(j.k);
+    const int n = j.n_cones;
+    const int32_t* f0 = j.ftT;
+    const int32_t* f1 = j.ftT + ftab_t_base(1, j.X, j.Y, n);
+    const int32_t* f2 = j.ftT + ftab_t_base(2, j.X, j.Y, n);
+    // support boxes: the x-slab boxes of T, or the one box (or the grid)
+    const c_i32* sl = j.slabs ? as_const(j.slabs) : nullptr;
+    const int ns = sl ? sl[0] : 1;
 
     float sum = 0.0f, comp = 0.0f;
-    if (lo0 <= hi0 && lo1 <= hi1 && lo2 <= hi2) {
-        for (int cb = 0; cb < j.n_cones; cb += 64) {
-            // lane l: union range and tap list of cone cb + l
+    for (int cb = 0; cb < n; cb += 64) {
+        for (int s = 0; s < ns; ++s) {
+            int lo0 = 0, lo1 = 0, lo2 = 0;
+            int hi0 = j.X - 1, hi1 = j.Y - 1, hi2 = j.Z - 1;
+            const c_i32* b = sl ? sl + kSlabStride * (1 + s)
+                                : (j.box ? as_const(j.box) : nullptr);
+            if (b) {
+                lo0 = max(lo0, b[0]);
+                lo1 = max(lo1, b[1]);
+                lo2 = max(lo2, b[2]);
+                hi0 = min(hi0, b[3]);
+                hi1 = min(hi1, b[4]);
+                hi2 = min(hi2, b[5]);
+            }
+            if (lo0 > hi0 || lo1 > hi1 || lo2 > hi2)
+                continue;
+            // lane l: the tile's union range of cone cb + l within the box
             int r_lo = 0, r_hi = 0, r_tb = 0;
             const int c = cb + lane;
-            if (c < j.n_cones) {
+            if (c < n) {
                 const ConeMeta* m = j.meta + c;
-                const int n = j.n_cones;
-                const int32_t* f0 = j.ftT;
-                const int32_t* f1 = j.ftT + ftab_t_base(1, j.X, j.Y, n);
-                const int32_t* f2 = j.ftT + ftab_t_base(2, j.X, j.Y, n);
                 int ax, bx, ay1, by1, ay2, by2, az1, bz1, az2, bz2;
                 vrange_t(x, lo0, hi0, m->step[0], f0, j.X + 2, n, c, ax, bx);
                 vrange_t(y0, lo1, hi1, m->step[1], f1, j.Y + 2, n, c, ay1,
@@ -198,21 +205,39 @@ __device__ __forceinline__ void gather_tile(const ConvJob& j, int tile,
                 r_hi = min(min(bx, max(by1, by2)), max(bz1, bz2));
                 r_tb = m->tap_begin;
             }
-            const int nc = min(64, j.n_cones - cb);
-            for (int i = 0; i < nc; ++i) {
+            // cones with taps in this box
+            uint64_t live = __ballot(r_lo < r_hi);
+            while (live) {
+                const int i = __builtin_ctzll(live);
+                live &= live - 1;
                 const int ulo = __builtin_amdgcn_readlane(r_lo, i);
                 const int uhi = __builtin_amdgcn_readlane(r_hi, i);
-                if (ulo >= uhi)
-                    continue;
                 const int tb = __builtin_amdgcn_readlane(r_tb, i);
                 const c_i32* ml = lin + tb;
                 const c_f32* mk = kk + tb;
                 float acc = 0.0f, ac = 0.0f;
-                for (int t = ulo; t < uhi; t += kUnroll) {
+                int t = ulo;
+                for (; t + kUnroll <= uhi; t += kUnroll) {
                     float v[kUnroll], kc[kUnroll];
 #pragma unroll
                     for (int u = 0; u < kUnroll; ++u) {
                         kc[u] = mk[t + u];
+                        v[u] = __int_as_float(
+                            __builtin_amdgcn_raw_buffer_load_b32(
+                                rsrc, (int)(base4 - (uint32_t)ml[t + u]), 0,
+                                0));
+                    }
+                    accumulate_chunk<ACC>(acc, ac, kc, v);
+                }
+                if (t < uhi) {
+                    // the last, partial chunk: taps past the range end get
+                    // a zero weight (their sources may lie in the next
+                    // slab's box); the loads stay in bounds (tap padding,
+                    // y/z padding, x range check)
+                    float v[kUnroll], kc[kUnroll];
+#pragma unroll
+                    for (int u = 0; u < kUnroll; ++u) {
+                        kc[u] = t + u < uhi ? mk[t + u] : 0.0f;
                         v[u] = __int_as_float(
                             __builtin_amdgcn_raw_buffer_load_b32(
                                 rsrc, (int)(base4 - (uint32_t)ml[t + u]), 0,
@@ -708,6 +733,129 @@ __global__ __launch_bounds__(256) void pad_kernel(const float* __restrict__ T,
                : 0.0f;
 }
 
+// ------------------------------------------------------ slab boxes ----
+
+// Extent of T != 0 in x plane blockIdx.x (within the caller's box when
+// given): planes[4 p + 0..3] = min y, max y, min z, max z (min > max when
+// the plane is empty).
+__global__ __launch_bounds__(256) void plane_box_kernel(
+    const float* __restrict__ T, int Y, int Z, const int32_t* box,
+    int32_t* planes)
+{
+    const int p = blockIdx.x;
+    const float* P = T + (size_t)p * Y * Z;
+    int32_t v[6] = {INT_MAX, INT_MAX, INT_MAX, INT_MIN, INT_MIN, INT_MIN};
+    bool in_box = true;
+    int ylo = 0, yhi = Y - 1, zlo = 0, zhi = Z - 1;
+    if (box) {
+        in_box = p >= box[0] && p <= box[3];
+        ylo = max(ylo, box[1]);
+        yhi = min(yhi, box[4]);
+        zlo = max(zlo, box[2]);
+        zhi = min(zhi, box[5]);
+    }
+    if (in_box)
+        for (int y = ylo + (int)(threadIdx.x >> 6); y <= yhi; y += 4)
+            for (int z = zlo + (int)(threadIdx.x & 63); z <= zhi; z += 64)
+                if (P[(size_t)y * Z + z] != 0.0f) {
+                    v[1] = min(v[1], y);
+                    v[4] = max(v[4], y);
+                    v[2] = min(v[2], z);
+                    v[5] = max(v[5], z);
+                }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        v[1] = min(v[1], __shfl_xor(v[1], o, 64));
+        v[2] = min(v[2], __shfl_xor(v[2], o, 64));
+        v[4] = max(v[4], __shfl_xor(v[4], o, 64));
+        v[5] = max(v[5], __shfl_xor(v[5], o, 64));
+    }
+    __shared__ int32_t part[4][4];
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        part[w][0] = v[1];
+        part[w][1] = v[4];
+        part[w][2] = v[2];
+        part[w][3] = v[5];
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        const int a = threadIdx.x;  // 0, 2: minima; 1, 3: maxima
+        int32_t r = part[0][a];
+        for (int k = 1; k < 4; ++k)
+            r = (a & 1) ? max(r, part[k][a]) : min(r, part[k][a]);
+        planes[4 * p + a] = r;
+    }
+}
+
+// Slab records from the plane extents (one block of 64 threads): lane s
+// builds slab s, then lane 0 merges neighbours whose union box is at most
+// 2 % larger than the two boxes together and writes the records.
+__global__ __launch_bounds__(64) void slab_kernel(const int32_t* planes,
+                                                  int X, int32_t* slabs)
+{
+    __shared__ int32_t raw[64][6];
+    const int n_raw = (X + kSlabW - 1) / kSlabW;
+    for (int s = threadIdx.x; s < n_raw; s += 64) {
+        int32_t b[6] = {INT_MAX, INT_MAX, INT_MAX, INT_MIN, INT_MIN,
+                        INT_MIN};
+        for (int p = s * kSlabW; p < min(X, (s + 1) * kSlabW); ++p) {
+            const int32_t* q = planes + 4 * p;
+            if (q[0] > q[1])
+                continue;
+            b[0] = min(b[0], p);
+            b[3] = max(b[3], p);
+            b[1] = min(b[1], q[0]);
+            b[4] = max(b[4], q[1]);
+            b[2] = min(b[2], q[2]);
+            b[5] = max(b[5], q[3]);
+        }
+        // slabs beyond the shared table (X > 64 kSlabW) are written in
+        // place and merged below from global memory
+        int32_t* dst = s < 64 ? raw[s] : slabs + kSlabStride * (1 + s);
+        for (int a = 0; a < 6; ++a)
+            dst[a] = b[a];
+    }
+    __syncthreads();
+    if (threadIdx.x != 0)
+        return;
+    auto vol = [](const int32_t* b) {
+        return (int64_t)(b[3] - b[0] + 1) * (b[4] - b[1] + 1) *
+               (b[5] - b[2] + 1);
+    };
+    int n = 0;
+    int32_t cur[6];
+    for (int s = 0; s < n_raw; ++s) {
+        const int32_t* b = s < 64 ? raw[s] : slabs + kSlabStride * (1 + s);
+        int32_t nb[6];
+        for (int a = 0; a < 6; ++a)
+            nb[a] = b[a];
+        if (nb[0] > nb[3])
+            continue;  // no T in this slab
+        if (n > 0) {
+            int32_t u[6];
+            for (int a = 0; a < 3; ++a) {
+                u[a] = min(cur[a], nb[a]);
+                u[3 + a] = max(cur[3 + a], nb[3 + a]);
+            }
+            if (50 * vol(u) <= 51 * (vol(cur) + vol(nb))) {
+                for (int a = 0; a < 6; ++a)
+                    cur[a] = u[a];
+                continue;
+            }
+            for (int a = 0; a < 6; ++a)
+                slabs[kSlabStride * n + a] = cur[a];  // record n of 1..
+        }
+        ++n;
+        for (int a = 0; a < 6; ++a)
+            cur[a] = nb[a];
+    }
+    if (n > 0)
+        for (int a = 0; a < 6; ++a)
+            slabs[kSlabStride * n + a] = cur[a];
+    slabs[0] = n;
+}
+
 // ----------------------------------------------------------- dispatch ----
 
 template <typename OutT>
@@ -765,6 +913,7 @@ static ConvJob make_job(const ch_plan* p, const float* padded, void* dose,
     j.k = p->d_k;
     j.ftab = p->d_ftab;
     j.ftT = p->d_ftT;
+    j.slabs = nullptr;
     j.meta = p->d_meta;
     j.off = p->d_off;
     j.lin2 = p->d_lin2;
@@ -912,6 +1061,24 @@ ch_status pad_pair(const ch_plan* p, const float* a, const float* b,
     return CH_OK;
 }
 
+// The x-slab boxes of T (used by the direct gather) into `ints`
+// (slab_ints(X) of them); returns the slab records.
+static int32_t* launch_slabs(const ch_plan* p, const float* T,
+                             const int32_t* box, int32_t* ints,
+                             hipStream_t s)
+{
+    int32_t* planes = ints;
+    int32_t* slabs = ints + 4 * (size_t)p->shape[0];
+    hipLaunchKernelGGL(plane_box_kernel, dim3((unsigned)p->shape[0]),
+                       dim3(256), 0, s, T, p->shape[1], p->shape[2], box,
+                       planes);
+    hipLaunchKernelGGL(slab_kernel, dim3(1), dim3(64), 0, s,
+                       (const int32_t*)planes, p->shape[0], slabs);
+    return slabs;
+}
+
+static bool uses_slabs(const ch_plan* p) { return p->tile[0] == 1; }
+
 template <typename OutT>
 static ch_status convolve_one(const ch_plan* p, const float* d_terma,
                               OutT* d_dose, const int32_t* d_box,
@@ -925,13 +1092,19 @@ static ch_status convolve_one(const ch_plan* p, const float* d_terma,
     hipStream_t s = (hipStream_t)stream;
     std::lock_guard<std::mutex> lock(g_ws_mu);
     Workspace* w = nullptr;
-    st = workspace(s, padded_elems(p), 0, &w);
+    const size_t np = padded_elems(p);
+    st = workspace(s, np + slab_ints(p->shape[0]), 0, &w);
     if (st != CH_OK)
         return st;
     st = pad_into(d_terma, w->pad, p, s);
     if (st != CH_OK)
         return st;
     ConvJob j = make_job(p, w->pad, d_dose, d_box, accumulate);
+    if (uses_slabs(p)) {
+        j.slabs = launch_slabs(p, d_terma, d_box,
+                               reinterpret_cast<int32_t*>(w->pad + np), s);
+        CH_HIP(hipGetLastError());
+    }
     const int64_t n_tiles = job_tiles(j);
     CH_REQUIRE(n_tiles < INT_MAX / 2, "too many tiles");
     const unsigned blocks =
@@ -1045,16 +1218,21 @@ ch_status ch_convolve_batch(int32_t n_jobs, const ch_conv_job* h_jobs,
         ch_status st = kernels_for<float>(p0, k);
         if (st != CH_OK)
             return st;
-        size_t pad_total = 0;
-        for (int32_t i : group)
+        size_t pad_total = 0, ints_total = 0;
+        for (int32_t i : group) {
             pad_total += padded_elems(h_jobs[i].plan);
+            if (uses_slabs(h_jobs[i].plan))
+                ints_total += slab_ints(h_jobs[i].plan->shape[0]);
+        }
         Workspace* w = nullptr;
-        st = workspace(s, pad_total, (size_t)ng * sizeof(ConvJob), &w);
+        st = workspace(s, pad_total + ints_total,
+                       (size_t)ng * sizeof(ConvJob), &w);
         if (st != CH_OK)
             return st;
         ConvJob* tab = (ConvJob*)w->h_tab;
         int64_t tiles = 0;
         size_t pad_off = 0;
+        int32_t* ints = reinterpret_cast<int32_t*>(w->pad + pad_total);
         for (int32_t g = 0; g < ng; ++g) {
             const ch_conv_job& hj = h_jobs[group[g]];
             float* P = w->pad + pad_off;
@@ -1064,6 +1242,12 @@ ch_status ch_convolve_batch(int32_t n_jobs, const ch_conv_job* h_jobs,
             pad_off += padded_elems(hj.plan);
             ConvJob j = make_job(hj.plan, P, hj.d_dose, hj.d_box,
                                  hj.accumulate);
+            if (uses_slabs(hj.plan)) {
+                j.slabs = launch_slabs(hj.plan, hj.d_terma, hj.d_box, ints,
+                                       s);
+                CH_HIP(hipGetLastError());
+                ints += slab_ints(hj.plan->shape[0]);
+            }
             j.tile_begin = (int32_t)tiles;
             tiles += job_tiles(j);
             CH_REQUIRE(tiles < INT_MAX / 2, "batch has too many tiles");

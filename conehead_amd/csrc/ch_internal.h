@@ -36,6 +36,18 @@ constexpr int32_t kRowLenMin = kMaxIndex + 1;
 // Zero taps appended to every cone's tap list (>= the gather's unroll).
 constexpr int32_t kTapPad = 8;
 constexpr int kUnroll = 8;  // taps per gather step (<= kTapPad)
+// x-slab support boxes of a TERMA grid: slabs of kSlabW x planes, each with
+// the bounding box of its T != 0 voxels; neighbours whose union box is
+// hardly larger than the pair merge.  Record 0 holds the count, record
+// 1 + s the box of slab s (min x, y, z, max x, y, z), kSlabStride ints each.
+constexpr int kSlabW = 16;
+constexpr int kSlabStride = 8;
+inline int slab_records(int X) { return 1 + (X + kSlabW - 1) / kSlabW; }
+// ints of a slab workspace: per-plane extents, then the slab records
+inline size_t slab_ints(int X)
+{
+    return (size_t)4 * X + (size_t)kSlabStride * slab_records(X);
+}
 
 // Per-cone metadata of a plan (device and host copies).
 //  taps [tap_begin, tap_begin + n_taps): source byte offset and increment.
@@ -86,6 +98,7 @@ struct ConvJob {
     const float* k;            // per tap FP32 increment
     const int32_t* ftab;
     const int32_t* ftT;        // first-step tables, cone-minor (ftab_t)
+    const int32_t* slabs;      // x-slab support boxes (nullable: box)
     const ConeMeta* meta;
     const int32_t* off;        // per tap offset (x, y, z)
     const int32_t* lin2;       // per tap byte offset in the cone's LDS box

@@ -299,3 +299,32 @@ def test_cuda_dose_semantics(oracle, engine):
                                 np.diff(cum[cs.rows[c]][idx[:n]])])
         assert np.array_equal(off, vox[:n] - [shape[0], shape[1], shape[2]])
         np.testing.assert_array_equal(k, ref_k)
+
+
+@pytest.mark.parametrize("tile", [(1, 4, 16), (1, 2, 32), (1, 1, 64)])
+def test_diagonal_support_uses_slab_boxes(oracle, engine, kern, tile):
+    """A TERMA band running diagonally in x-y: its support box is loose, so
+    the gather splits it into x-slab boxes (with a zero-weighted partial
+    chunk at every slab's range end).  Checked against the oracle's scatter
+    form, with and without the caller's support box."""
+    from conehead_amd import engine as E
+    from conehead_amd.device import DeviceArray
+    shape, vs = (72, 56, 20), (0.3, 0.3, 0.3)
+    cs, _ = cones(kern, 6, 4)
+    rng = np.random.default_rng(11)
+    T = np.zeros(shape)
+    xx, yy = np.meshgrid(np.arange(shape[0]), np.arange(shape[1]),
+                         indexing="ij")
+    band = np.abs(yy - 0.6 * xx - 6) < 4
+    T[band, 6:14] = 1.0 + rng.random((band.sum(), 8))
+    ref = np.zeros(shape)
+    oracle.convolve_dirs(T, ref, vs, cs.dirs, cs.rows, cs.cum, n_threads=8)
+    plan = engine.plan(shape, vs, cs)
+    plan.set_tile(*tile)
+    d_t = DeviceArray.from_host(T.astype(np.float32))
+    d_box = DeviceArray((6,), np.int32)
+    E.support_box(d_t, shape, d_box)
+    for box in (d_box, None):
+        d_d = DeviceArray(shape, zero=True)
+        E.convolve(plan, d_t, d_d, box)
+        assert rel_err(d_d.download(), ref) < TOL
