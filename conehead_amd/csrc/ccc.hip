@@ -98,32 +98,32 @@ __device__ __forceinline__ void urange(int v, int lo, int hi, int step,
 }
 
 
-// Sum of one chunk of kUnroll taps into a cone line's accumulator:
+// Sum of one chunk of N taps (kUnroll unless stated) into a cone line's
+// accumulator:
 //   ACC 0  Kahan-compensated per tap,
 //   ACC 1  plain FMA along the line (the line total is compensated),
 //   ACC 2  plain sum of the chunk, Kahan-compensated across chunks.
-template <int ACC>
+template <int ACC, int N = kUnroll>
 __device__ __forceinline__ void accumulate_chunk(float& acc, float& ac,
                                                  const float* kc,
                                                  const float* v)
 {
     if (ACC == 0) {
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u)
+        for (int u = 0; u < N; ++u)
             kahan_add(acc, ac, kc[u] * v[u]);
     } else if (ACC == 1) {
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u)
+        for (int u = 0; u < N; ++u)
             acc = fmaf(kc[u], v[u], acc);
     } else {
         float part = kc[0] * v[0];
 #pragma unroll
-        for (int u = 1; u < kUnroll; ++u)
+        for (int u = 1; u < N; ++u)
             part = fmaf(kc[u], v[u], part);
         kahan_add(acc, ac, part);
     }
 }
-
 
 // One wavefront owns a 1 x TY x TZ output tile (TY * TZ = 64 lanes, z
 // fastest).  The tile's union tap range of 64 cones at a time is computed
@@ -229,21 +229,21 @@ __device__ __forceinline__ void gather_tile(const ConvJob& j, int tile,
                     }
                     accumulate_chunk<ACC>(acc, ac, kc, v);
                 }
-                if (t < uhi) {
-                    // the last, partial chunk: taps past the range end get
-                    // a zero weight (their sources may lie in the next
-                    // slab's box); the loads stay in bounds (tap padding,
-                    // y/z padding, x range check)
-                    float v[kUnroll], kc[kUnroll];
+                // the rest in chunks of 4, the last one partial: taps past
+                // the range end get a zero weight (their sources may lie in
+                // the next slab's box); the loads stay in bounds (tap
+                // padding, y/z padding, x range check)
+                for (; t < uhi; t += 4) {
+                    float v[4], kc[4];
 #pragma unroll
-                    for (int u = 0; u < kUnroll; ++u) {
+                    for (int u = 0; u < 4; ++u) {
                         kc[u] = t + u < uhi ? mk[t + u] : 0.0f;
                         v[u] = __int_as_float(
                             __builtin_amdgcn_raw_buffer_load_b32(
                                 rsrc, (int)(base4 - (uint32_t)ml[t + u]), 0,
                                 0));
                     }
-                    accumulate_chunk<ACC>(acc, ac, kc, v);
+                    accumulate_chunk<ACC, 4>(acc, ac, kc, v);
                 }
                 kahan_add(sum, comp, acc);
             }
@@ -539,6 +539,34 @@ __device__ __forceinline__ GroupJob load_gjob(const c_gjob* p)
     return j;
 }
 
+// N taps from t of one cone for every beam of a group; with `masked`,
+// taps at or past uhi read offset 0 with increment 0.
+template <int ACC, int N>
+__device__ __forceinline__ void group_chunk(int t, int uhi, bool masked,
+                                            const c_i32* ml, const c_f32* mk,
+                                            const char* Tb, uint32_t base16,
+                                            float* acc, float* ac)
+{
+    float4 v[N];
+    float kc[N];
+#pragma unroll
+    for (int u = 0; u < N; ++u) {
+        const bool in = !masked || t + u < uhi;
+        const uint32_t o = (uint32_t)ml[t + u] << 2;
+        v[u] = *reinterpret_cast<const float4*>(Tb + (base16 - (in ? o : 0u)));
+        kc[u] = in ? mk[t + u] : 0.0f;
+    }
+#pragma unroll
+    for (int b = 0; b < kGroup; ++b) {
+        float vb[N];
+#pragma unroll
+        for (int u = 0; u < N; ++u)
+            vb[u] = b == 0 ? v[u].x
+                           : (b == 1 ? v[u].y : (b == 2 ? v[u].z : v[u].w));
+        accumulate_chunk<ACC, N>(acc[b], ac[b], kc, vb);
+    }
+}
+
 template <int TY, int TZ, int ACC>
 __device__ __forceinline__ void gather_tile_group(const GroupJob& j, int tile,
                                                   int lane)
@@ -620,32 +648,16 @@ __device__ __forceinline__ void gather_tile_group(const GroupJob& j, int tile,
             const c_i32* ml = lin + m->tap_begin;
             const c_f32* mk = kk + m->tap_begin;
             float acc[kGroup] = {}, ac[kGroup] = {};
-            // whole chunks without a per-tap condition, then one guarded
-            // remainder whose taps past uhi read offset 0 with increment 0
-            const int tend = ulo + (uhi - ulo) / kUnroll * kUnroll;
-            for (int t = ulo; t < uhi; t += kUnroll) {
-                const bool whole = t < tend;
-                float4 v[kUnroll];
-                float kc[kUnroll];
-#pragma unroll
-                for (int u = 0; u < kUnroll; ++u) {
-                    const bool in = whole || t + u < uhi;
-                    const uint32_t o = (uint32_t)ml[t + u] << 2;
-                    v[u] = *reinterpret_cast<const float4*>(
-                        Tb + (base16 - (in ? o : 0u)));
-                    kc[u] = in ? mk[t + u] : 0.0f;
-                }
-#pragma unroll
-                for (int b = 0; b < kGroup; ++b) {
-                    float vb[kUnroll];
-#pragma unroll
-                    for (int u = 0; u < kUnroll; ++u)
-                        vb[u] = b == 0 ? v[u].x
-                                       : (b == 1 ? v[u].y
-                                                 : (b == 2 ? v[u].z : v[u].w));
-                    accumulate_chunk<ACC>(acc[b], ac[b], kc, vb);
-                }
-            }
+            // whole chunks of kUnroll, then chunks of 4 (the direct
+            // gather's chunking, so a one-beam group sums like it); taps
+            // past uhi read offset 0 with increment 0
+            int t = ulo;
+            for (; t + kUnroll <= uhi; t += kUnroll)
+                group_chunk<ACC, kUnroll>(t, uhi, false, ml, mk, Tb, base16,
+                                          acc, ac);
+            for (; t < uhi; t += 4)
+                group_chunk<ACC, 4>(t, uhi, true, ml, mk, Tb, base16, acc,
+                                    ac);
 #pragma unroll
             for (int b = 0; b < kGroup; ++b)
                 kahan_add(sum[b], comp[b], acc[b]);
