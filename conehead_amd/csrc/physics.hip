@@ -16,7 +16,7 @@
 // output tile and reads T and the densities from padded copies with
 // range-checked buffer loads.  Every lane walks its cone from its own voxel
 // (R needs the whole path), loading densities only up to the first tap of
-// the tile's union range and both grids from there on.  A workgroup of 16
+// the tile's union range and both grids from there on.  A workgroup of 8
 // wavefronts stages each kernel row (5996 FP64 values) in LDS once and runs
 // the cones of that row.  R, the index and the increments are FP64 and
 // round like the FP64 oracle (this file is built without contraction); the
@@ -37,7 +37,7 @@ struct RadJob {
     const int32_t* box;
     const int32_t* lin;    // per tap: byte offset into the padded grids
     const double* seg;     // per tap: path length inside the voxel
-    const int32_t* ftab;
+    const int32_t* ftT;    // first-step tables, cone-minor
     const ConeMeta* meta;
     const int32_t* perm;   // cones ordered by kernel row
     const int32_t* row_begin;
@@ -46,7 +46,7 @@ struct RadJob {
     int32_t X, Y, Z, n_cones, accumulate, nty, ntz;
 };
 
-constexpr int kRadTY = 4, kRadTZ = 16, kRadWaves = 16;
+constexpr int kRadTY = 4, kRadTZ = 16, kRadWaves = 8;
 constexpr int kRadRow = kMaxIndex + 1;
 
 __device__ __forceinline__ int rad_index(double r)
@@ -127,19 +127,22 @@ __global__ __launch_bounds__(64 * kRadWaves) void gather_rad(RadJob j,
             // lane l: the tile's union range of cone perm[q + l]
             int r_lo = 0, r_hi = 0, r_tb = 0;
             if (q + lane < ce) {
-                const ConeMeta* m = j.meta + j.perm[q + lane];
-                const int32_t* f = j.ftab;
+                const int c = j.perm[q + lane];
+                const ConeMeta* m = j.meta + c;
+                const int n = j.n_cones;
+                const int32_t* f0 = j.ftT;
+                const int32_t* f1 = j.ftT + ftab_t_base(1, j.X, j.Y, n);
+                const int32_t* f2 = j.ftT + ftab_t_base(2, j.X, j.Y, n);
                 int ax, bx, ay1, by1, ay2, by2, az1, bz1, az2, bz2;
-                vrange(x, lo0, hi0, m->step[0], f + m->f_begin[0],
-                       m->f_len[0], ax, bx);
-                vrange(y0, lo1, hi1, m->step[1], f + m->f_begin[1],
-                       m->f_len[1], ay1, by1);
-                vrange(yl, lo1, hi1, m->step[1], f + m->f_begin[1],
-                       m->f_len[1], ay2, by2);
-                vrange(z0, lo2, hi2, m->step[2], f + m->f_begin[2],
-                       m->f_len[2], az1, bz1);
-                vrange(zl, lo2, hi2, m->step[2], f + m->f_begin[2],
-                       m->f_len[2], az2, bz2);
+                vrange_t(x, lo0, hi0, m->step[0], f0, j.X + 2, n, c, ax, bx);
+                vrange_t(y0, lo1, hi1, m->step[1], f1, j.Y + 2, n, c, ay1,
+                         by1);
+                vrange_t(yl, lo1, hi1, m->step[1], f1, j.Y + 2, n, c, ay2,
+                         by2);
+                vrange_t(z0, lo2, hi2, m->step[2], f2, j.Z + 2, n, c, az1,
+                         bz1);
+                vrange_t(zl, lo2, hi2, m->step[2], f2, j.Z + 2, n, c, az2,
+                         bz2);
                 r_lo = max(max(ax, min(ay1, ay2)), min(az1, az2));
                 r_hi = min(min(bx, max(by1, by2)), max(bz1, bz2));
                 r_tb = m->tap_begin;
@@ -246,7 +249,7 @@ ch_status ch_convolve_rad(const ch_plan* plan, const float* d_terma,
     j.box = d_box;
     j.lin = plan->d_lin;
     j.seg = plan->d_k64;
-    j.ftab = plan->d_ftab;
+    j.ftT = plan->d_ftT;
     j.meta = plan->d_meta;
     j.perm = plan->d_perm;
     j.row_begin = plan->d_row_begin;
