@@ -19,6 +19,10 @@
  *  - ch_ref_terma: closed forms (rho = 1 radiological depth equals the
  *    geometric exit distance, inverse-square fluence on the central axis,
  *    blocked fraction of a square field in multiples of 1/samples^3).
+ *  - ch_ref_convolve_rad (physics mode, no reference counterpart): an
+ *    impulse against a forward walk of ch_ref_dda_3d with the cuda_dose
+ *    index, rho = 1 against the geometric mode, rho = 2 against doubled
+ *    voxels, and a 1-D slab closed form.
  */
 #include "ch_ref.h"
 
