@@ -166,6 +166,7 @@ SIGNATURES = {
     "ch_convolve_f64": (ctypes.c_int32, [c_vp, c_vp, c_vp, c_vp, c_vp]),
     "ch_convolve_batch": (ctypes.c_int32, [ctypes.c_int32,
                                            ctypes.POINTER(ConvJob), c_vp]),
+    "ch_set_launch_events": (ctypes.c_int32, [c_vp, c_vp]),
     "ch_convolve_rad": (ctypes.c_int32, [
         c_vp, c_vp, c_vp, c_vp, c_vp, ctypes.c_int32, c_vp]),
     "ch_resample_linear": (ctypes.c_int32, [

@@ -117,13 +117,24 @@ bool capturing(hipStream_t s);
 
 // Optional HIP-event timing of one launch (ch_set_timing): stop() waits for
 // the launch and records its duration for ch_last_kernel_ms().
+// The calling thread's pending launch-event pair (ch_set_launch_events):
+// taken (and cleared) by the next gather or scatter launch.
+bool take_launch_events(hipEvent_t* start, hipEvent_t* stop);
+
+// Also records the caller's launch-event pair, if one is pending, right
+// around the launch (no wait).
 struct LaunchTimer {
     hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipEvent_t u0 = nullptr, u1 = nullptr;
     hipStream_t s;
-    bool on;
+    bool on, user;
     explicit LaunchTimer(hipStream_t st)
-        : s(st), on(timing_enabled() && !capturing(st))
+        : s(st), on(timing_enabled() && !capturing(st)), user(false)
     {
+        if (!capturing(st))
+            user = take_launch_events(&u0, &u1);
+        if (user)
+            (void)hipEventRecord(u0, s);
         if (on) {
             (void)hipEventCreate(&e0);
             (void)hipEventCreate(&e1);
@@ -132,6 +143,8 @@ struct LaunchTimer {
     }
     void stop()
     {
+        if (user)
+            (void)hipEventRecord(u1, s);
         if (!on)
             return;
         (void)hipEventRecord(e1, s);

@@ -57,6 +57,19 @@ bool capturing(hipStream_t s)
 }
 bool timing_enabled() { return g_timing; }
 
+static thread_local hipEvent_t g_launch_ev0 = nullptr;
+static thread_local hipEvent_t g_launch_ev1 = nullptr;
+
+bool take_launch_events(hipEvent_t* start, hipEvent_t* stop)
+{
+    if (!g_launch_ev0 || !g_launch_ev1)
+        return false;
+    *start = g_launch_ev0;
+    *stop = g_launch_ev1;
+    g_launch_ev0 = g_launch_ev1 = nullptr;
+    return true;
+}
+
 // Grid-stride zero fill in 16-byte units plus a byte tail.
 __global__ void zero_kernel(uint8_t* p, size_t n)
 {
@@ -468,6 +481,15 @@ ch_status ch_set_timing(int32_t enabled)
 ch_status ch_last_kernel_ms(float* ms)
 {
     *ms = g_last_kernel_ms;
+    return CH_OK;
+}
+
+ch_status ch_set_launch_events(void* start, void* stop)
+{
+    CH_REQUIRE((start == nullptr) == (stop == nullptr),
+               "ch_set_launch_events: give both events or neither");
+    g_launch_ev0 = (hipEvent_t)start;
+    g_launch_ev1 = (hipEvent_t)stop;
     return CH_OK;
 }
 

@@ -42,6 +42,12 @@ def _h(stream):
     return stream.handle if isinstance(stream, Stream) else stream
 
 
+def arm_launch_events(start, stop):
+    """The calling thread's next gather or scatter launch records `start`
+    and `stop` (Events) right around its kernel (ch_set_launch_events)."""
+    _lib.check(_lib.lib().ch_set_launch_events(start.handle, stop.handle))
+
+
 class Event:
     def __init__(self):
         h = ctypes.c_void_p()

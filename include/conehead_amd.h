@@ -372,6 +372,15 @@ ch_status ch_raster_mlc(float* d_raster, const int32_t* d_table,
 ch_status ch_set_timing(int32_t enabled);
 ch_status ch_last_kernel_ms(float* ms);
 
+/* Arms a pair of events (ch_event_create) for the calling thread: the next
+ * gather or scatter launch of this thread records `start` on its stream
+ * right before the kernel and `stop` right after it, without waiting, and
+ * disarms the pair (read it with ch_event_elapsed once `stop` completed).
+ * Pass two nulls to disarm.  Launches that prepare the gather (padding,
+ * support and slab boxes) fall outside the pair.  No counterpart in the
+ * reference: it is the measurement hook of the benchmark. */
+ch_status ch_set_launch_events(void* start, void* stop);
+
 #ifdef __cplusplus
 }
 #endif
