@@ -207,3 +207,30 @@ def test_beam_aligned_cones():
     with pytest.raises(ValueError):
         ConeSet([0.0], [7.5], np.zeros((1, 5996)), semantics="cuda_dose",
                 frame=np.eye(3))
+
+
+def test_convolve_c_dropin_argument_errors():
+    """The drop-in raises where the reference's typed memoryviews and kernel
+    lookup raise (conehead/convolve_c.pyx:17-20, :269-273), before any GPU
+    work: wrong ndim, wrong dtype, read-only dose, mismatched grids, and a
+    missing kernel key."""
+    from conehead_amd.convolve_c import convolve_c
+    k = merged_kernel(PolyenergeticKernel(), 4)
+    phis = np.array(k.keys(), float)
+    t = np.zeros((4, 4, 4))
+    d = np.zeros((4, 4, 4))
+    dims = np.array([0.5, 0.5, 0.5])
+    with pytest.raises(ValueError, match="dimensions"):
+        convolve_c(np.zeros((4, 4)), d, dims, default_thetas(2), phis, k)
+    with pytest.raises(ValueError, match="dtype"):
+        convolve_c(t.astype(np.float32), d, dims, default_thetas(2), phis, k)
+    with pytest.raises(ValueError, match="dtype"):
+        convolve_c(t, d.astype(np.float32), dims, default_thetas(2), phis, k)
+    ro = np.zeros((4, 4, 4))
+    ro.flags.writeable = False
+    with pytest.raises(ValueError, match="read-only"):
+        convolve_c(t, ro, dims, default_thetas(2), phis, k)
+    with pytest.raises(ValueError, match="does not match"):
+        convolve_c(t, np.zeros((4, 4, 5)), dims, default_thetas(2), phis, k)
+    with pytest.raises(KeyError):
+        convolve_c(t, d, dims, default_thetas(2), [12.0], k)
