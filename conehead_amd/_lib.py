@@ -144,6 +144,7 @@ SIGNATURES = {
     "ch_event_create": (ctypes.c_int32, [c_vpp]),
     "ch_event_destroy": (ctypes.c_int32, [c_vp]),
     "ch_event_record": (ctypes.c_int32, [c_vp, c_vp]),
+    "ch_stream_wait": (ctypes.c_int32, [c_vp, c_vp]),
     "ch_event_sync": (ctypes.c_int32, [c_vp]),
     "ch_event_query": (ctypes.c_int32, [c_vp, c_i32p]),
     "ch_event_elapsed": (ctypes.c_int32, [c_f32p, c_vp, c_vp]),

@@ -446,6 +446,13 @@ ch_status ch_event_record(void* event, void* stream)
     return CH_OK;
 }
 
+ch_status ch_stream_wait(void* stream, void* event)
+{
+    CH_REQUIRE(event, "ch_stream_wait: null event");
+    CH_HIP(hipStreamWaitEvent((hipStream_t)stream, (hipEvent_t)event, 0));
+    return CH_OK;
+}
+
 ch_status ch_event_sync(void* event)
 {
     CH_HIP(hipEventSynchronize((hipEvent_t)event));

@@ -23,6 +23,10 @@ class Stream:
     def sync(self):
         _lib.check(_lib.lib().ch_stream_sync(self.handle))
 
+    def wait(self, event):
+        """Later work on this stream waits for `event`."""
+        _lib.check(_lib.lib().ch_stream_wait(self.handle, event.handle))
+
     def close(self):
         if self._own and self.handle:
             _lib.lib().ch_stream_destroy(self.handle)

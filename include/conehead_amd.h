@@ -56,6 +56,8 @@ ch_status ch_device_sync(void);
 ch_status ch_event_create(void** event);
 ch_status ch_event_destroy(void* event);
 ch_status ch_event_record(void* event, void* stream);
+/* Later work on `stream` waits for `event` (recorded on another stream). */
+ch_status ch_stream_wait(void* stream, void* event);
 ch_status ch_event_sync(void* event);
 /* 1 if the event has completed, 0 if not. */
 ch_status ch_event_query(void* event, int32_t* done);
