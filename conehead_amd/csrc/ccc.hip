@@ -6,14 +6,17 @@
 // zeros on both ends and every x-plane kYPad zero rows on both sides.
 //
 // One wavefront owns a 1 x TY x TZ tile of outputs (64 lanes, z fastest).
-// For a cone, x is uniform, so the box clips the tap list to one range from
-// the per-axis first-step tables; along y and z the per-voxel ranges shift
-// monotonically, and the union over the tile is taken from its end points.
-// Every tap of the union is applied to every lane without a predicate: a
-// lane whose source falls outside the box reads a zero (inside the grid, or
-// in the padding, which is at least one tile wide).  Tap offsets and
-// increments are wave-uniform scalar loads; T loads are TY runs of TZ
-// contiguous floats.
+// The sources are clipped to x-slab support boxes of T (plane_box_kernel
+// and slab_kernel below, one record per group of x planes).  For a box and
+// a cone, x is uniform over the tile, so the box clips the tap list to one
+// range from the per-axis first-step tables; along y and z the per-voxel
+// ranges shift monotonically, and the union over the tile is taken from its
+// end points.  Every tap of the union is applied to every lane without a
+// predicate: a lane whose source falls outside the box along y or z reads a
+// zero (T is zero there within the slab's planes, or it is padding, which
+// is at least one tile wide), and a plane outside the grid reads zero
+// through the range-checked buffer load.  Tap offsets and increments are
+// wave-uniform scalar loads; T loads are TY runs of TZ contiguous floats.
 // No atomics: every output is written once, in a fixed order, so results
 // are bitwise reproducible alone or inside any batch.
 #include <algorithm>

@@ -215,30 +215,6 @@ __device__ __forceinline__ void block_box_partial(int32_t v[6],
     }
 }
 
-// Tap range [a, b) of one axis for coordinate v and box [lo, hi]: the taps
-// whose source coordinate v - o lies in the box, from the axis' first-step
-// table f (per lane: the cone may differ between lanes).  Out-of-range
-// arguments clamp to the ends of the table, which yields empty ranges and
-// keeps a and b monotone in v.
-__device__ __forceinline__ void vrange(int v, int lo, int hi, int step,
-                                       const int32_t* __restrict__ f,
-                                       int f_len, int& a, int& b)
-{
-    int L, U;
-    if (step > 0) {
-        L = v - hi;
-        U = v - lo;
-    } else {
-        L = lo - v;
-        U = hi - v;
-    }
-    const int last = f_len - 1;
-    L = L < 0 ? 0 : (L > last ? last : L);
-    U = U + 1 < 0 ? 0 : (U + 1 > last ? last : U + 1);
-    a = f[L];
-    b = f[U];
-}
-
 constexpr int kBufferFlags = 0x00020000;  // raw 32-bit buffer resource
 
 // Start of axis a's block in the cone-minor first-step tables of a grid
@@ -251,8 +227,12 @@ __host__ __device__ __forceinline__ int64_t ftab_t_base(int a, int X, int Y,
                             : (int64_t)(X + 2 + Y + 2) * n);
 }
 
-// vrange on the cone-minor tables: ft points at the axis block, rows = the
-// axis' dimension + 2, n = number of cones, c = this lane's cone.
+// Tap range [a, b) of one axis for coordinate v and box [lo, hi]: the taps
+// whose source coordinate v - o lies in the box, read from the cone-minor
+// first-step tables (ft points at the axis block, rows = the axis'
+// dimension + 2, n = number of cones, c = this lane's cone).  Out-of-range
+// arguments clamp to the ends of the table, which yields empty ranges and
+// keeps a and b monotone in v.
 __device__ __forceinline__ void vrange_t(int v, int lo, int hi, int step,
                                          const int32_t* __restrict__ ft,
                                          int rows, int n, int c, int& a,
