@@ -184,6 +184,7 @@ class BatchEngine:
         self.pipeline = max(1, int(pipeline))
         self._resident = {}
         self._gather_streams = {}
+        self._default = None
         self._lock = threading.Lock()
 
     # -- resident phantoms -------------------------------------------------
@@ -210,12 +211,32 @@ class BatchEngine:
 
     # -- one batch ---------------------------------------------------------
     def _companion(self, s):
-        """The gather stream paired with TERMA stream `s`."""
+        """The gather stream paired with TERMA stream `s` (created once per
+        stream; close() releases them)."""
         with self._lock:
             g = self._gather_streams.get(s.handle)
             if g is None:
                 g = self._gather_streams[s.handle] = Stream()
             return g
+
+    def _default_stream(self):
+        with self._lock:
+            if self._default is None:
+                self._default = Stream()
+            return self._default
+
+    def close(self):
+        """Releases the engine's own streams (their gather workspaces with
+        them).  Batches still in flight must be collected first."""
+        with self._lock:
+            streams = list(self._gather_streams.values())
+            if self._default is not None:
+                streams.append(self._default)
+            self._gather_streams = {}
+            self._default = None
+        for st in streams:
+            st.sync()
+            st.close()
 
     def launch(self, requests, stream=None):
         """Queue TERMA and the gathers for `requests`; returns a BatchHandle
@@ -229,7 +250,7 @@ class BatchEngine:
         VALU for TERMA, the L1 data path for the gather).  pipeline = 1
         queues all TERMA and then one gather launch on `stream`.  Results
         are bitwise the same either way."""
-        s = stream or Stream()
+        s = stream or self._default_stream()
         if self.pipeline > 1 and not self.group_beams:
             return self._launch_pipelined(requests, s)
         eng = self.engine
