@@ -298,7 +298,7 @@ def run_batched(args, eng, steps=5):
         t0 = time.perf_counter()
         e0 = Event().record(s)
         h = be.launch([req], s)
-        h.event.sync()   # the batch's last gather (on its gather stream)
+        h.event.sync()
         times.append((time.perf_counter() - t0) * 1e3)
         gpu.append(e0.elapsed_ms(h.event))
         if i == steps - 1:
@@ -319,8 +319,7 @@ def run_batched(args, eng, steps=5):
     nb = len(case.beams)
     r = {"workload": "64 MLC beams (gantry U[0,360), seed 2024, 60 leaf "
                      "pairs x 5 mm, apertures 2-20 cm) on the 128^3 "
-                     "heterogeneous phantom, 96 cones; TERMA and the "
-                     "gathers pipelined in 4 parts on two streams",
+                     "heterogeneous phantom, 96 cones, one gather launch",
          "gather_tile": list(list(tile.values())[0]),
          "ms_per_batch": ms, "gpu_ms_per_batch": float(np.median(gpu)),
          "beams_per_s": nb / (ms * 1e-3),
@@ -390,7 +389,7 @@ def run_uneven(args, eng):
     for i in range(max(1, min(args.steps, 5))):
         t0 = time.perf_counter()
         h = be.launch(reqs, s)
-        Event().record(s).sync()
+        h.event.sync()
         times.append((time.perf_counter() - t0) * 1e3)
         if i == 0:
             doses = be.collect(h)

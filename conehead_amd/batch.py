@@ -2,12 +2,10 @@
 
 A DoseRequest is one phantom (densities on its dose grid), a list of beams
 and a cone set.  A batch of requests runs as one TERMA launch per beam and
-gather launches over the convolution jobs of the batch (jobs of different
-grids, voxel sizes and cone sets share a launch; ch_convolve_batch): by
-default the jobs are cut into parts whose gathers run on a second stream
-while the TERMA of the next part runs, or (pipeline=1) one launch after all
-TERMA.  With beam groups, beams of a request whose support boxes nearly
-coincide are convolved in interleaved groups (ch_convolve_groups).
+one gather launch over every convolution job of the batch (jobs of
+different grids, voxel sizes and cone sets share it; ch_convolve_batch),
+plus one launch for the beams convolved in interleaved groups (beams of a
+request whose support boxes nearly coincide; ch_convolve_groups).
 Two output modes:
   per_beam=True   one dose grid per beam (each beam is its own gather job);
   per_beam=False  the plan dose of all beams: their TERMA grids are summed
@@ -176,14 +174,11 @@ class BatchEngine:
     beams that share a support box (arc control points a few degrees
     apart), not for beams at unrelated gantry angles."""
 
-    def __init__(self, engine=None, pool=None, group_beams=False,
-                 pipeline=4):
+    def __init__(self, engine=None, pool=None, group_beams=False):
         self.engine = engine or E.Engine()
         self.pool = pool or DevicePool()
         self.group_beams = group_beams
-        self.pipeline = max(1, int(pipeline))
         self._resident = {}
-        self._gather_streams = {}
         self._default = None
         self._lock = threading.Lock()
 
@@ -210,15 +205,6 @@ class BatchEngine:
             self.pool.put(d)
 
     # -- one batch ---------------------------------------------------------
-    def _companion(self, s):
-        """The gather stream paired with TERMA stream `s` (created once per
-        stream; close() releases them)."""
-        with self._lock:
-            g = self._gather_streams.get(s.handle)
-            if g is None:
-                g = self._gather_streams[s.handle] = Stream()
-            return g
-
     def _default_stream(self):
         with self._lock:
             if self._default is None:
@@ -226,33 +212,19 @@ class BatchEngine:
             return self._default
 
     def close(self):
-        """Releases the engine's own streams (their gather workspaces with
-        them).  Batches still in flight must be collected first."""
+        """Releases the engine's default stream (its gather workspace with
+        it).  Batches still in flight on it must be collected first."""
         with self._lock:
-            streams = list(self._gather_streams.values())
-            if self._default is not None:
-                streams.append(self._default)
-            self._gather_streams = {}
-            self._default = None
-        for st in streams:
+            st, self._default = self._default, None
+        if st is not None:
             st.sync()
             st.close()
 
     def launch(self, requests, stream=None):
-        """Queue TERMA and the gathers for `requests`; returns a BatchHandle
-        (outputs valid once handle.done()).
-
-        With pipeline = k > 1 (and no beam groups) the jobs are cut into k
-        parts: TERMA runs on `stream` part after part, and each part's
-        gather (one ch_convolve_batch launch) runs on a companion stream as
-        soon as its TERMA is done, so the TERMA of later parts overlaps the
-        gathers of earlier ones (the two are bound by different units: FP64
-        VALU for TERMA, the L1 data path for the gather).  pipeline = 1
-        queues all TERMA and then one gather launch on `stream`.  Results
-        are bitwise the same either way."""
+        """Queue TERMA and one gather launch for `requests` on `stream` (the
+        engine's default stream if None); returns a BatchHandle (outputs
+        valid once handle.done())."""
         s = stream or self._default_stream()
-        if self.pipeline > 1 and not self.group_beams:
-            return self._launch_pipelined(requests, s)
         eng = self.engine
         jobs, groups, outputs, temps, pending = [], [], [], [], []
         for req in requests:
@@ -295,49 +267,6 @@ class BatchEngine:
         E.convolve_groups(groups, stream=s)
         ev = Event().record(s)
         return BatchHandle(list(requests), outputs, temps, ev, s)
-
-    def _launch_pipelined(self, requests, s):
-        eng = self.engine
-        g = self._companion(s)
-        units, outputs, temps = [], [], []
-        for req in requests:
-            d_rho, own = self.densities(req, s)
-            if own:
-                temps.append(d_rho)
-            plan = eng.plan(req.shape, req.spacing, req.cones)
-            n_jobs = req.n_jobs
-            t_bufs = [self.pool.get(req.shape) for _ in range(n_jobs)]
-            boxes = [self.pool.get((8,), np.int32) for _ in range(n_jobs)]
-            outs = [self.pool.get(req.shape) for _ in range(n_jobs)]
-            per_job = [[] for _ in range(n_jobs)]
-            for b, (src, blk, st) in enumerate(req.beams):
-                j = b if req.per_beam else 0
-                bs = E.beam_struct(src, st, eng.aperture(blk, s), d_rho,
-                                   t_bufs[j], boxes[j])
-                bs.accumulate = 0 if (req.per_beam or b == 0) else 1
-                per_job[j].append(bs)
-            for j in range(n_jobs):
-                units.append((req, per_job[j],
-                              (plan, t_bufs[j], outs[j], boxes[j], False)))
-            temps.extend(t_bufs)
-            temps.extend(boxes)
-            outputs.append(outs)
-        k = min(self.pipeline, len(units))
-        events = []
-        for part in range(k):
-            chunk = units[part * len(units) // k:
-                          (part + 1) * len(units) // k]
-            for req, beams, _ in chunk:
-                E.terma(beams, req.shape, req.origin, req.spacing, stream=s,
-                        check=False)
-            ev = Event().record(s)
-            events.append(ev)
-            g.wait(ev)
-            E.convolve_batch([job for _, _, job in chunk], stream=g)
-        done = Event().record(g)
-        h = BatchHandle(list(requests), outputs, temps, done, s)
-        h.events = events   # kept alive until the batch is collected
-        return h
 
     def collect(self, handle, download=True):
         """Wait for a batch; returns per request a float32 array
