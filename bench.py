@@ -130,7 +130,7 @@ def run_hetero(args, eng):
     box = d_box.download(stream=s)
     if args.tile == "auto":
         tile, _ = autotune.tune(plan, box=tuple(int(b) for b in box),
-                                stream=s, use_cache=False)
+                                stream=s, use_cache=False, terma=d_t)
     else:
         tile = tuple(int(v) for v in args.tile.split("x"))
         plan.set_tile(*tile)
@@ -354,8 +354,8 @@ def uneven_requests(per_beam=True):
 
 
 def tune_requests(eng, be, reqs, s):
-    """Autotune the tile of every distinct plan on its first beam's TERMA
-    support (cached per plan key)."""
+    """Autotune the tile of every distinct plan on its first beam's TERMA,
+    with the batched gather the requests will run in."""
     from conehead_amd import autotune
     from conehead_amd import engine as E
     from conehead_amd.device import DeviceArray
@@ -372,7 +372,8 @@ def tune_requests(eng, be, reqs, s):
                                box)], r.shape, r.origin, r.spacing, stream=s)
         b = box.download(stream=s)[:6]
         seen[plan.handle] = autotune.tune(plan, box=tuple(int(v) for v in b),
-                                          stream=s, use_cache=False)[0]
+                                          stream=s, use_cache=False,
+                                          terma=t, batch=True)[0]
     return seen
 
 

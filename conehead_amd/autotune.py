@@ -65,14 +65,19 @@ def _save_cache(d):
 
 
 def tune(plan, box=None, candidates=CANDIDATES, reps=3, stream=None,
-         use_cache=True):
+         use_cache=True, terma=None, batch=False):
     """Time every candidate tile on `plan` and keep the fastest.
 
     box: (lo0, lo1, lo2, hi0, hi1, hi2) support of the TERMA to tune for
-    (default: the whole grid).  Returns (best_tile, {tile: ms})."""
+    (default: the whole grid); terma: a device TERMA grid to tune on (by
+    default the box is filled with random values); batch: time the batched
+    gather (ch_convolve_batch, which uses its own workgroup size) instead of
+    the single-grid one.  Returns (best_tile, {tile: ms})."""
     key = _shape_key(plan)
     if box is not None:
         key += "/" + ",".join(str(int(b)) for b in box)
+    if batch:
+        key += "/batch"
     if use_cache:
         with _lock:
             cached = _load_cache().get(key)
@@ -83,25 +88,35 @@ def tune(plan, box=None, candidates=CANDIDATES, reps=3, stream=None,
             return tile, {tuple(map(int, k.split("x"))): v
                           for k, v in cached["ms"].items()}
     shape = plan.shape
-    rng = np.random.default_rng(0)
-    t = np.zeros(shape, np.float32)
     b = box or (0, 0, 0, shape[0] - 1, shape[1] - 1, shape[2] - 1)
-    t[b[0]:b[3] + 1, b[1]:b[4] + 1, b[2]:b[5] + 1] = rng.random(
-        (b[3] - b[0] + 1, b[4] - b[1] + 1, b[5] - b[2] + 1),
-        dtype=np.float32)
     s = stream or Stream()
-    d_t = DeviceArray.from_host(t)
+    if terma is not None:
+        d_t = terma
+    else:
+        rng = np.random.default_rng(0)
+        t = np.zeros(shape, np.float32)
+        t[b[0]:b[3] + 1, b[1]:b[4] + 1, b[2]:b[5] + 1] = rng.random(
+            (b[3] - b[0] + 1, b[4] - b[1] + 1, b[5] - b[2] + 1),
+            dtype=np.float32)
+        d_t = DeviceArray.from_host(t)
     d_d = DeviceArray(shape, np.float32)
     d_box = DeviceArray.from_host(np.asarray(b, np.int32))
-    from .engine import convolve
+    from .engine import convolve, convolve_batch
+
+    def run():
+        if batch:
+            convolve_batch([(plan, d_t, d_d, d_box, False)], stream=s)
+        else:
+            convolve(plan, d_t, d_d, d_box, stream=s)
+
     e0, e1 = Event(), Event()
     times = {}
     for tile in candidates:
         plan.set_tile(*tile)
-        convolve(plan, d_t, d_d, d_box, stream=s)  # warm-up
+        run()  # warm-up
         e0.record(s)
         for _ in range(reps):
-            convolve(plan, d_t, d_d, d_box, stream=s)
+            run()
         e1.record(s)
         e1.sync()
         times[tile] = e0.elapsed_ms(e1) / reps
