@@ -133,6 +133,7 @@ SIGNATURES = {
                                      ctypes.POINTER(c_size)]),
     "ch_memcpy_h2d": (ctypes.c_int32, [c_vp, c_vp, c_size, c_vp]),
     "ch_memcpy_d2h": (ctypes.c_int32, [c_vp, c_vp, c_size, c_vp]),
+    "ch_memcpy_d2h_async": (ctypes.c_int32, [c_vp, c_vp, c_size, c_vp]),
     "ch_memcpy_d2d": (ctypes.c_int32, [c_vp, c_vp, c_size, c_vp]),
     "ch_zero": (ctypes.c_int32, [c_vp, c_size, c_vp]),
     "ch_host_alloc": (ctypes.c_int32, [c_vpp, c_size]),

@@ -23,7 +23,7 @@ import numpy as np
 
 from . import _lib
 from . import engine as E
-from .device import DeviceArray, Event, Stream, _h
+from .device import DeviceArray, Event, HostPool, Stream, _h
 
 PAD_Y = 32   # kYPad of the padded TERMA copy (ch_internal.h)
 PAD_Z = 80   # kZPad
@@ -111,12 +111,14 @@ class DevicePool:
 class BatchHandle:
     """A launched batch: its completion event and device outputs."""
 
-    def __init__(self, requests, outputs, temps, event, stream):
+    def __init__(self, requests, outputs, temps, event, stream,
+                 staged=None):
         self.requests = requests
         self.outputs = outputs     # per request: list of dose DeviceArrays
         self.temps = temps         # buffers to return to the pool
         self.event = event
         self.stream = stream
+        self.staged = staged       # per request: pinned HostBuffer, or None
 
     def done(self):
         return self.event.done()
@@ -174,9 +176,11 @@ class BatchEngine:
     beams that share a support box (arc control points a few degrees
     apart), not for beams at unrelated gantry angles."""
 
-    def __init__(self, engine=None, pool=None, group_beams=False):
+    def __init__(self, engine=None, pool=None, group_beams=False,
+                 host_pool=None):
         self.engine = engine or E.Engine()
         self.pool = pool or DevicePool()
+        self.host_pool = host_pool or HostPool()
         self.group_beams = group_beams
         self._resident = {}
         self._default = None
@@ -220,10 +224,13 @@ class BatchEngine:
             st.sync()
             st.close()
 
-    def launch(self, requests, stream=None):
+    def launch(self, requests, stream=None, stage=False):
         """Queue TERMA and one gather launch for `requests` on `stream` (the
         engine's default stream if None); returns a BatchHandle (outputs
-        valid once handle.done())."""
+        valid once handle.done()).  stage=True also queues the copies of
+        the doses into pinned host buffers behind the gather, so they run
+        on the DMA engine while other streams compute and collect() hands
+        them out without a copy of its own."""
         s = stream or self._default_stream()
         eng = self.engine
         jobs, groups, outputs, temps, pending = [], [], [], [], []
@@ -265,25 +272,45 @@ class BatchEngine:
                                           for j in part], False))
         E.convolve_batch(jobs, stream=s)
         E.convolve_groups(groups, stream=s)
+        staged = None
+        if stage:
+            staged = []
+            for req, outs in zip(requests, outputs):
+                per = 4 * req.n_voxels
+                hb = self.host_pool.get(per * len(outs))
+                for i, d in enumerate(outs):
+                    d.download_async(hb, i * per, stream=s)
+                staged.append(hb)
         ev = Event().record(s)
-        return BatchHandle(list(requests), outputs, temps, ev, s)
+        return BatchHandle(list(requests), outputs, temps, ev, s, staged)
 
     def collect(self, handle, download=True):
         """Wait for a batch; returns per request a float32 array
         [n_jobs][X][Y][Z] (or the device arrays if download=False) and
-        returns the batch's temporaries to the pool."""
+        returns the batch's temporaries to the pool.  Arrays of a staged
+        batch view pinned buffers, which return to the host pool when the
+        arrays are dropped."""
         handle.event.sync()
         from . import _lib
         _lib.check(_lib.lib().ch_terma_check(_h(handle.stream)))
         results = []
-        for req, outs in zip(handle.requests, handle.outputs):
-            if download:
+        staged = handle.staged or [None] * len(handle.requests)
+        handle.staged = None
+        for req, outs, hb in zip(handle.requests, handle.outputs, staged):
+            if download and hb is not None:
+                results.append(self.host_pool.as_array(
+                    hb, (len(outs),) + req.shape))
+                for d in outs:
+                    self.pool.put(d)
+            elif download:
                 arr = np.empty((len(outs),) + req.shape, np.float32)
                 for i, d in enumerate(outs):
                     d.download(arr[i], stream=handle.stream)
                     self.pool.put(d)
                 results.append(arr)
             else:
+                if hb is not None:
+                    self.host_pool.put(hb)
                 results.append(outs)
         for t in handle.temps:
             self.pool.put(t)

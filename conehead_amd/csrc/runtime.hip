@@ -289,6 +289,17 @@ ch_status ch_memcpy_d2h(void* h_dst, const void* d_src, size_t bytes,
     return CH_OK;
 }
 
+ch_status ch_memcpy_d2h_async(void* h_dst, const void* d_src, size_t bytes,
+                              void* stream)
+{
+    if (bytes == 0)
+        return CH_OK;
+    CH_REQUIRE(h_dst && d_src, "ch_memcpy_d2h_async: null pointer");
+    CH_HIP(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost,
+                          (hipStream_t)stream));
+    return CH_OK;
+}
+
 ch_status ch_memcpy_d2d(void* d_dst, const void* d_src, size_t bytes,
                         void* stream)
 {

@@ -3,6 +3,7 @@
 The library allocates HBM itself (ch_malloc); numpy arrays are the host side.
 """
 import ctypes
+import threading
 
 import numpy as np
 
@@ -133,6 +134,16 @@ class DeviceArray:
                                             self.nbytes, _h(stream)))
         return out
 
+    def download_async(self, host, offset=0, stream=None):
+        """Queues a copy into the pinned HostBuffer `host` at byte `offset`;
+        the data is there once an event recorded after it has completed."""
+        if offset < 0 or offset + self.nbytes > host.nbytes:
+            raise ValueError("download_async: %d bytes at %d into %d" %
+                             (self.nbytes, offset, host.nbytes))
+        _lib.check(_lib.lib().ch_memcpy_d2h_async(host.ptr + offset,
+                                                  self.ptr, self.nbytes,
+                                                  _h(stream)))
+
     def zero(self, stream=None):
         _lib.check(_lib.lib().ch_zero(self.ptr, self.nbytes, _h(stream)))
         return self
@@ -162,3 +173,88 @@ def mem_info():
 
 def device_sync():
     _lib.check(_lib.lib().ch_device_sync())
+
+
+class HostBuffer:
+    """Pinned host memory (ch_host_alloc): the target of asynchronous
+    device-to-host copies."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        _lib.check(_lib.lib().ch_host_alloc(ctypes.byref(p), self.nbytes))
+        self.ptr = p.value
+
+    def free(self):
+        if self.ptr:
+            _lib.lib().ch_host_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class _PinnedView:
+    """Owner of a HostBuffer while numpy arrays view it; gives the buffer
+    back to its pool when the last array is dropped."""
+
+    def __init__(self, buf, pool, shape, dtype):
+        self._buf = buf
+        self._pool = pool
+        dt = np.dtype(dtype)
+        self.__array_interface__ = {
+            "shape": tuple(shape), "typestr": dt.str,
+            "data": (buf.ptr, False), "version": 3}
+
+    def __del__(self):
+        try:
+            self._pool.put(self._buf)
+        except Exception:
+            pass
+
+
+class HostPool:
+    """Reusable pinned host buffers keyed by size.  as_array() hands a buffer
+    out as a numpy array that owns it: the buffer returns to the pool when
+    that array (and every view of it) is gone, so results are delivered
+    without a host-side copy."""
+
+    def __init__(self, max_free_bytes=4 << 30):
+        self._free = {}
+        self._bytes = 0
+        self._max = max_free_bytes
+        self._lock = threading.RLock()
+
+    def get(self, nbytes):
+        with self._lock:
+            lst = self._free.get(int(nbytes))
+            if lst:
+                b = lst.pop()
+                self._bytes -= b.nbytes
+                return b
+        return HostBuffer(nbytes)
+
+    def put(self, buf):
+        with self._lock:
+            if self._bytes + buf.nbytes > self._max:
+                buf.free()
+                return
+            self._free.setdefault(buf.nbytes, []).append(buf)
+            self._bytes += buf.nbytes
+
+    def as_array(self, buf, shape, dtype=np.float32):
+        if int(np.prod(shape)) * np.dtype(dtype).itemsize > buf.nbytes:
+            raise ValueError("as_array: %s %s does not fit %d bytes" %
+                             (shape, dtype, buf.nbytes))
+        return np.asarray(_PinnedView(buf, self, shape, dtype))
+
+    def clear(self):
+        with self._lock:
+            for lst in self._free.values():
+                for b in lst:
+                    b.free()
+            self._free.clear()
+            self._bytes = 0

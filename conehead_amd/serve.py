@@ -67,7 +67,7 @@ class DoseServer:
     """Request loop around a BatchEngine."""
 
     def __init__(self, batch_engine=None, hbm_budget=None, max_inflight=2,
-                 max_batch_jobs=64):
+                 max_batch_jobs=64, download=True):
         from .batch import BatchEngine
         from .device import Stream, mem_info
         self.be = batch_engine or BatchEngine()
@@ -76,6 +76,9 @@ class DoseServer:
             hbm_budget = int(free * 0.8)
         self.queue = AdmissionQueue(hbm_budget, max_batch_jobs)
         self.max_inflight = max_inflight
+        # download=False delivers each result as its device dose grids
+        # (the caller returns them with batch_engine.release([grids]))
+        self.download = download
         self.streams = [Stream() for _ in range(max_inflight)]
         self._free_streams = list(self.streams)
         self._inflight = []
@@ -116,7 +119,8 @@ class DoseServer:
         for handle, items, stream in self._inflight:
             if handle.done():
                 try:
-                    results = self.be.collect(handle)
+                    results = self.be.collect(handle,
+                                              download=self.download)
                     t = time.perf_counter()
                     for (req, fut, t0), res in zip(items, results):
                         self.records.append((t0, t, len(req.beams),
@@ -141,7 +145,8 @@ class DoseServer:
                 return
             stream = self._free_streams.pop()
             try:
-                handle = self.be.launch([it[0] for it in items], stream)
+                handle = self.be.launch([it[0] for it in items], stream,
+                                        stage=self.download)
             except Exception as e:
                 for _, fut, _ in items:
                     fut.set_exception(e)

@@ -42,6 +42,11 @@ ch_status ch_memcpy_h2d(void* d_dst, const void* h_src, size_t bytes,
                         void* stream);
 ch_status ch_memcpy_d2h(void* h_dst, const void* d_src, size_t bytes,
                         void* stream);
+/* Queues a device-to-host copy and returns at once.  With a pinned h_dst
+ * (ch_host_alloc) the copy runs on the DMA engine in stream order; the
+ * data is there once later work on the stream (an event) has completed. */
+ch_status ch_memcpy_d2h_async(void* h_dst, const void* d_src, size_t bytes,
+                              void* stream);
 /* Device-to-device copy by a kernel (16-byte aligned pointers). */
 ch_status ch_memcpy_d2d(void* d_dst, const void* d_src, size_t bytes,
                         void* stream);

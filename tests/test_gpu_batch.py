@@ -128,3 +128,28 @@ def test_batch_vs_oracle_chain(be, oracle):
         err = np.abs(dose[b] - ref).max() / ref.max()
         assert err < 1e-4, (b, err)
 
+
+
+def test_staged_results(be):
+    """Doses staged into pinned buffers equal the plain download; a result
+    still held keeps its buffer while later batches run, and a dropped one
+    returns its buffer for reuse."""
+    cases = small_cases()
+    reqs = [DoseRequest.from_case(c) for c in cases]
+    s = Stream()
+    plain = be.run(reqs, s)
+    staged = be.collect(be.launch(reqs, s, stage=True))
+    for a, b in zip(staged, plain):
+        assert np.array_equal(a, b)
+    held = staged[0]
+    keep = held.copy()
+    del staged
+    again = be.collect(be.launch(reqs[::-1], s, stage=True))[::-1]
+    for a, b in zip(again, plain):
+        assert np.array_equal(a, b)
+    assert np.array_equal(held, keep)
+    ptr = again[1].__array_interface__["data"][0]
+    del again
+    reused = be.collect(be.launch([reqs[1]], s, stage=True))[0]
+    assert reused.__array_interface__["data"][0] == ptr
+    assert np.array_equal(reused, plain[1])

@@ -98,3 +98,31 @@ def test_group_beams_cost_model():
     parts = group_beams([(0, 0, 0, 9, 9, 9)] * 6)
     assert sorted(len(p) for p in parts) in ([1, 1, 4], [3, 3])
     assert sorted(i for p in parts for i in p) == list(range(6))
+
+
+def test_pinned_view_ownership():
+    """A HostPool array owns its buffer: views keep it out of the pool, and
+    it returns when the last of them is dropped."""
+    import gc
+    from conehead_amd.device import HostPool
+
+    class Buf:                  # host memory standing in for a pinned one
+        def __init__(self, n):
+            self.mem = np.zeros(n // 4, np.float32)
+            self.ptr = self.mem.ctypes.data
+            self.nbytes = n
+
+    pool = HostPool()
+    b = Buf(64)
+    a = pool.as_array(b, (2, 8))
+    a[1, 3] = 5.0
+    assert b.mem[11] == 5.0
+    v = a[1]
+    del a
+    gc.collect()
+    assert pool._bytes == 0
+    del v
+    gc.collect()
+    assert pool._bytes == 64 and pool.get(64) is b
+    with pytest.raises(ValueError):
+        pool.as_array(b, (4, 8))
