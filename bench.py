@@ -102,6 +102,48 @@ def oracle_beam(case, b, threads):
     return dose, out["terma"], updates, time.perf_counter() - c0
 
 
+def oracle_beam_one_thread(case, b, terma, stride=16):
+    """The oracle on one thread, as the reference runs (GIL held, single
+    threaded): the full TERMA, then the convolution over every stride-th
+    T!=0 source voxel, scaled to the whole beam by the ratio of dose += T*k
+    executions.  Returns (seconds per beam, sample seconds, description)."""
+    from conehead_amd import engine as E
+    from oracle import ref
+    src, blk, st = case.beams[b]
+    e, w, mu = E.spectrum(st)
+    rb = ref.make_beam(src.position, src.v_y, src.transform, src.sad, st, e,
+                       w, mu)
+    c0 = time.perf_counter()
+    ref.terma(rb, case.shape, case.origin, case.spacing, case.densities,
+              blk.block_values, n_threads=1)
+    t_terma = time.perf_counter() - c0
+    nz = np.flatnonzero(terma)
+    pick = nz[::stride]
+    ts = np.zeros(terma.shape)
+    ts.flat[pick] = terma.flat[pick]
+    dose = np.zeros(case.shape)
+    c0 = time.perf_counter()
+    u_sample = ref.convolve_dirs(ts, dose, case.spacing, case.cones.dirs,
+                                 case.cones.rows, case.cones.cum, n_threads=1)
+    t_conv = time.perf_counter() - c0
+    return t_terma, t_conv, u_sample, len(pick), len(nz)
+
+
+def host_info():
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(),
+            "cpus_usable": len(os.sched_getaffinity(0)),
+            "cpu_model": model}
+
+
 def err_over_dmax(got, ref):
     return float(np.abs(got - ref).max() / ref.max())
 
@@ -241,6 +283,17 @@ def run_hetero(args, eng):
                          (int(np.count_nonzero(ref_t)), n_cones, updates,
                           cpu_s),
                "dda_step_updates": int(updates)}
+        log("cpu baseline: oracle on 1 thread, sampled")
+        t_terma, t_conv, u_s, n_pick, n_src = oracle_beam_one_thread(
+            case, 0, ref_t)
+        per_beam = t_terma + t_conv * updates / u_s
+        cpu["one_thread"] = {
+            "value": 1.0 / per_beam, "unit": "beams/s", "cores": 1,
+            "kind": "port",
+            "sample": "full FP64 TERMA (%.2f s) + convolve_c scatter over "
+                      "every 16th T!=0 source voxel (%d of %d, %d updates, "
+                      "%.2f s), scaled to the beam's %d updates" %
+                      (t_terma, n_pick, n_src, u_s, t_conv, updates)}
 
     g_ms = float(gather_ms.mean())
     achieved = BYTES_PER_VOXEL_CONE * nv * n_cones / (g_ms * 1e-3) / 1e9
@@ -519,6 +572,7 @@ def main():
             line["batched"] = bt
             line["plain"] = pl
     line["device"] = _lib.device_info()
+    line["host"] = host_info()
     print(json.dumps(line))
     return 0
 
