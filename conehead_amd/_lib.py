@@ -166,6 +166,14 @@ SIGNATURES = {
     "ch_convolve": (ctypes.c_int32, [c_vp, c_vp, c_vp, c_vp, ctypes.c_int32,
                                      c_vp]),
     "ch_convolve_f64": (ctypes.c_int32, [c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "ch_plan_spectrum": (ctypes.c_int32, [c_vp, c_vp]),
+    "ch_plan_spectrum_info": (ctypes.c_int32, [c_vp, c_i32p, c_i32p,
+                                               c_i64p]),
+    "ch_convolve_fft": (ctypes.c_int32, [c_vp, c_vp, c_vp, ctypes.c_int32,
+                                         c_vp]),
+    "ch_convolve_fft_batch": (ctypes.c_int32, [ctypes.c_int32,
+                                               ctypes.POINTER(ConvJob),
+                                               c_vp]),
     "ch_convolve_batch": (ctypes.c_int32, [ctypes.c_int32,
                                            ctypes.POINTER(ConvJob), c_vp]),
     "ch_set_launch_events": (ctypes.c_int32, [c_vp, c_vp]),

@@ -174,14 +174,22 @@ class BatchEngine:
     the boxes coincide), chosen by group_beams(); every other beam is its
     own job of one ch_convolve_batch launch.  Off by default: it pays for
     beams that share a support box (arc control points a few degrees
-    apart), not for beams at unrelated gantry angles."""
+    apart), not for beams at unrelated gantry angles.
+
+    method: "gather" (the direct stencil sum, ch_convolve_batch) or "fft"
+    (the same convolution through zero-padded FFTs, ch_convolve_fft_batch:
+    five HBM passes per beam whatever the cone count; beams on one plan
+    share each launch)."""
 
     def __init__(self, engine=None, pool=None, group_beams=False,
-                 host_pool=None):
+                 host_pool=None, method="gather"):
+        if method not in ("gather", "fft"):
+            raise ValueError("method must be 'gather' or 'fft'")
         self.engine = engine or E.Engine()
         self.pool = pool or DevicePool()
         self.host_pool = host_pool or HostPool()
-        self.group_beams = group_beams
+        self.group_beams = group_beams and method == "gather"
+        self.method = method
         self._resident = {}
         self._default = None
         self._lock = threading.Lock()
@@ -270,8 +278,11 @@ class BatchEngine:
                 else:
                     groups.append((plan, [(t_bufs[j], outs[j], boxes[j])
                                           for j in part], False))
-        E.convolve_batch(jobs, stream=s)
-        E.convolve_groups(groups, stream=s)
+        if self.method == "fft":
+            E.convolve_fft_batch(jobs, stream=s)
+        else:
+            E.convolve_batch(jobs, stream=s)
+            E.convolve_groups(groups, stream=s)
         staged = None
         if stage:
             staged = []

@@ -26,7 +26,8 @@ COMMON = [
 EXACT = {"dda.hip", "plan.hip", "terma.hip", "raster.hip", "scatter.hip",
          "resample.hip", "physics.hip"}
 SOURCES = ["runtime.hip", "dda.hip", "plan.hip", "ccc.hip", "terma.hip",
-           "raster.hip", "scatter.hip", "resample.hip", "physics.hip"]
+           "raster.hip", "scatter.hip", "resample.hip", "physics.hip",
+           "spectral.hip"]
 
 
 def hipcc():

@@ -163,7 +163,8 @@ struct LaunchTimer {
 enum ScratchSlot {
     kScratchBoxPartials = 0,  // per-block partial support boxes
     kScratchStatus = 1,       // TERMA geometry flag (one int32)
-    kScratchSlots = 2
+    kScratchSpectrum = 2,     // complex work grids of the spectral passes
+    kScratchSlots = 3
 };
 ch_status stream_scratch(hipStream_t s, int slot, size_t bytes, void** out);
 // The slot's buffer if this stream has one, else nullptr (no allocation).
@@ -171,6 +172,7 @@ void* stream_scratch_peek(hipStream_t s, int slot);
 // Free a stream's scratch and gather workspace (before it is destroyed).
 void release_stream_scratch(hipStream_t s);
 void release_gather_workspace(hipStream_t s);
+void release_spectral_tables(hipStream_t s);
 // Pads two grids of plan p's shape into the stream's gather workspace
 // (the layout the gathers read) and returns the interior voxel (0, 0, 0)
 // of each copy.
@@ -286,4 +288,13 @@ struct ch_plan {
     int32_t* d_row_begin;
     int32_t n_rows;
     int32_t row_len;
+    // Spectral form (ch_plan_spectrum): frame lengths, the row pitch of the
+    // half spectra, K^ (Fx x Fy x pitch complex) and per-axis twiddles
+    // exp(-2 pi i q / F).  Null until built.
+    int32_t fft_n[3];
+    int32_t fft_kp;
+    float2* d_khat;
+    float2* d_tw[3];
 };
+// Frees a plan's spectral tables (the plan itself stays).
+void spectrum_free(ch_plan* p);

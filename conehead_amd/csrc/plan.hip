@@ -128,6 +128,7 @@ static void plan_free(ch_plan* p)
     hipFree(p->d_cum);
     hipFree(p->d_perm);
     hipFree(p->d_row_begin);
+    spectrum_free(p);
     delete p;
 }
 
@@ -244,6 +245,10 @@ ch_status ch_plan_create(const int32_t h_shape[3], const double h_vsize[3],
     p->d_row_begin = nullptr;
     p->n_rows = n_rows;
     p->row_len = row_len;
+    p->fft_n[0] = p->fft_n[1] = p->fft_n[2] = 0;
+    p->fft_kp = 0;
+    p->d_khat = nullptr;
+    p->d_tw[0] = p->d_tw[1] = p->d_tw[2] = nullptr;
 
     const size_t nt = (size_t)n_cones * p->tap_stride;
     const size_t nf = (size_t)n_cones * p->f_stride;

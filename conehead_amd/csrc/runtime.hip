@@ -366,6 +366,7 @@ ch_status ch_stream_destroy(void* stream)
     if (stream) {
         hipStream_t s = (hipStream_t)stream;
         release_gather_workspace(s);
+        release_spectral_tables(s);
         release_stream_scratch(s);
         CH_HIP(hipStreamDestroy(s));
     }

@@ -1,0 +1,1017 @@
+// Spectral form of the convolution: dose = T (*) K on zero-padded frames.
+//
+// convolve_c (conehead/convolve_c.pyx:77-192) adds, for every source voxel
+// s with T != 0 and every cone c, k_cm * T(s) to the voxel s + o_cm of the
+// cone's m-th DDA step.  The offsets o_cm and increments k_cm depend only
+// on the cone (the plan's tap tables), so the dose is ONE linear 3D
+// convolution of T with the kernel K(o) = sum over (c, m) with o_cm = o of
+// k_cm, taken with T zero outside the grid.  Every offset component lies in
+// (-n_a, n_a), so on a frame of F_a >= 2 n_a - 1 points per axis the
+// circular convolution equals the linear one on the grid:
+//   dose = IDFT( DFT(T zero-padded) * DFT(K wrapped) )[grid].
+// The plan's taps are merged into K in FP64 on the host; K's transform
+// (K^, FP32, scaled by 1 / (Fx Fy Fz)) is built once per plan on the GPU.
+//
+// Each beam then costs five HBM-bound passes over a complex grid of
+// X x Fy x Kp points (Kp >= Fz / 2 + 1), nothing of O(taps):
+//   z_fwd   real lines of T, two per complex FFT, split into half spectra;
+//   col     y transforms of the x < X slabs (rows y >= Y are zero);
+//   col     x transforms, times K^, inverse x transforms, rows x < X kept;
+//   col     inverse y transforms, rows y < Y kept;
+//   z_inv   two half spectra per complex inverse FFT into the dose lines.
+// The transforms are mixed-radix (2, 3, 4, 5, 8) Stockham FFTs in LDS, W
+// lines per workgroup; frame lengths are products of 2, 3 and 5.
+#include "ch_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <mutex>
+#include <unordered_map>
+#include <vector>
+
+namespace ch {
+namespace {
+
+constexpr int kFftThreads = 256;
+constexpr int kFftMax = 1024;      // longest transform
+constexpr int kFftLdsElems = 4096; // F * W per ping-pong buffer
+// Work grids one batched launch may use (beams beyond it take further
+// launches).
+constexpr size_t kSpectralWorkBytes = (size_t)4 << 30;
+
+struct FftLen {
+    int32_t n;
+    int32_t nfac;
+    int32_t fac[12];
+};
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b)
+{
+    return make_float2(a.x + b.x, a.y + b.y);
+}
+__device__ __forceinline__ float2 csub(float2 a, float2 b)
+{
+    return make_float2(a.x - b.x, a.y - b.y);
+}
+__device__ __forceinline__ float2 cscale(float2 a, float s)
+{
+    return make_float2(a.x * s, a.y * s);
+}
+// -i * a for the forward transform, +i * a for the inverse
+template <bool INV>
+__device__ __forceinline__ float2 rot(float2 a)
+{
+    return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft2(float2& a, float2& b)
+{
+    const float2 t = a;
+    a = cadd(t, b);
+    b = csub(t, b);
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft4(float2& a, float2& b, float2& c,
+                                     float2& d)
+{
+    const float2 t0 = cadd(a, c), t1 = csub(a, c);
+    const float2 t2 = cadd(b, d), t3 = rot<INV>(csub(b, d));
+    a = cadd(t0, t2);
+    c = csub(t0, t2);
+    b = cadd(t1, t3);
+    d = csub(t1, t3);
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft(float2 (&v)[2])
+{
+    dft2<INV>(v[0], v[1]);
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft(float2 (&v)[4])
+{
+    dft4<INV>(v[0], v[1], v[2], v[3]);
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft(float2 (&v)[8])
+{
+    // even and odd halves, then X[k] = E[k] + w^k O[k], X[k+4] = E - w^k O
+    float2 e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+    float2 o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+    dft4<INV>(e0, e1, e2, e3);
+    dft4<INV>(o0, o1, o2, o3);
+    const float h = 0.70710678118654752f;
+    // w^1 = (h, -h) forward, (h, h) inverse; w^3 = (-h, -h) / (-h, h)
+    const float2 w1o = INV ? make_float2(h * (o1.x - o1.y), h * (o1.x + o1.y))
+                           : make_float2(h * (o1.x + o1.y), h * (o1.y - o1.x));
+    const float2 w2o = rot<INV>(o2);
+    const float2 w3o = INV
+                           ? make_float2(-h * (o3.x + o3.y), h * (o3.x - o3.y))
+                           : make_float2(h * (o3.y - o3.x), -h * (o3.x + o3.y));
+    v[0] = cadd(e0, o0);
+    v[4] = csub(e0, o0);
+    v[1] = cadd(e1, w1o);
+    v[5] = csub(e1, w1o);
+    v[2] = cadd(e2, w2o);
+    v[6] = csub(e2, w2o);
+    v[3] = cadd(e3, w3o);
+    v[7] = csub(e3, w3o);
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft(float2 (&v)[3])
+{
+    const float s3 = 0.86602540378443865f;  // sin(pi / 3)
+    const float2 t1 = cadd(v[1], v[2]);
+    const float2 t2 = csub(v[0], cscale(t1, 0.5f));
+    const float2 t3 = rot<INV>(cscale(csub(v[1], v[2]), s3));
+    v[0] = cadd(v[0], t1);
+    v[1] = cadd(t2, t3);
+    v[2] = csub(t2, t3);
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft(float2 (&v)[5])
+{
+    const float c1 = 0.30901699437494742f;   // cos(2 pi / 5)
+    const float c2 = -0.80901699437494742f;  // cos(4 pi / 5)
+    const float s1 = 0.95105651629515357f;   // sin(2 pi / 5)
+    const float s2 = 0.58778525229247313f;   // sin(4 pi / 5)
+    const float2 a = v[0];
+    const float2 sbe = cadd(v[1], v[4]), dbe = csub(v[1], v[4]);
+    const float2 scd = cadd(v[2], v[3]), dcd = csub(v[2], v[3]);
+    const float2 a1 = cadd(a, cadd(cscale(sbe, c1), cscale(scd, c2)));
+    const float2 a2 = cadd(a, cadd(cscale(sbe, c2), cscale(scd, c1)));
+    const float2 b1 = rot<INV>(cadd(cscale(dbe, s1), cscale(dcd, s2)));
+    const float2 b2 = rot<INV>(csub(cscale(dbe, s2), cscale(dcd, s1)));
+    v[0] = cadd(a, cadd(sbe, scd));
+    v[1] = cadd(a1, b1);
+    v[4] = csub(a1, b1);
+    v[2] = cadd(a2, b2);
+    v[3] = csub(a2, b2);
+}
+
+// One Stockham stage of radix R over W lines of length F in LDS: element
+// (n, w) of a buffer at n * sn + w * sw.  Sub-transforms of length Ns are
+// done; this stage makes them Ns * R long, in natural order.  WFAST maps
+// consecutive threads to consecutive lines (w), else to consecutive j.
+template <int R, bool INV, bool WFAST>
+__device__ __forceinline__ void fft_stage(const float2* __restrict__ src,
+                                          float2* __restrict__ dst, int F,
+                                          int W, int Ns, int sn, int sw,
+                                          const float2* __restrict__ tw)
+{
+    const int nb = F / R;
+    const int twstep = F / (Ns * R);
+    const int total = nb * W;
+    for (int it = threadIdx.x; it < total; it += blockDim.x) {
+        int j, w;
+        if (WFAST) {
+            w = it % W;
+            j = it / W;
+        } else {
+            j = it % nb;
+            w = it / nb;
+        }
+        const int jm = j % Ns;
+        float2 v[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            v[r] = src[(j + r * nb) * sn + w * sw];
+        if (Ns > 1) {
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                float2 t = tw[r * jm * twstep];
+                if (INV)
+                    t.y = -t.y;
+                v[r] = cmul(v[r], t);
+            }
+        }
+        dft<INV>(v);
+        const int d0 = (j - jm) * R + jm;
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            dst[(d0 + r * Ns) * sn + w * sw] = v[r];
+    }
+}
+
+// Full transform of W lines held in `a`; returns the buffer (a or b) that
+// holds the result.  Ends with a barrier.
+template <bool INV, bool WFAST>
+__device__ float2* lds_fft(float2* a, float2* b, const float2* tw,
+                           const FftLen& L, int W, int sn, int sw)
+{
+    int Ns = 1;
+    for (int s = 0; s < L.nfac; ++s) {
+        const int R = L.fac[s];
+        switch (R) {
+        case 8:
+            fft_stage<8, INV, WFAST>(a, b, L.n, W, Ns, sn, sw, tw);
+            break;
+        case 4:
+            fft_stage<4, INV, WFAST>(a, b, L.n, W, Ns, sn, sw, tw);
+            break;
+        case 2:
+            fft_stage<2, INV, WFAST>(a, b, L.n, W, Ns, sn, sw, tw);
+            break;
+        case 3:
+            fft_stage<3, INV, WFAST>(a, b, L.n, W, Ns, sn, sw, tw);
+            break;
+        default:
+            fft_stage<5, INV, WFAST>(a, b, L.n, W, Ns, sn, sw, tw);
+            break;
+        }
+        __syncthreads();
+        Ns *= R;
+        float2* t = a;
+        a = b;
+        b = t;
+    }
+    return a;
+}
+
+__device__ __forceinline__ void load_twiddles(float2* s_tw, const float2* g,
+                                              int F)
+{
+    for (int i = threadIdx.x; i < F; i += blockDim.x)
+        s_tw[i] = g[i];
+}
+
+// The beams of one launch: TERMA and dose grids from device tables (a
+// batch) or one pointer each (a single beam); work grid b at a_beam * b.
+// Workgroups take blockIdx.x = group * n_beams + beam, so consecutive
+// workgroups are the same columns of different beams (K^ chunks are read
+// once per launch from HBM and then from L2).
+struct BeamIO {
+    const float* t;
+    float* d;
+    const float* const* t_tab;
+    float* const* d_tab;
+    const int32_t* acc_tab;
+    int32_t accumulate;
+    int32_t n_beams;
+    int64_t a_beam;
+};
+
+__device__ __forceinline__ int beam_of_block(const BeamIO& io)
+{
+    return (int)(blockIdx.x % (unsigned)io.n_beams);
+}
+__device__ __forceinline__ int group_of_block(const BeamIO& io)
+{
+    return (int)(blockIdx.x / (unsigned)io.n_beams);
+}
+
+// Forward z transforms.  Lines (x, y) of T (X x Y x Z, z fastest) are taken
+// in pairs (y even, y + 1) as one complex line T_y + i T_{y+1}, zero-padded
+// to F; after the FFT the two half spectra (k = 0 .. F/2) are separated,
+// X_y[k] = (C[k] + conj C[F-k]) / 2 and X_{y+1}[k] = (C[k] - conj C[F-k]) /
+// 2i, and written to A at row (x * a_rows + y) with pitch kp.  Workgroup:
+// W pairs, line-major in LDS with pitch F + 1.
+template <int W>
+__global__ __launch_bounds__(kFftThreads) void z_fwd_kernel(
+    BeamIO io, float2* __restrict__ A, int32_t X, int32_t Y, int32_t Z,
+    int32_t a_rows, int32_t kp, FftLen L, const float2* __restrict__ g_tw)
+{
+    extern __shared__ float2 sm[];
+    const int F = L.n, LP = F + 1;
+    float2* s_tw = sm;
+    float2* s_a = sm + F;
+    float2* s_b = s_a + W * LP;
+    load_twiddles(s_tw, g_tw, F);
+    const int beam = beam_of_block(io);
+    const float* __restrict__ T = io.t_tab ? io.t_tab[beam] : io.t;
+    A += beam * io.a_beam;
+    const int Yp = (Y + 1) / 2;
+    const int64_t npairs = (int64_t)X * Yp;
+    const int64_t p0 = (int64_t)group_of_block(io) * W;
+    for (int it = threadIdx.x; it < W * F; it += blockDim.x) {
+        const int w = it / F, n = it - w * F;
+        const int64_t p = p0 + w;
+        float2 v = make_float2(0.0f, 0.0f);
+        if (p < npairs && n < Z) {
+            const int64_t x = p / Yp;
+            const int y = 2 * (int)(p - x * Yp);
+            const float* row = T + (x * Y + y) * (int64_t)Z;
+            v.x = row[n];
+            if (y + 1 < Y)
+                v.y = row[Z + n];
+        }
+        s_a[w * LP + n] = v;
+    }
+    __syncthreads();
+    const float2* r = lds_fft<false, false>(s_a, s_b, s_tw, L, W, 1, LP);
+    const int K = F / 2 + 1;
+    for (int it = threadIdx.x; it < W * K; it += blockDim.x) {
+        const int w = it / K, k = it - w * K;
+        const int64_t p = p0 + w;
+        if (p >= npairs)
+            continue;
+        const int64_t x = p / Yp;
+        const int y = 2 * (int)(p - x * Yp);
+        const float2 c = r[w * LP + k];
+        const float2 cm = r[w * LP + (k == 0 ? 0 : F - k)];
+        float2* out = A + (x * a_rows + y) * (int64_t)kp;
+        out[k] = make_float2(0.5f * (c.x + cm.x), 0.5f * (c.y - cm.y));
+        if (y + 1 < Y)
+            out[kp + k] =
+                make_float2(0.5f * (c.y + cm.y), -0.5f * (c.x - cm.x));
+    }
+}
+
+// Inverse z transforms: the half spectra of lines y and y + 1 (Hermitian
+// extension conj X[F-k] above F/2, imaginary parts of the k = 0 bin and,
+// for even F, the k = F/2 bin dropped) as C = X_y + i X_{y+1}; the inverse
+// FFT gives line y in the
+// real and line y + 1 in the imaginary part; z < Z is stored or added.
+template <int W>
+__global__ __launch_bounds__(kFftThreads) void z_inv_kernel(
+    BeamIO io, const float2* __restrict__ A, int32_t X, int32_t Y,
+    int32_t Z, int32_t a_rows, int32_t kp, FftLen L,
+    const float2* __restrict__ g_tw)
+{
+    extern __shared__ float2 sm[];
+    const int F = L.n, LP = F + 1;
+    float2* s_tw = sm;
+    float2* s_a = sm + F;
+    float2* s_b = s_a + W * LP;
+    load_twiddles(s_tw, g_tw, F);
+    const int beam = beam_of_block(io);
+    float* __restrict__ D = io.d_tab ? io.d_tab[beam] : io.d;
+    const int accumulate = io.acc_tab ? io.acc_tab[beam] : io.accumulate;
+    A += beam * io.a_beam;
+    const int Yp = (Y + 1) / 2;
+    const int64_t npairs = (int64_t)X * Yp;
+    const int64_t p0 = (int64_t)group_of_block(io) * W;
+    const int half = F / 2;
+    for (int it = threadIdx.x; it < W * F; it += blockDim.x) {
+        const int w = it / F, k = it - w * F;
+        const int64_t p = p0 + w;
+        float2 c = make_float2(0.0f, 0.0f);
+        if (p < npairs) {
+            const int64_t x = p / Yp;
+            const int y = 2 * (int)(p - x * Yp);
+            const float2* row = A + (x * a_rows + y) * (int64_t)kp;
+            const bool lo = k <= half;
+            const int kk = lo ? k : F - k;
+            float2 a = row[kk];
+            float2 b = (y + 1 < Y) ? row[kp + kk] : make_float2(0.0f, 0.0f);
+            if (!lo) {
+                a.y = -a.y;
+                b.y = -b.y;
+            }
+            if (k == 0 || 2 * k == F) {
+                a.y = 0.0f;
+                b.y = 0.0f;
+            }
+            c = make_float2(a.x - b.y, a.y + b.x);
+        }
+        s_a[w * LP + k] = c;
+    }
+    __syncthreads();
+    const float2* r = lds_fft<true, false>(s_a, s_b, s_tw, L, W, 1, LP);
+    for (int it = threadIdx.x; it < W * Z; it += blockDim.x) {
+        const int w = it / Z, n = it - w * Z;
+        const int64_t p = p0 + w;
+        if (p >= npairs)
+            continue;
+        const int64_t x = p / Yp;
+        const int y = 2 * (int)(p - x * Yp);
+        float* out = D + (x * Y + y) * (int64_t)Z;
+        const float2 c = r[w * LP + n];
+        if (accumulate) {
+            out[n] += c.x;
+            if (y + 1 < Y)
+                out[Z + n] += c.y;
+        } else {
+            out[n] = c.x;
+            if (y + 1 < Y)
+                out[Z + n] = c.y;
+        }
+    }
+}
+
+enum ColMode { kColFwd = 0, kColInv = 1, kColConv = 2 };
+
+// Transforms along a strided axis: W adjacent columns (kz .. kz + W - 1) of
+// the rows n = 0 .. F-1 at base + n * row_stride, base = A + blockIdx.y *
+// outer_stride + W * blockIdx.x.  Rows n >= n_in read as zero; rows n <
+// n_out are written back in place.  kColConv multiplies by K^ (same
+// layout, its own strides) between the forward and the inverse transform.
+template <int W, int MODE>
+__global__ __launch_bounds__(kFftThreads) void col_kernel(
+    BeamIO io, float2* __restrict__ A, int64_t row_stride,
+    int64_t outer_stride, int32_t n_cols, int32_t n_in, int32_t n_out,
+    FftLen L, const float2* __restrict__ g_tw,
+    const float2* __restrict__ khat, int64_t k_row_stride,
+    int64_t k_outer_stride)
+{
+    extern __shared__ float2 sm[];
+    const int F = L.n;
+    float2* s_tw = sm;
+    float2* s_a = sm + F;
+    float2* s_b = s_a + F * W;
+    load_twiddles(s_tw, g_tw, F);
+    const int k0 = group_of_block(io) * W;
+    const int nv = min(W, n_cols - k0);
+    float2* base = A + beam_of_block(io) * io.a_beam +
+                   blockIdx.y * outer_stride + k0;
+    for (int it = threadIdx.x; it < F * W; it += blockDim.x) {
+        const int n = it / W, w = it % W;
+        float2 v = make_float2(0.0f, 0.0f);
+        if (n < n_in && w < nv)
+            v = base[n * row_stride + w];
+        s_a[n * W + w] = v;
+    }
+    __syncthreads();
+    float2* r;
+    if (MODE == kColInv)
+        r = lds_fft<true, true>(s_a, s_b, s_tw, L, W, W, 1);
+    else
+        r = lds_fft<false, true>(s_a, s_b, s_tw, L, W, W, 1);
+    if (MODE == kColConv) {
+        const float2* kb = khat + blockIdx.y * k_outer_stride + k0;
+        for (int it = threadIdx.x; it < F * W; it += blockDim.x) {
+            const int n = it / W, w = it % W;
+            if (w < nv)
+                r[n * W + w] = cmul(r[n * W + w], kb[n * k_row_stride + w]);
+        }
+        __syncthreads();
+        float2* o = (r == s_a) ? s_b : s_a;
+        r = lds_fft<true, true>(r, o, s_tw, L, W, W, 1);
+    }
+    for (int it = threadIdx.x; it < n_out * W; it += blockDim.x) {
+        const int n = it / W, w = it % W;
+        if (w < nv)
+            base[n * row_stride + w] = r[n * W + w];
+    }
+}
+
+__global__ void scatter_frame_kernel(float* __restrict__ frame,
+                                     const int64_t* __restrict__ idx,
+                                     const float* __restrict__ val, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        frame[idx[i]] = val[i];
+}
+
+// ---------------------------------------------------------------- host --
+
+int lines_per_group(int F)
+{
+    int w = 16;
+    while (w > 4 && F * w > kFftLdsElems)
+        w /= 2;
+    return w;
+}
+
+size_t z_lds_bytes(int F, int W)
+{
+    return (size_t)(F + 2 * W * (F + 1)) * sizeof(float2);
+}
+
+size_t col_lds_bytes(int F, int W)
+{
+    return (size_t)(F + 2 * F * W) * sizeof(float2);
+}
+
+template <typename K>
+ch_status allow_lds(K kernel, size_t bytes)
+{
+    CH_HIP(hipFuncSetAttribute((const void*)kernel,
+                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)bytes));
+    return CH_OK;
+}
+
+bool fft_factor(int n, FftLen& L)
+{
+    L.n = n;
+    L.nfac = 0;
+    int m = n;
+    const int order[5] = {8, 4, 2, 3, 5};
+    for (int r : order)
+        while (m % r == 0 && L.nfac < 12) {
+            L.fac[L.nfac++] = r;
+            m /= r;
+        }
+    return m == 1;
+}
+
+FftLen plan_len(const ch_plan* p, int a)
+{
+    FftLen L;
+    fft_factor(p->fft_n[a], L);
+    return L;
+}
+
+template <int W>
+ch_status launch_z_fwd(const BeamIO& io, float2* A, int X, int Y, int Z,
+                       int a_rows, int kp, const FftLen& L, const float2* tw,
+                       hipStream_t s)
+{
+    const size_t lds = z_lds_bytes(L.n, W);
+    ch_status st = allow_lds(z_fwd_kernel<W>, lds);
+    if (st != CH_OK)
+        return st;
+    const int64_t groups = ((int64_t)X * ((Y + 1) / 2) + W - 1) / W;
+    hipLaunchKernelGGL(z_fwd_kernel<W>,
+                       dim3((unsigned)(groups * io.n_beams)),
+                       dim3(kFftThreads), lds, s, io, A, X, Y, Z, a_rows, kp,
+                       L, tw);
+    CH_HIP(hipGetLastError());
+    return CH_OK;
+}
+
+template <int W>
+ch_status launch_z_inv(const BeamIO& io, const float2* A, int X, int Y,
+                       int Z, int a_rows, int kp, const FftLen& L,
+                       const float2* tw, hipStream_t s)
+{
+    const size_t lds = z_lds_bytes(L.n, W);
+    ch_status st = allow_lds(z_inv_kernel<W>, lds);
+    if (st != CH_OK)
+        return st;
+    const int64_t groups = ((int64_t)X * ((Y + 1) / 2) + W - 1) / W;
+    hipLaunchKernelGGL(z_inv_kernel<W>,
+                       dim3((unsigned)(groups * io.n_beams)),
+                       dim3(kFftThreads), lds, s, io, A, X, Y, Z, a_rows, kp,
+                       L, tw);
+    CH_HIP(hipGetLastError());
+    return CH_OK;
+}
+
+template <int W, int MODE>
+ch_status launch_col(const BeamIO& io, float2* A, int64_t row_stride,
+                     int64_t outer_stride, int n_outer, int n_cols, int n_in,
+                     int n_out, const FftLen& L, const float2* tw,
+                     const float2* khat, int64_t k_row_stride,
+                     int64_t k_outer_stride, hipStream_t s)
+{
+    const size_t lds = col_lds_bytes(L.n, W);
+    ch_status st = allow_lds(col_kernel<W, MODE>, lds);
+    if (st != CH_OK)
+        return st;
+    const int64_t groups = (n_cols + W - 1) / W;
+    hipLaunchKernelGGL((col_kernel<W, MODE>),
+                       dim3((unsigned)(groups * io.n_beams), (unsigned)n_outer),
+                       dim3(kFftThreads), lds, s, io, A, row_stride,
+                       outer_stride, n_cols, n_in, n_out, L, tw, khat,
+                       k_row_stride, k_outer_stride);
+    CH_HIP(hipGetLastError());
+    return CH_OK;
+}
+
+// W is chosen from the transform length at run time.
+ch_status z_fwd(const BeamIO& io, float2* A, int X, int Y, int Z, int a_rows,
+                int kp, const FftLen& L, const float2* tw, hipStream_t s)
+{
+    switch (lines_per_group(L.n)) {
+    case 16:
+        return launch_z_fwd<16>(io, A, X, Y, Z, a_rows, kp, L, tw, s);
+    case 8:
+        return launch_z_fwd<8>(io, A, X, Y, Z, a_rows, kp, L, tw, s);
+    default:
+        return launch_z_fwd<4>(io, A, X, Y, Z, a_rows, kp, L, tw, s);
+    }
+}
+
+ch_status z_inv(const BeamIO& io, const float2* A, int X, int Y, int Z,
+                int a_rows, int kp, const FftLen& L, const float2* tw,
+                hipStream_t s)
+{
+    switch (lines_per_group(L.n)) {
+    case 16:
+        return launch_z_inv<16>(io, A, X, Y, Z, a_rows, kp, L, tw, s);
+    case 8:
+        return launch_z_inv<8>(io, A, X, Y, Z, a_rows, kp, L, tw, s);
+    default:
+        return launch_z_inv<4>(io, A, X, Y, Z, a_rows, kp, L, tw, s);
+    }
+}
+
+template <int MODE>
+ch_status col(const BeamIO& io, float2* A, int64_t row_stride,
+              int64_t outer_stride, int n_outer, int n_cols, int n_in,
+              int n_out, const FftLen& L, const float2* tw,
+              const float2* khat, int64_t k_row_stride,
+              int64_t k_outer_stride, hipStream_t s)
+{
+    switch (lines_per_group(L.n)) {
+    case 16:
+        return launch_col<16, MODE>(io, A, row_stride, outer_stride, n_outer,
+                                    n_cols, n_in, n_out, L, tw, khat,
+                                    k_row_stride, k_outer_stride, s);
+    case 8:
+        return launch_col<8, MODE>(io, A, row_stride, outer_stride, n_outer,
+                                   n_cols, n_in, n_out, L, tw, khat,
+                                   k_row_stride, k_outer_stride, s);
+    default:
+        return launch_col<4, MODE>(io, A, row_stride, outer_stride, n_outer,
+                                   n_cols, n_in, n_out, L, tw, khat,
+                                   k_row_stride, k_outer_stride, s);
+    }
+}
+
+BeamIO single_beam(const float* t, float* d, int32_t accumulate)
+{
+    BeamIO io;
+    io.t = t;
+    io.d = d;
+    io.t_tab = nullptr;
+    io.d_tab = nullptr;
+    io.acc_tab = nullptr;
+    io.accumulate = accumulate;
+    io.n_beams = 1;
+    io.a_beam = 0;
+    return io;
+}
+
+// Smallest n' >= n whose prime factors are 2, 3 and 5.
+int32_t smooth_at_least(int32_t n)
+{
+    for (int32_t m = std::max(n, 1);; ++m) {
+        int32_t k = m;
+        for (int32_t q : {2, 3, 5})
+            while (k % q == 0)
+                k /= q;
+        if (k == 1)
+            return m;
+    }
+}
+
+// Complex points of one beam's work grid.
+size_t beam_points(const ch_plan* p)
+{
+    return (size_t)p->shape[0] * p->fft_n[1] * p->fft_kp;
+}
+
+// The five passes over the beams of `io` (all on plan p), work grids in A.
+ch_status spectral_passes(const ch_plan* p, BeamIO io, float2* A,
+                          hipStream_t s)
+{
+    const int X = p->shape[0], Y = p->shape[1], Z = p->shape[2];
+    const int Fy = p->fft_n[1];
+    const int kz = p->fft_n[2] / 2 + 1, kp = p->fft_kp;
+    const FftLen Lx = plan_len(p, 0), Ly = plan_len(p, 1), Lz = plan_len(p, 2);
+    io.a_beam = (int64_t)beam_points(p);
+    const int64_t slab = (int64_t)Fy * kp;
+    ch_status st = z_fwd(io, A, X, Y, Z, Fy, kp, Lz, p->d_tw[2], s);
+    if (st == CH_OK)
+        st = col<kColFwd>(io, A, kp, slab, X, kz, Y, Fy, Ly, p->d_tw[1],
+                          nullptr, 0, 0, s);
+    if (st == CH_OK)
+        st = col<kColConv>(io, A, slab, kp, Fy, kz, X, X, Lx, p->d_tw[0],
+                           p->d_khat, slab, kp, s);
+    if (st == CH_OK)
+        st = col<kColInv>(io, A, kp, slab, X, kz, Fy, Y, Ly, p->d_tw[1],
+                          nullptr, 0, 0, s);
+    if (st == CH_OK)
+        st = z_inv(io, A, X, Y, Z, Fy, kp, Lz, p->d_tw[2], s);
+    return st;
+}
+
+// Per-stream device table of a batch (TERMA pointers, dose pointers,
+// accumulate flags), staged through pinned memory that is rewritten only
+// after the event of its last copy.
+struct BeamTable {
+    hipStream_t stream;
+    void* h;
+    void* d;
+    size_t bytes;
+    hipEvent_t copied;
+};
+std::vector<BeamTable*> g_tabs;
+std::mutex g_tab_mu;
+
+ch_status beam_table(hipStream_t s, size_t bytes, BeamTable** out)
+{
+    BeamTable* t = nullptr;
+    for (auto* x : g_tabs)
+        if (x->stream == s)
+            t = x;
+    if (!t) {
+        t = new BeamTable();
+        t->stream = s;
+        t->h = nullptr;
+        t->d = nullptr;
+        t->bytes = 0;
+        CH_HIP(hipEventCreateWithFlags(&t->copied, hipEventDisableTiming));
+        g_tabs.push_back(t);
+    }
+    CH_HIP(hipEventSynchronize(t->copied));
+    if (t->bytes < bytes) {
+        CH_HIP(hipStreamSynchronize(s));
+        CH_HIP(hipFree(t->d));
+        CH_HIP(hipHostFree(t->h));
+        t->d = nullptr;
+        t->h = nullptr;
+        t->bytes = 0;
+        const size_t cap = std::max(bytes, (size_t)4096);
+        CH_HIP(hipMalloc(&t->d, cap));
+        CH_HIP(hipHostMalloc(&t->h, cap, hipHostMallocDefault));
+        t->bytes = cap;
+    }
+    *out = t;
+    return CH_OK;
+}
+
+}  // namespace
+}  // namespace ch
+
+using namespace ch;
+
+void ch::release_spectral_tables(hipStream_t s)
+{
+    std::lock_guard<std::mutex> lock(g_tab_mu);
+    for (size_t i = 0; i < g_tabs.size(); ++i) {
+        BeamTable* t = g_tabs[i];
+        if (t->stream != s)
+            continue;
+        (void)hipStreamSynchronize(s);
+        (void)hipFree(t->d);
+        (void)hipHostFree(t->h);
+        (void)hipEventDestroy(t->copied);
+        delete t;
+        g_tabs.erase(g_tabs.begin() + i);
+        return;
+    }
+}
+
+void spectrum_free(ch_plan* p)
+{
+    hipFree(p->d_khat);
+    for (int a = 0; a < 3; ++a)
+        hipFree(p->d_tw[a]);
+    p->d_khat = nullptr;
+    p->d_tw[0] = p->d_tw[1] = p->d_tw[2] = nullptr;
+}
+
+extern "C" {
+
+ch_status ch_plan_spectrum(ch_plan* plan, void* stream)
+{
+    CH_REQUIRE(plan, "ch_plan_spectrum: null plan");
+    if (plan->d_khat)
+        return CH_OK;
+    CH_REQUIRE(plan->semantics != CH_SEM_RADIOLOGICAL,
+               "ch_plan_spectrum: radiological plans are not a convolution "
+               "(the kernel distance depends on the densities)");
+    hipStream_t s = (hipStream_t)stream;
+    CH_REQUIRE(!capturing(s), "ch_plan_spectrum allocates: build the "
+                              "spectrum before capturing");
+    int32_t F[3];
+    for (int a = 0; a < 3; ++a) {
+        F[a] = smooth_at_least(2 * plan->shape[a] - 1);
+        CH_REQUIRE(F[a] <= kFftMax,
+                   "ch_plan_spectrum: axis %d of %d voxels needs a %d-point "
+                   "transform (at most %d)", a, plan->shape[a], F[a], kFftMax);
+    }
+    const int32_t kz = F[2] / 2 + 1;
+    const int32_t kp = (kz + 15) / 16 * 16;
+
+    // K on the frame: every cone's taps at (o mod F), merged in FP64 and
+    // scaled by 1 / (Fx Fy Fz), the inverse transform's normalisation
+    const int nc = plan->n_cones;
+    const size_t nt = (size_t)nc * plan->tap_stride;
+    std::vector<int32_t> off(nt * 3);
+    std::vector<double> k64(nt);
+    CH_HIP(hipMemcpy(off.data(), plan->d_off, nt * 3 * sizeof(int32_t),
+                     hipMemcpyDeviceToHost));
+    CH_HIP(hipMemcpy(k64.data(), plan->d_k64, nt * sizeof(double),
+                     hipMemcpyDeviceToHost));
+    std::unordered_map<int64_t, double> acc;
+    for (int c = 0; c < nc; ++c) {
+        const ConeMeta& m = plan->h_meta[c];
+        for (int32_t t = 0; t < m.n_taps; ++t) {
+            const size_t i = (size_t)m.tap_begin + t;
+            int64_t q = 0;
+            for (int a = 0; a < 3; ++a) {
+                const int32_t o = off[3 * i + a];
+                q = q * F[a] + ((o % F[a]) + F[a]) % F[a];
+            }
+            acc[q] += k64[i];
+        }
+    }
+    std::vector<std::pair<int64_t, double>> items(acc.begin(), acc.end());
+    std::sort(items.begin(), items.end());
+    const double scale = 1.0 / ((double)F[0] * F[1] * F[2]);
+    std::vector<int64_t> h_idx(items.size());
+    std::vector<float> h_val(items.size());
+    for (size_t i = 0; i < items.size(); ++i) {
+        h_idx[i] = items[i].first;
+        h_val[i] = (float)(items[i].second * scale);
+    }
+
+    ch_status st = CH_OK;
+    float* frame = nullptr;
+    int64_t* d_idx = nullptr;
+    float* d_val = nullptr;
+    float2* khat = nullptr;
+    float2* tw[3] = {nullptr, nullptr, nullptr};
+    const size_t n_frame = (size_t)F[0] * F[1] * F[2];
+    const size_t n_khat = (size_t)F[0] * F[1] * kp;
+    hipError_t e = hipSuccess;
+#define CH_TRY(x)                                                           \
+    do {                                                                    \
+        if (e == hipSuccess)                                                \
+            e = (x);                                                        \
+    } while (0)
+    CH_TRY(hipMalloc((void**)&frame, n_frame * sizeof(float)));
+    CH_TRY(hipMalloc((void**)&khat, n_khat * sizeof(float2)));
+    CH_TRY(hipMalloc((void**)&d_idx, std::max<size_t>(1, h_idx.size()) *
+                                         sizeof(int64_t)));
+    CH_TRY(hipMalloc((void**)&d_val,
+                     std::max<size_t>(1, h_val.size()) * sizeof(float)));
+    for (int a = 0; a < 3; ++a) {
+        std::vector<float2> h(F[a]);
+        for (int32_t q = 0; q < F[a]; ++q) {
+            const double ang = 2.0 * M_PI * (double)q / (double)F[a];
+            h[q] = make_float2((float)std::cos(ang), (float)-std::sin(ang));
+        }
+        CH_TRY(hipMalloc((void**)&tw[a], F[a] * sizeof(float2)));
+        CH_TRY(hipMemcpy(tw[a], h.data(), F[a] * sizeof(float2),
+                         hipMemcpyHostToDevice));
+    }
+    CH_TRY(hipMemcpy(d_idx, h_idx.data(), h_idx.size() * sizeof(int64_t),
+                     hipMemcpyHostToDevice));
+    CH_TRY(hipMemcpy(d_val, h_val.data(), h_val.size() * sizeof(float),
+                     hipMemcpyHostToDevice));
+#undef CH_TRY
+    if (e != hipSuccess) {
+        set_error("ch_plan_spectrum: %s", hipGetErrorString(e));
+        st = CH_ERR_HIP;
+    }
+    if (st == CH_OK)
+        st = ch_zero(frame, n_frame * sizeof(float), stream);
+    if (st == CH_OK && !h_idx.empty()) {
+        const int64_t n = (int64_t)h_idx.size();
+        hipLaunchKernelGGL(scatter_frame_kernel,
+                           dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                           frame, (const int64_t*)d_idx, (const float*)d_val,
+                           n);
+        if (hipGetLastError() != hipSuccess) {
+            set_error("ch_plan_spectrum: scatter launch failed");
+            st = CH_ERR_HIP;
+        }
+    }
+    if (st == CH_OK) {
+        plan->fft_n[0] = F[0];
+        plan->fft_n[1] = F[1];
+        plan->fft_n[2] = F[2];
+        plan->fft_kp = kp;
+        plan->d_tw[0] = tw[0];
+        plan->d_tw[1] = tw[1];
+        plan->d_tw[2] = tw[2];
+        const FftLen Lx = plan_len(plan, 0), Ly = plan_len(plan, 1),
+                     Lz = plan_len(plan, 2);
+        const int64_t slab = (int64_t)F[1] * kp;
+        // the whole frame is the grid here: no zero rows, all rows kept
+        const BeamIO io = single_beam(frame, nullptr, 0);
+        st = z_fwd(io, khat, F[0], F[1], F[2], F[1], kp, Lz, tw[2], s);
+        if (st == CH_OK)
+            st = col<kColFwd>(io, khat, kp, slab, F[0], kz, F[1], F[1], Ly,
+                              tw[1], nullptr, 0, 0, s);
+        if (st == CH_OK)
+            st = col<kColFwd>(io, khat, slab, kp, F[1], kz, F[0], F[0], Lx,
+                              tw[0], nullptr, 0, 0, s);
+        if (st == CH_OK && hipStreamSynchronize(s) != hipSuccess) {
+            set_error("ch_plan_spectrum: %s",
+                      hipGetErrorString(hipGetLastError()));
+            st = CH_ERR_HIP;
+        }
+    }
+    hipFree(frame);
+    hipFree(d_idx);
+    hipFree(d_val);
+    if (st == CH_OK) {
+        plan->d_khat = khat;
+    } else {
+        hipFree(khat);
+        for (int a = 0; a < 3; ++a)
+            hipFree(tw[a]);
+        plan->d_tw[0] = plan->d_tw[1] = plan->d_tw[2] = nullptr;
+        plan->fft_n[0] = plan->fft_n[1] = plan->fft_n[2] = 0;
+    }
+    return st;
+}
+
+ch_status ch_plan_spectrum_info(const ch_plan* plan, int32_t* frame3,
+                                int32_t* pitch, int64_t* work_bytes)
+{
+    CH_REQUIRE(plan, "ch_plan_spectrum_info: null plan");
+    CH_REQUIRE(plan->d_khat, "ch_plan_spectrum_info: no spectrum (call "
+                             "ch_plan_spectrum first)");
+    if (frame3)
+        for (int a = 0; a < 3; ++a)
+            frame3[a] = plan->fft_n[a];
+    if (pitch)
+        *pitch = plan->fft_kp;
+    if (work_bytes)
+        *work_bytes = (int64_t)(beam_points(plan) * sizeof(float2));
+    return CH_OK;
+}
+
+ch_status ch_convolve_fft(const ch_plan* plan, const float* d_terma,
+                          float* d_dose, int32_t accumulate, void* stream)
+{
+    CH_REQUIRE(plan && d_terma && d_dose, "ch_convolve_fft: null pointer");
+    CH_REQUIRE(plan->d_khat, "ch_convolve_fft: the plan has no spectrum "
+                             "(call ch_plan_spectrum first)");
+    hipStream_t s = (hipStream_t)stream;
+    void* work = nullptr;
+    ch_status st = stream_scratch(s, kScratchSpectrum,
+                                  beam_points(plan) * sizeof(float2), &work);
+    if (st != CH_OK)
+        return st;
+    LaunchTimer timer(s);
+    st = spectral_passes(plan, single_beam(d_terma, d_dose, accumulate),
+                         (float2*)work, s);
+    timer.stop();
+    return st;
+}
+
+ch_status ch_convolve_fft_batch(int32_t n_jobs, const ch_conv_job* h_jobs,
+                                void* stream)
+{
+    CH_REQUIRE(n_jobs >= 0 && (n_jobs == 0 || h_jobs),
+               "ch_convolve_fft_batch: bad job list");
+    hipStream_t s = (hipStream_t)stream;
+    CH_REQUIRE(!capturing(s), "ch_convolve_fft_batch stages a host table and "
+                              "cannot be captured; capture ch_convolve_fft");
+    for (int32_t i = 0; i < n_jobs; ++i) {
+        const ch_conv_job& j = h_jobs[i];
+        CH_REQUIRE(j.plan && j.d_terma && j.d_dose,
+                   "ch_convolve_fft_batch: job %d has a null pointer", i);
+        CH_REQUIRE(j.plan->d_khat,
+                   "ch_convolve_fft_batch: job %d's plan has no spectrum "
+                   "(call ch_plan_spectrum first)", i);
+    }
+    // jobs on one plan share launches, in order of first appearance
+    std::vector<const ch_plan*> plans;
+    std::vector<std::vector<int32_t>> members;
+    for (int32_t i = 0; i < n_jobs; ++i) {
+        size_t g = 0;
+        while (g < plans.size() && plans[g] != h_jobs[i].plan)
+            ++g;
+        if (g == plans.size()) {
+            plans.push_back(h_jobs[i].plan);
+            members.emplace_back();
+        }
+        members[g].push_back(i);
+    }
+    std::lock_guard<std::mutex> lock(g_tab_mu);
+    LaunchTimer timer(s);
+    for (size_t g = 0; g < plans.size(); ++g) {
+        const ch_plan* p = plans[g];
+        const size_t per = beam_points(p) * sizeof(float2);
+        const size_t cap = std::max<size_t>(1, kSpectralWorkBytes / per);
+        const std::vector<int32_t>& m = members[g];
+        for (size_t b0 = 0; b0 < m.size(); b0 += cap) {
+            const int32_t nb = (int32_t)std::min(cap, m.size() - b0);
+            void* work = nullptr;
+            ch_status st =
+                stream_scratch(s, kScratchSpectrum, per * nb, &work);
+            if (st != CH_OK)
+                return st;
+            BeamTable* t = nullptr;
+            const size_t ptrs = (size_t)nb * sizeof(void*);
+            st = beam_table(s, 2 * ptrs + (size_t)nb * sizeof(int32_t), &t);
+            if (st != CH_OK)
+                return st;
+            const float** ht = (const float**)t->h;
+            float** hd = (float**)((char*)t->h + ptrs);
+            int32_t* ha = (int32_t*)((char*)t->h + 2 * ptrs);
+            for (int32_t b = 0; b < nb; ++b) {
+                const ch_conv_job& j = h_jobs[m[b0 + b]];
+                ht[b] = j.d_terma;
+                hd[b] = j.d_dose;
+                ha[b] = j.accumulate ? 1 : 0;
+            }
+            CH_HIP(hipMemcpyAsync(t->d, t->h, 2 * ptrs + nb * sizeof(int32_t),
+                                  hipMemcpyHostToDevice, s));
+            CH_HIP(hipEventRecord(t->copied, s));
+            BeamIO io = single_beam(nullptr, nullptr, 0);
+            io.t_tab = (const float* const*)t->d;
+            io.d_tab = (float* const*)((char*)t->d + ptrs);
+            io.acc_tab = (const int32_t*)((char*)t->d + 2 * ptrs);
+            io.n_beams = nb;
+            st = spectral_passes(p, io, (float2*)work, s);
+            if (st != CH_OK)
+                return st;
+        }
+    }
+    timer.stop();
+    return CH_OK;
+}
+
+}  // extern "C"

@@ -76,6 +76,17 @@ class Plan:
             k.ctypes.data_as(_lib.c_f64p), cap, ctypes.byref(n)))
         return off[:n.value].copy(), k[:n.value].copy()
 
+    def spectrum(self, stream=None):
+        """Builds the kernel spectrum the FFT convolution needs (once);
+        returns (frame lengths, half-spectrum pitch, work bytes per beam)."""
+        _lib.check(_lib.lib().ch_plan_spectrum(self.handle, _h(stream)))
+        f = (ctypes.c_int32 * 3)()
+        pitch = ctypes.c_int32()
+        work = ctypes.c_int64()
+        _lib.check(_lib.lib().ch_plan_spectrum_info(
+            self.handle, f, ctypes.byref(pitch), ctypes.byref(work)))
+        return tuple(f), pitch.value, work.value
+
     def close(self):
         if self.handle:
             _lib.lib().ch_plan_destroy(self.handle)
@@ -111,6 +122,16 @@ def convolve(plan, d_terma, d_dose, d_box=None, accumulate=False,
                                       1 if accumulate else 0, _h(stream)))
 
 
+def convolve_fft(plan, d_terma, d_dose, accumulate=False, stream=None):
+    """dose (+)= the same convolution as convolve(), computed as one linear
+    convolution of T with the plan's summed kernel through zero-padded FFTs
+    (ch_convolve_fft).  Builds the plan's spectrum on first use."""
+    plan.spectrum(stream)
+    _lib.check(_lib.lib().ch_convolve_fft(plan.handle, d_terma.ptr,
+                                          d_dose.ptr, 1 if accumulate else 0,
+                                          _h(stream)))
+
+
 def convolve_rad(plan, d_terma, d_density, d_dose, d_box=None,
                  accumulate=False, stream=None):
     """Physics mode: dose (+)= convolution of terma with the kernel looked
@@ -143,6 +164,27 @@ def convolve_batch(jobs, stream=None):
         arr[i].d_box = box.ptr if box is not None else None
         arr[i].accumulate = 1 if acc else 0
     _lib.check(_lib.lib().ch_convolve_batch(n, arr, _h(stream)))
+
+
+def convolve_fft_batch(jobs, stream=None):
+    """convolve_fft over many (plan, terma, dose, box, accumulate) jobs: the
+    jobs of one plan share each pass's launch (ch_convolve_fft_batch); the
+    box is not used.  Builds missing plan spectra first."""
+    n = len(jobs)
+    if n == 0:
+        return
+    seen = set()
+    arr = (_lib.ConvJob * n)()
+    for i, (plan, t, d, box, acc) in enumerate(jobs):
+        if plan.handle not in seen:
+            plan.spectrum(stream)
+            seen.add(plan.handle)
+        arr[i].plan = plan.handle
+        arr[i].d_terma = t.ptr
+        arr[i].d_dose = d.ptr
+        arr[i].d_box = None
+        arr[i].accumulate = 1 if acc else 0
+    _lib.check(_lib.lib().ch_convolve_fft_batch(n, arr, _h(stream)))
 
 
 def convolve_groups(groups, stream=None):

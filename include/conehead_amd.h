@@ -173,6 +173,32 @@ typedef struct ch_conv_job {
 ch_status ch_convolve_batch(int32_t n_jobs, const ch_conv_job* h_jobs,
                             void* stream);
 
+/* ------------------------------------------------- spectral form ------ */
+
+/* Builds the plan's kernel spectrum: the taps of every cone summed into one
+ * kernel K on a frame of F_a >= 2 n_a - 1 points per axis (prime factors 2,
+ * 3, 5; F_a <= 1024), transformed once.  Idempotent.  Radiological plans
+ * are refused: their kernel distance depends on the densities.  No
+ * reference counterpart: it precomputes what ch_convolve_fft needs. */
+ch_status ch_plan_spectrum(ch_plan* plan, void* stream);
+/* Frame lengths, half-spectrum row pitch and the work-grid bytes one beam
+ * uses (held per stream) of a plan whose spectrum is built. */
+ch_status ch_plan_spectrum_info(const ch_plan* plan, int32_t* frame3,
+                                int32_t* pitch, int64_t* work_bytes);
+/* The same dose as ch_convolve (convolve_c, conehead/convolve_c.pyx:17-197,
+ * as dose[v] (+)= sum_c sum_m k_cm T[v - o_cm]) computed as one linear
+ * convolution of T with K through zero-padded FP32 FFTs: five HBM passes
+ * per beam whatever the number of cones or taps.  Needs ch_plan_spectrum;
+ * float dose; accumulate adds into d_dose. */
+ch_status ch_convolve_fft(const ch_plan* plan, const float* d_terma,
+                          float* d_dose, int32_t accumulate, void* stream);
+/* ch_convolve_fft for many jobs (the ch_conv_job list of
+ * ch_convolve_batch; d_box is not used): jobs on one plan share each pass's
+ * launch, up to 4 GiB of work grids per launch.  Dose buffers must be
+ * distinct within a call.  Stages a host table, so it cannot be captured. */
+ch_status ch_convolve_fft_batch(int32_t n_jobs, const ch_conv_job* h_jobs,
+                                void* stream);
+
 /* Up to CH_MAX_GROUP beams on the same plan convolved together: their
  * TERMA grids are interleaved so each tap is one 16-byte load for all of
  * them, and the tap walk is shared.  Tap ranges use the union of the

@@ -1,0 +1,174 @@
+"""Spectral (FFT) form of the convolution against the FP64 oracle.
+
+convolve_c is a fixed stencil per cone, so summed over cones the dose is one
+linear convolution of T with the plan's kernel K; ch_convolve_fft computes
+it through zero-padded FP32 FFTs.  Checks:
+  - the oracle's convolve_c (scatter form) on shapes whose frame lengths
+    need radices 2, 3, 4, 5 and 8, odd y (an unpaired z line), a one-voxel
+    z axis and both translation-invariant semantics;
+  - an impulse T = delta_s: the dose is the plan's summed tap table placed
+    at s (a closed form that a dropped tap, a wrong sign or a transposed
+    axis fails);
+  - accumulate mode, and radiological plans refused;
+  - the direct gather on the same plan.
+Tolerance: max |D - D_ref| / max(D_ref) <= 1e-5 (target 1e-3; FP32 FFTs
+land near 3e-7)."""
+import numpy as np
+import pytest
+
+from conehead_amd import engine as E
+from conehead_amd.cones import ConeSet, default_thetas
+from conehead_amd.device import DeviceArray
+from conehead_amd.kernel2 import PolyenergeticKernel, merged_kernel
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def rel_err(d, ref):
+    return float(np.abs(d - ref).max() / np.abs(ref).max())
+
+
+@pytest.fixture(scope="module")
+def kern():
+    return PolyenergeticKernel()
+
+
+def cone_set(kern, n_theta, n_phi, semantics="convolve_c"):
+    k = merged_kernel(kern, n_phi)
+    phis = np.array(k.keys(), float)
+    if semantics != "convolve_c":
+        # table rows instead of the kernel dict
+        k = np.stack([k.cumulative["%.2f" % p] for p in phis])
+    return ConeSet(default_thetas(n_theta), phis, k, semantics=semantics)
+
+
+def beam_like(shape, rng, frac=0.5):
+    T = np.zeros(shape)
+    sl = tuple(slice(int(s * (1 - frac) / 2), int(s * (1 + frac) / 2) + 1)
+               for s in shape)
+    T[sl] = 1.0 + rng.random(T[sl].shape)
+    return T
+
+
+def run_fft(plan, T, accumulate_onto=None):
+    d_t = DeviceArray.from_host(T.astype(np.float32))
+    if accumulate_onto is None:
+        d_d = DeviceArray(T.shape, np.float32)
+    else:
+        d_d = DeviceArray.from_host(accumulate_onto.astype(np.float32))
+    E.convolve_fft(plan, d_t, d_d, accumulate=accumulate_onto is not None)
+    return d_d.download()
+
+
+@pytest.mark.parametrize("shape,vs,semantics", [
+    ((24, 26, 22), (0.4, 0.4, 0.4), "convolve_c"),    # frames 48, 54, 45
+    ((17, 9, 31), (0.3, 0.5, 0.25), "convolve_c"),    # odd y: 36, 18, 64
+    ((20, 13, 1), (0.4, 0.4, 0.4), "convolve_c"),     # one z voxel: 40, 25, 1
+    ((30, 28, 40), (0.35, 0.3, 0.3), "cuda_dose"),    # 60, 60, 80
+])
+def test_matches_oracle(oracle, engine, kern, shape, vs, semantics):
+    rng = np.random.default_rng(sum(shape))
+    T = beam_like(shape, rng)
+    cs = cone_set(kern, 6, 4, semantics)
+    ref = np.zeros(shape)
+    if semantics == "convolve_c":
+        oracle.convolve_dirs(T, ref, vs, cs.dirs, cs.rows, cs.cum,
+                             n_threads=8)
+    else:
+        # the cuda_dose index |floor(100 t - 5)| clamped at 5995, geometric
+        oracle.convolve_rad(T, np.ones(shape, np.float32), ref, vs, cs.dirs,
+                            cs.rows, cs.cum, radiological=False, n_threads=8)
+    plan = engine.plan(shape, vs, cs)
+    frame, pitch, work = plan.spectrum()
+    assert all(f >= 2 * n - 1 for f, n in zip(frame, shape))
+    assert pitch >= frame[2] // 2 + 1
+    d = run_fft(plan, T)
+    assert rel_err(d, ref) < TOL
+
+
+def test_impulse_gives_the_tap_table(engine, kern):
+    """T = delta at s: dose(s + o) = sum of the increments of every tap at
+    offset o (over all cones), zero elsewhere."""
+    shape, vs = (21, 18, 25), (0.3, 0.3, 0.3)
+    cs = cone_set(kern, 6, 4)
+    plan = engine.plan(shape, vs, cs)
+    s = np.array([7, 11, 9])
+    T = np.zeros(shape)
+    T[tuple(s)] = 1.0
+    want = np.zeros(shape)
+    for c in range(cs.n_cones):
+        off, k = plan.cone_taps(c)
+        v = off + s
+        ok = np.all((v >= 0) & (v < np.array(shape)), axis=1)
+        np.add.at(want, tuple(v[ok].T), k[ok])
+    d = run_fft(plan, T)
+    assert rel_err(d, want) < TOL
+    far = want == 0
+    assert np.abs(d[far]).max() < 1e-5 * want.max()
+
+
+def test_accumulate_and_direct_gather(engine, kern):
+    rng = np.random.default_rng(5)
+    shape, vs = (40, 36, 44), (0.3, 0.35, 0.25)
+    cs = cone_set(kern, 12, 8)
+    plan = engine.plan(shape, vs, cs)
+    T = beam_like(shape, rng)
+    d_t = DeviceArray.from_host(T.astype(np.float32))
+    d_box = DeviceArray((6,), np.int32)
+    E.support_box(d_t, shape, d_box)
+    d_g = DeviceArray(shape, np.float32)
+    E.convolve(plan, d_t, d_g, d_box)
+    gather = d_g.download()
+    fft = run_fft(plan, T)
+    assert rel_err(fft, gather) < TOL
+    base = rng.random(shape).astype(np.float32)
+    both = run_fft(plan, T, accumulate_onto=base)
+    assert rel_err(both - base, gather) < TOL
+
+
+def test_radiological_plan_refused(engine, kern):
+    cs = cone_set(kern, 4, 4, "physics")
+    plan = engine.plan((16, 16, 16), (0.4, 0.4, 0.4), cs)
+    with pytest.raises(RuntimeError, match="radiological"):
+        plan.spectrum()
+
+
+def test_batch_equals_single_bitwise(engine, kern):
+    """Jobs on three plans (shared launches per plan, accumulate flags
+    mixed) give the same bits as one ch_convolve_fft call each."""
+    rng = np.random.default_rng(9)
+    jobs, want = [], []
+    specs = [((20, 30, 40), 4), ((33, 17, 29), 8), ((20, 30, 40), 4),
+             ((48, 40, 36), 6), ((33, 17, 29), 8)]
+    for i, (shape, nphi) in enumerate(specs):
+        plan = engine.plan(shape, (0.3, 0.25, 0.35), cone_set(kern, 4, nphi))
+        T = beam_like(shape, rng).astype(np.float32)
+        base = rng.random(shape).astype(np.float32)
+        acc = i % 2 == 1
+        want.append(run_fft(plan, T, accumulate_onto=base if acc else None))
+        d_t = DeviceArray.from_host(T)
+        d_d = DeviceArray.from_host(base)
+        jobs.append((plan, d_t, d_d, None, acc))
+    E.convolve_fft_batch(jobs)
+    for (_, _, d_d, _, _), w in zip(jobs, want):
+        assert np.array_equal(d_d.download(), w)
+
+
+def test_batch_engine_fft_method(engine):
+    from conehead_amd import workloads as W
+    from conehead_amd.batch import BatchEngine, DoseRequest
+    from conehead_amd.device import Stream
+    cases = W.uneven(n_phantoms=3, seed=11, max_beams=3, xy=(16, 36),
+                     z=(14, 40), spacing=(0.4, 0.8), n_theta=6,
+                     n_phis=(4, 6))
+    reqs = [DoseRequest.from_case(c) for c in cases]
+    s = Stream()
+    gather = BatchEngine(engine).run(reqs, s)
+    fft_be = BatchEngine(engine, method="fft")
+    fft = fft_be.run(reqs, s)
+    alone = [fft_be.run([r], s)[0] for r in reqs]
+    for g, f, a in zip(gather, fft, alone):
+        assert rel_err(f, g) < TOL
+        assert np.array_equal(f, a)
