@@ -56,6 +56,11 @@ def parse():
     p.add_argument("--tile", default="auto",
                    help="gather tile TXxTYxTZ or 'auto' (autotune)")
     p.add_argument("--plain-beams", type=int, default=4)
+    p.add_argument("--method", default="auto",
+                   choices=["auto", "gather", "fft"],
+                   help="convolution of the headline and batched numbers: "
+                        "the direct stencil gather, the FFT form, or the "
+                        "faster of the two (both are always timed)")
     p.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                    help="first run one device copy kernel of MIB MiB (the "
                         "known-size reference for PMC byte counters)")
@@ -66,10 +71,11 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def traffic_from_profile():
-    """HBM bytes per gather launch from the committed rocprofv3 PMC
-    summary, if present (line 'traffic_bytes_per_launch: N')."""
-    path = os.path.join(ROOT, "profiles", "gather_hetero_pmc.md")
+def traffic_from_profile(name="gather_hetero_pmc.md"):
+    """HBM bytes per launch of a kernel from a committed rocprofv3 PMC
+    summary under profiles/, if present (line 'traffic_bytes_per_launch:
+    N')."""
+    path = os.path.join(ROOT, "profiles", name)
     try:
         with open(path) as f:
             m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)",
@@ -177,57 +183,107 @@ def run_hetero(args, eng):
         tile = tuple(int(v) for v in args.tile.split("x"))
         plan.set_tile(*tile)
 
-    for _ in range(args.warmup):
-        E.terma([beam], shape, case.origin, case.spacing, stream=s,
-                check=False)
-        E.convolve(plan, d_t, d_d, d_box, stream=s)
-    s.sync()
+    def conv_fn(method):
+        if method == "gather":
+            return lambda: E.convolve(plan, d_t, d_d, d_box, stream=s)
+        plan.spectrum(s)
+        return lambda: E.convolve_fft(plan, d_t, d_d, stream=s)
 
-    # timed region: K back-to-back steps, events at every step boundary and
-    # around every gather launch (all on stream s)
-    # (g0, g1: the whole convolve call -- padding, slab boxes, gather;
-    # k0, k1: the gather kernel alone, recorded by the library)
-    K = args.steps
-    ev = [Event() for _ in range(K + 1)]
-    g0 = [Event() for _ in range(K)]
-    g1 = [Event() for _ in range(K)]
-    k0 = [Event() for _ in range(K)]
-    k1 = [Event() for _ in range(K)]
-    s.sync()
-    t_start = time.perf_counter()
-    ev[0].record(s)
-    for i in range(K):
-        E.terma([beam], shape, case.origin, case.spacing, stream=s,
-                check=False)
-        g0[i].record(s)
-        arm_launch_events(k0[i], k1[i])
-        E.convolve(plan, d_t, d_d, d_box, stream=s)
-        g1[i].record(s)
-        ev[i + 1].record(s)
-    s.sync()
-    wall = time.perf_counter() - t_start
-    _lib.check(_lib.lib().ch_terma_check(s.handle))
-    step_ms = np.array([ev[i].elapsed_ms(ev[i + 1]) for i in range(K)])
-    call_ms = np.array([g0[i].elapsed_ms(g1[i]) for i in range(K)])
-    gather_ms = np.array([k0[i].elapsed_ms(k1[i]) for i in range(K)])
-    gpu_total_ms = ev[0].elapsed_ms(ev[K])
-    dose = d_d.download(stream=s)
-    terma_gpu = d_t.download(stream=s)
-
-    # the same step captured once as a HIP graph and replayed K times
-    with E.Graph.capture(s) as cap:
-        E.terma([beam], shape, case.origin, case.spacing, stream=s,
-                check=False)
-        E.convolve(plan, d_t, d_d, d_box, stream=s)
-    graph = cap.graph
-    graph.launch(s)
-    s.sync()
-    t0 = time.perf_counter()
-    for _ in range(K):
+    def timed(method):
+        """K back-to-back steps (TERMA + convolution) with events at every
+        step boundary (ev), around every convolve call (g0, g1) and around
+        the method's dominant kernel, recorded by the library (k0, k1):
+        the gather, or the FFT form's convolution pass; then the same step
+        captured once as a HIP graph and replayed K times."""
+        conv = conv_fn(method)
+        for _ in range(args.warmup):
+            E.terma([beam], shape, case.origin, case.spacing, stream=s,
+                    check=False)
+            conv()
+        s.sync()
+        K = args.steps
+        ev = [Event() for _ in range(K + 1)]
+        g0 = [Event() for _ in range(K)]
+        g1 = [Event() for _ in range(K)]
+        k0 = [Event() for _ in range(K)]
+        k1 = [Event() for _ in range(K)]
+        s.sync()
+        t_start = time.perf_counter()
+        ev[0].record(s)
+        for i in range(K):
+            E.terma([beam], shape, case.origin, case.spacing, stream=s,
+                    check=False)
+            g0[i].record(s)
+            arm_launch_events(k0[i], k1[i])
+            conv()
+            g1[i].record(s)
+            ev[i + 1].record(s)
+        s.sync()
+        wall = time.perf_counter() - t_start
+        _lib.check(_lib.lib().ch_terma_check(s.handle))
+        r = {"wall": wall,
+             "step_ms": np.array([ev[i].elapsed_ms(ev[i + 1])
+                                  for i in range(K)]),
+             "call_ms": np.array([g0[i].elapsed_ms(g1[i])
+                                  for i in range(K)]),
+             "kernel_ms": np.array([k0[i].elapsed_ms(k1[i])
+                                    for i in range(K)]),
+             "gpu_total_ms": ev[0].elapsed_ms(ev[K]),
+             "dose": d_d.download(stream=s)}
+        with E.Graph.capture(s) as cap:
+            E.terma([beam], shape, case.origin, case.spacing, stream=s,
+                    check=False)
+            conv()
+        graph = cap.graph
         graph.launch(s)
-    s.sync()
-    graph_ms = (time.perf_counter() - t0) * 1e3 / K
-    graph.close()
+        s.sync()
+        t0 = time.perf_counter()
+        for _ in range(K):
+            graph.launch(s)
+        s.sync()
+        r["graph_ms"] = (time.perf_counter() - t0) * 1e3 / K
+        graph.close()
+        return r
+
+    K = args.steps
+    runs = {}
+    for method in ("gather", "fft"):
+        log("hetero: %s" % method)
+        runs[method] = timed(method)
+    terma_gpu = d_t.download(stream=s)
+    frame, pitch, _ = plan.spectrum(s)
+    kz = frame[2] // 2 + 1
+    # algorithmic bytes per launch of each method's dominant kernel
+    kernel_bytes = {
+        # one FP32 T read + one FP32 dose read-modify-write per voxel-cone
+        "gather": BYTES_PER_VOXEL_CONE * nv * n_cones,
+        # convolution pass: the x < X half-spectrum slabs read and written
+        # once (complex64) and K^ read once
+        "fft": 8 * kz * frame[1] * (2 * shape[0] + frame[0]),
+    }
+    traffic_file = {"gather": "gather_hetero_pmc.md",
+                    "fft": "spectral_hetero_pmc.md"}
+    kernel_name = {"gather": "gather_single (direct stencil gather)",
+                   "fft": "col_kernel<8, kColConv> (x transforms, "
+                          "times K^, inverse x transforms)"}
+    for method, r in runs.items():
+        k_ms = float(r["kernel_ms"].mean())
+        achieved = kernel_bytes[method] / (k_ms * 1e-3) / 1e9
+        r["beams_per_s"] = K / r["wall"]
+        r["roofline"] = {"bound": "hbm", "kernel": kernel_name[method],
+                         "achieved": achieved, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                         "algorithmic_bytes": kernel_bytes[method],
+                         "traffic": traffic_from_profile(
+                             traffic_file[method])}
+    if args.method == "auto":
+        head = max(runs, key=lambda m: runs[m]["beams_per_s"])
+    else:
+        head = args.method
+    hr = runs[head]
+    wall, step_ms, call_ms = hr["wall"], hr["step_ms"], hr["call_ms"]
+    gpu_total_ms, graph_ms, dose = (hr["gpu_total_ms"], hr["graph_ms"],
+                                    hr["dose"])
 
     # the convolve_c drop-in on host float64 buffers: PCIe-inclusive rate
     from conehead_amd.convolve_c import convolve_c
@@ -275,6 +331,9 @@ def run_hetero(args, eng):
             "dose_max_abs_err_over_dmax": err_over_dmax(dose, ref_dose),
             "terma_max_abs_err_over_tmax": err_over_dmax(terma_gpu, ref_t),
         }
+        for r in runs.values():
+            r["dose_max_abs_err_over_dmax"] = err_over_dmax(r["dose"],
+                                                            ref_dose)
         cpu = {"value": 1.0 / cpu_s, "unit": "beams/s", "cores": threads,
                "kind": "port",
                "sample": "one full beam of this workload: FP64 TERMA + "
@@ -295,9 +354,19 @@ def run_hetero(args, eng):
                       "%.2f s), scaled to the beam's %d updates" %
                       (t_terma, n_pick, n_src, u_s, t_conv, updates)}
 
-    g_ms = float(gather_ms.mean())
-    achieved = BYTES_PER_VOXEL_CONE * nv * n_cones / (g_ms * 1e-3) / 1e9
     beams_per_s = K / wall
+    methods = {}
+    for m, r in runs.items():
+        methods[m] = {
+            "beams_per_s": r["beams_per_s"],
+            "ms_per_step": r["wall"] * 1e3 / K,
+            "kernel_ms": float(r["kernel_ms"].mean()),
+            "convolve_call_ms": float(r["call_ms"].mean()),
+            "graph_ms_per_step": r["graph_ms"],
+            "dose_max_abs_err_over_dmax":
+                r.get("dose_max_abs_err_over_dmax"),
+            "roofline": r["roofline"]}
+    methods["fft"]["frame"] = list(frame)
     return {
         "value": beams_per_s,
         "ms_per_step": wall * 1e3 / K,
@@ -307,6 +376,7 @@ def run_hetero(args, eng):
                         "12x12 = 144 cones, 1 beam per step",
             "grid": list(shape), "voxel_mm": 2.5, "cones": n_cones,
             "beams_per_step": 1, "gather_tile": list(tile),
+            "method": head,
         },
         "voxel_cone_updates_per_s": nv * n_cones * beams_per_s,
         "dda_step_updates_per_s": (cpu["dda_step_updates"] * beams_per_s
@@ -315,73 +385,89 @@ def run_hetero(args, eng):
                        "p99": float(np.percentile(step_ms, 99))},
         "gpu_ms_per_step": gpu_total_ms / K,
         "graph_ms_per_step": graph_ms,
-        "gather_ms": g_ms,
+        "kernel_ms": methods[head]["kernel_ms"],
         "convolve_call_ms": float(call_ms.mean()),
         "terma_ms": float((step_ms - call_ms).mean()),
+        "methods": methods,
         "error_vs_fp64_oracle": result_err,
         "dropin_convolve_c_ms": dropin_ms,
         "physics_gather_ms": physics_ms,
-        "roofline": {
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic_from_profile(),
-        },
+        "roofline": runs[head]["roofline"],
         "cpu_baseline": cpu,
     }
 
 
 # --------------------------------------------------------------- c3 ------
-def run_batched(args, eng, steps=5):
-    from conehead_amd.batch import BatchEngine, DoseRequest
-    from conehead_amd.device import Event, Stream
-    from conehead_amd.workloads import batched
-
-    case = batched()
-    be = BatchEngine(eng)
-    req = DoseRequest.from_case(case, key="batched-phantom")
-    s = Stream()
-    tile = tune_requests(eng, be, [req], s)
-    be.release(be.collect(be.launch([req], s), download=False))  # warm-up
-    times, gpu = [], []
-    out = None
+def time_batches(be, reqs, s, steps):
+    """Median wall and GPU ms of `steps` launches of one batch (after a
+    warm-up launch, which also builds plan spectra for the FFT form); the
+    first timed batch's doses."""
+    from conehead_amd.device import Event
+    be.release(be.collect(be.launch(reqs, s), download=False))
+    times, gpu, first = [], [], None
     for i in range(steps):
         t0 = time.perf_counter()
         e0 = Event().record(s)
-        h = be.launch([req], s)
+        h = be.launch(reqs, s)
         h.event.sync()
         times.append((time.perf_counter() - t0) * 1e3)
         gpu.append(e0.elapsed_ms(h.event))
-        if i == steps - 1:
-            out = be.collect(h)[0]
+        if i == 0:
+            first = be.collect(h)
         else:
             be.release(be.collect(h, download=False))
-    # the same batch with interleaved beam groups where boxes allow
-    be1 = BatchEngine(eng, group_beams=True)
-    be1.release(be1.collect(be1.launch([req], s), download=False))
-    t1 = []
-    for i in range(2):
-        t0 = time.perf_counter()
-        h = be1.launch([req], s)
-        h.event.sync()
-        t1.append((time.perf_counter() - t0) * 1e3)
-        be1.release(be1.collect(h, download=False))
-    ms = float(np.median(times))
+    return float(np.median(times)), float(np.median(gpu)), first
+
+
+def pick(args, per):
+    if args.method != "auto":
+        return args.method
+    return max(per, key=lambda m: per[m]["beams_per_s"])
+
+
+def run_batched(args, eng, steps=5):
+    from conehead_amd.batch import BatchEngine, DoseRequest
+    from conehead_amd.device import Stream
+    from conehead_amd.workloads import batched
+
+    case = batched()
+    req = DoseRequest.from_case(case, key="batched-phantom")
+    s = Stream()
     nb = len(case.beams)
+    per, outs, tile = {}, {}, None
+    for method in ("gather", "fft"):
+        log("batched: %s" % method)
+        be = BatchEngine(eng, method=method)
+        if method == "gather":
+            tile = tune_requests(eng, be, [req], s)
+        ms, gms, out = time_batches(be, [req], s, steps)
+        outs[method] = out[0]
+        per[method] = {"ms_per_batch": ms, "gpu_ms_per_batch": gms,
+                       "beams_per_s": nb / (ms * 1e-3)}
+    # the gather with interleaved beam groups where boxes allow
+    be1 = BatchEngine(eng, group_beams=True)
+    ms1, _, _ = time_batches(be1, [req], s, 2)
+    per["gather"]["ms_per_batch_with_beam_groups"] = ms1
+    head = pick(args, per)
+    ms = per[head]["ms_per_batch"]
     r = {"workload": "64 MLC beams (gantry U[0,360), seed 2024, 60 leaf "
                      "pairs x 5 mm, apertures 2-20 cm) on the 128^3 "
-                     "heterogeneous phantom, 96 cones, one gather launch",
-         "gather_tile": list(list(tile.values())[0]),
-         "ms_per_batch": ms, "gpu_ms_per_batch": float(np.median(gpu)),
+                     "heterogeneous phantom, 96 cones: TERMA per beam, "
+                     "then one launch per convolution kernel for all 64",
+         "method": head, "gather_tile": list(list(tile.values())[0]),
+         "ms_per_batch": ms,
+         "gpu_ms_per_batch": per[head]["gpu_ms_per_batch"],
          "beams_per_s": nb / (ms * 1e-3),
-         "ms_per_batch_with_beam_groups": float(np.median(t1)),
          "voxel_cone_updates_per_s":
-             case.n_voxels * case.cones.n_cones * nb / (ms * 1e-3)}
+             case.n_voxels * case.cones.n_cones * nb / (ms * 1e-3),
+         "methods": per}
     if not args.no_cpu_baseline:
         ref, _, _, cpu_s = oracle_beam(case, 0, cpu_threads(args))
-        r["error_vs_fp64_oracle_beam0"] = err_over_dmax(out[0], ref)
+        for m in per:
+            per[m]["error_vs_fp64_oracle_beam0"] = err_over_dmax(outs[m][0],
+                                                                 ref)
+        r["error_vs_fp64_oracle_beam0"] = \
+            per[head]["error_vs_fp64_oracle_beam0"]
     return r, case
 
 
@@ -432,28 +518,26 @@ def tune_requests(eng, be, reqs, s):
 
 def run_uneven(args, eng):
     from conehead_amd.batch import BatchEngine
-    from conehead_amd.device import Event, Stream
+    from conehead_amd.device import Stream
     log("uneven: generating phantoms")
     cases, reqs = uneven_requests()
-    be = BatchEngine(eng)
     s = Stream()
-    tiles = tune_requests(eng, be, reqs, s)
-    be.release(be.collect(be.launch(reqs, s), download=False))  # warm-up
-    times = []
-    for i in range(max(1, min(args.steps, 5))):
-        t0 = time.perf_counter()
-        h = be.launch(reqs, s)
-        h.event.sync()
-        times.append((time.perf_counter() - t0) * 1e3)
-        if i == 0:
-            doses = be.collect(h)
-        else:
-            be.release(be.collect(h, download=False))
-    ms = float(np.median(times))
     nb = sum(len(c.beams) for c in cases)
     upd = sum(c.voxel_cone_updates() * len(c.beams) for c in cases)
+    steps = max(1, min(args.steps, 5))
+    per, outs, tiles = {}, {}, {}
+    for method in ("gather", "fft"):
+        log("uneven: %s" % method)
+        be = BatchEngine(eng, method=method)
+        if method == "gather":
+            tiles = tune_requests(eng, be, reqs, s)
+        ms, gms, outs[method] = time_batches(be, reqs, s, steps)
+        per[method] = {"ms_per_batch": ms, "gpu_ms_per_batch": gms,
+                       "beams_per_s": nb / (ms * 1e-3)}
+    head = pick(args, per)
+    ms = per[head]["ms_per_batch"]
     r = {"metric": METRIC, "value": nb / (ms * 1e-3), "unit": "beams/s",
-         "n_gpus": 1, "steps": len(times), "warmup": 1,
+         "n_gpus": 1, "steps": steps, "warmup": 1,
          "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
          "vs_baseline": None, "dtype": "f32 (f64 DDA alphas)",
          "data": "synthetic (seeded phantoms; EDKnrc varian_6 kernel)",
@@ -461,14 +545,19 @@ def run_uneven(args, eng):
                                 "64-200 x 64-200 x 64-160, 2-5 mm voxels, "
                                 "lung/bone ellipsoids, 1-9 beams, 48-144 "
                                 "cones, one batch",
-                    "beams": nb, "distinct_tiles": len(set(tiles.values()))},
-         "voxel_cone_updates_per_s": upd / (ms * 1e-3)}
+                    "beams": nb, "method": head,
+                    "distinct_tiles": len(set(tiles.values()))},
+         "voxel_cone_updates_per_s": upd / (ms * 1e-3),
+         "methods": per}
     if not args.no_cpu_baseline:
         i = int(np.argmin([c.n_voxels * c.cones.n_cones for c in cases]))
         ref, _, _, _ = oracle_beam(cases[i], 0, cpu_threads(args))
+        for m in per:
+            per[m]["error_vs_fp64_oracle"] = err_over_dmax(outs[m][i][0],
+                                                           ref)
         r["error_vs_fp64_oracle"] = {"case": cases[i].name, "beam": 0,
                                      "dose_max_abs_err_over_dmax":
-                                         err_over_dmax(doses[i][0], ref)}
+                                         per[head]["error_vs_fp64_oracle"]}
     return r
 
 
@@ -479,9 +568,11 @@ def run_serve(args, eng):
     from conehead_amd.serve import DoseServer, poisson_arrivals, replay
     log("serve: generating phantoms")
     cases, pool = uneven_requests()
-    be = BatchEngine(eng)
+    method = "fft" if args.method == "auto" else args.method
+    be = BatchEngine(eng, method=method)
     s = Stream()
-    tune_requests(eng, be, pool, s)
+    if method == "gather":
+        tune_requests(eng, be, pool, s)
     be.release(be.collect(be.launch(pool, s), download=False))  # warm-up
     rng = np.random.default_rng(99)
     sweep = []
@@ -523,7 +614,8 @@ def run_serve(args, eng):
             "config": {"workload": "serving: Poisson arrivals (seed 99), "
                                    "64 requests per lambda drawn from the "
                                    "uneven batch, resident densities, HBM "
-                                   "budget admission, 2 stream slots"},
+                                   "budget admission, 2 stream slots",
+                       "method": method},
             "sweep": sweep, "device_resident_results": resident}
 
 

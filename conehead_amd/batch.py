@@ -29,6 +29,26 @@ PAD_Y = 32   # kYPad of the padded TERMA copy (ch_internal.h)
 PAD_Z = 80   # kZPad
 
 
+def _smooth_at_least(n):
+    m = max(int(n), 1)
+    while True:
+        k = m
+        for q in (2, 3, 5):
+            while k % q == 0:
+                k //= q
+        if k == 1:
+            return m
+        m += 1
+
+
+def spectral_frame(shape):
+    """Frame lengths (smallest 2, 3, 5-smooth F >= 2 n - 1 per axis) and
+    half-spectrum pitch of the FFT form, as ch_plan_spectrum picks them."""
+    frame = tuple(_smooth_at_least(2 * int(n) - 1) for n in shape)
+    kz = frame[2] // 2 + 1
+    return frame, (kz + 15) // 16 * 16
+
+
 class DoseRequest:
     """One dose calculation: phantom + beams + cones."""
 
@@ -56,15 +76,21 @@ class DoseRequest:
     def n_jobs(self):
         return len(self.beams) if self.per_beam else 1
 
-    def device_bytes(self):
+    def device_bytes(self, method="gather"):
         """HBM this request holds while in flight: densities, TERMA, dose
-        and box per job, and the padded TERMA in the gather workspace (one
-        padded voxel per beam, in interleaved groups of four)."""
+        and box per job, plus the convolution's work space: the padded
+        TERMA of the gather workspace (one padded voxel per beam, in
+        interleaved groups of four) or, for the FFT form, one complex
+        half-spectrum work grid X x Fy x Kp per job."""
         X, Y, Z = self.shape
         nv = self.n_voxels
+        base = 4 * (nv + self.n_jobs * (2 * nv + 8))
+        if method == "fft":
+            frame, kp = spectral_frame(self.shape)
+            return base + 8 * self.n_jobs * X * frame[1] * kp
         padded = X * (Y + 2 * PAD_Y) * (Z + 2 * PAD_Z)
         slots = -(-self.n_jobs // 4) * 4
-        return 4 * (nv + self.n_jobs * (2 * nv + 8) + slots * padded)
+        return base + 4 * slots * padded
 
     def voxel_cone_updates(self):
         """Dense work count Nv x C per beam, summed over the beams."""

@@ -19,8 +19,8 @@
 //   col     x transforms, times K^, inverse x transforms, rows x < X kept;
 //   col     inverse y transforms, rows y < Y kept;
 //   z_inv   two half spectra per complex inverse FFT into the dose lines.
-// The transforms are mixed-radix (2, 3, 4, 5, 8) Stockham FFTs in LDS, W
-// lines per workgroup; frame lengths are products of 2, 3 and 5.
+// The transforms are mixed-radix (2, 3, 4, 5, 8, 16) Stockham FFTs in LDS,
+// W lines per workgroup; frame lengths are products of 2, 3 and 5.
 #include "ch_internal.h"
 
 #include <algorithm>
@@ -34,7 +34,9 @@ namespace {
 
 constexpr int kFftThreads = 256;
 constexpr int kFftMax = 1024;      // longest transform
-constexpr int kFftLdsElems = 4096; // F * W per ping-pong buffer
+constexpr int kFftLdsElems = 2048; // F * W per ping-pong buffer
+// Elements of one line group per thread (F * W <= kFftLdsElems).
+constexpr int kPerThread = kFftLdsElems / kFftThreads;
 // Work grids one batched launch may use (beams beyond it take further
 // launches).
 constexpr size_t kSpectralWorkBytes = (size_t)4 << 30;
@@ -159,37 +161,70 @@ __device__ __forceinline__ void dft(float2 (&v)[5])
     v[3] = csub(a2, b2);
 }
 
-// One Stockham stage of radix R over W lines of length F in LDS: element
-// (n, w) of a buffer at n * sn + w * sw.  Sub-transforms of length Ns are
-// done; this stage makes them Ns * R long, in natural order.  WFAST maps
-// consecutive threads to consecutive lines (w), else to consecutive j.
-template <int R, bool INV, bool WFAST>
+template <bool INV>
+__device__ __forceinline__ void dft(float2 (&v)[16])
+{
+    // 4 x 4: DFT4 over n1 of x[4 n1 + n2], twiddle W16^(n2 k1), DFT4 over
+    // n2, output k1 + 4 k2
+    const float c8 = 0.92387953251128674f, s8 = 0.38268343236508977f;
+    const float h = 0.70710678118654752f;
+    const float sg = INV ? 1.0f : -1.0f;
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2)
+        dft4<INV>(v[n2], v[4 + n2], v[8 + n2], v[12 + n2]);
+    // after this v[n2 + 4 k1] holds y[n2][k1]; W16^m = (cos, sg sin)
+    const float2 w1 = make_float2(c8, sg * s8), w2 = make_float2(h, sg * h),
+                 w3 = make_float2(s8, sg * c8), w6 = make_float2(-h, sg * h),
+                 w9 = make_float2(-c8, -sg * s8);
+    v[1 + 4] = cmul(v[1 + 4], w1);
+    v[1 + 8] = cmul(v[1 + 8], w2);
+    v[1 + 12] = cmul(v[1 + 12], w3);
+    v[2 + 4] = cmul(v[2 + 4], w2);
+    v[2 + 8] = rot<INV>(v[2 + 8]);
+    v[2 + 12] = cmul(v[2 + 12], w6);
+    v[3 + 4] = cmul(v[3 + 4], w3);
+    v[3 + 8] = cmul(v[3 + 8], w6);
+    v[3 + 12] = cmul(v[3 + 12], w9);
+    float2 o[16];
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+        float2 a = v[4 * k1], b = v[4 * k1 + 1], c = v[4 * k1 + 2],
+               d = v[4 * k1 + 3];
+        dft4<INV>(a, b, c, d);
+        o[k1] = a;
+        o[k1 + 4] = b;
+        o[k1 + 8] = c;
+        o[k1 + 12] = d;
+    }
+#pragma unroll
+    for (int i = 0; i < 16; ++i)
+        v[i] = o[i];
+}
+
+// One Stockham stage of radix R over W lines of length F in LDS, element
+// (n, w) at n * W + w.  Sub-transforms of length Ns are done; this stage
+// makes them Ns * R long, in natural order.  Each thread keeps one line
+// (w = tid % W) and steps j by kFftThreads / W: no divisions.
+template <int R, bool INV, int W>
 __device__ __forceinline__ void fft_stage(const float2* __restrict__ src,
                                           float2* __restrict__ dst, int F,
-                                          int W, int Ns, int sn, int sw,
-                                          const float2* __restrict__ tw)
+                                          int Ns, const float2* __restrict__ tw)
 {
     const int nb = F / R;
     const int twstep = F / (Ns * R);
-    const int total = nb * W;
-    for (int it = threadIdx.x; it < total; it += blockDim.x) {
-        int j, w;
-        if (WFAST) {
-            w = it % W;
-            j = it / W;
-        } else {
-            j = it % nb;
-            w = it / nb;
-        }
-        const int jm = j % Ns;
+    const int w = threadIdx.x % W;
+    const bool pow2 = (Ns & (Ns - 1)) == 0;
+    for (int j = threadIdx.x / W; j < nb; j += kFftThreads / W) {
+        const int jm = pow2 ? (j & (Ns - 1)) : (j % Ns);
         float2 v[R];
 #pragma unroll
         for (int r = 0; r < R; ++r)
-            v[r] = src[(j + r * nb) * sn + w * sw];
+            v[r] = src[(j + r * nb) * W + w];
         if (Ns > 1) {
+            const int step = jm * twstep;
 #pragma unroll
             for (int r = 1; r < R; ++r) {
-                float2 t = tw[r * jm * twstep];
+                float2 t = tw[r * step];
                 if (INV)
                     t.y = -t.y;
                 v[r] = cmul(v[r], t);
@@ -199,34 +234,37 @@ __device__ __forceinline__ void fft_stage(const float2* __restrict__ src,
         const int d0 = (j - jm) * R + jm;
 #pragma unroll
         for (int r = 0; r < R; ++r)
-            dst[(d0 + r * Ns) * sn + w * sw] = v[r];
+            dst[(d0 + r * Ns) * W + w] = v[r];
     }
 }
 
-// Full transform of W lines held in `a`; returns the buffer (a or b) that
-// holds the result.  Ends with a barrier.
-template <bool INV, bool WFAST>
+// Full transform of the W lines held in `a`; returns the buffer (a or b)
+// holding the result.  Ends with a barrier.
+template <bool INV, int W>
 __device__ float2* lds_fft(float2* a, float2* b, const float2* tw,
-                           const FftLen& L, int W, int sn, int sw)
+                           const FftLen& L)
 {
     int Ns = 1;
     for (int s = 0; s < L.nfac; ++s) {
         const int R = L.fac[s];
         switch (R) {
+        case 16:
+            fft_stage<16, INV, W>(a, b, L.n, Ns, tw);
+            break;
         case 8:
-            fft_stage<8, INV, WFAST>(a, b, L.n, W, Ns, sn, sw, tw);
+            fft_stage<8, INV, W>(a, b, L.n, Ns, tw);
             break;
         case 4:
-            fft_stage<4, INV, WFAST>(a, b, L.n, W, Ns, sn, sw, tw);
+            fft_stage<4, INV, W>(a, b, L.n, Ns, tw);
             break;
         case 2:
-            fft_stage<2, INV, WFAST>(a, b, L.n, W, Ns, sn, sw, tw);
+            fft_stage<2, INV, W>(a, b, L.n, Ns, tw);
             break;
         case 3:
-            fft_stage<3, INV, WFAST>(a, b, L.n, W, Ns, sn, sw, tw);
+            fft_stage<3, INV, W>(a, b, L.n, Ns, tw);
             break;
         default:
-            fft_stage<5, INV, WFAST>(a, b, L.n, W, Ns, sn, sw, tw);
+            fft_stage<5, INV, W>(a, b, L.n, Ns, tw);
             break;
         }
         __syncthreads();
@@ -270,58 +308,80 @@ __device__ __forceinline__ int group_of_block(const BeamIO& io)
     return (int)(blockIdx.x / (unsigned)io.n_beams);
 }
 
+// Line pair w of a z-pass workgroup: lines (x, y) and (x, y + 1) of the
+// grid, y even; valid = false past the last pair.
+struct LinePair {
+    int64_t x;
+    int32_t y;
+    bool valid;
+};
+__device__ __forceinline__ LinePair line_pair(int64_t p, int32_t X,
+                                              int32_t Y)
+{
+    const int Yp = (Y + 1) / 2;
+    LinePair lp;
+    lp.valid = p < (int64_t)X * Yp;
+    lp.x = lp.valid ? p / Yp : 0;
+    lp.y = lp.valid ? 2 * (int)(p - lp.x * Yp) : 0;
+    return lp;
+}
+
 // Forward z transforms.  Lines (x, y) of T (X x Y x Z, z fastest) are taken
 // in pairs (y even, y + 1) as one complex line T_y + i T_{y+1}, zero-padded
 // to F; after the FFT the two half spectra (k = 0 .. F/2) are separated,
 // X_y[k] = (C[k] + conj C[F-k]) / 2 and X_{y+1}[k] = (C[k] - conj C[F-k]) /
 // 2i, and written to A at row (x * a_rows + y) with pitch kp.  Workgroup:
-// W pairs, line-major in LDS with pitch F + 1.
+// W pairs, thread tid on pair tid % W.
 template <int W>
 __global__ __launch_bounds__(kFftThreads) void z_fwd_kernel(
     BeamIO io, float2* __restrict__ A, int32_t X, int32_t Y, int32_t Z,
     int32_t a_rows, int32_t kp, FftLen L, const float2* __restrict__ g_tw)
 {
     extern __shared__ float2 sm[];
-    const int F = L.n, LP = F + 1;
+    const int F = L.n;
     float2* s_tw = sm;
     float2* s_a = sm + F;
-    float2* s_b = s_a + W * LP;
+    float2* s_b = s_a + W * F;
     load_twiddles(s_tw, g_tw, F);
     const int beam = beam_of_block(io);
     const float* __restrict__ T = io.t_tab ? io.t_tab[beam] : io.t;
     A += beam * io.a_beam;
-    const int Yp = (Y + 1) / 2;
-    const int64_t npairs = (int64_t)X * Yp;
-    const int64_t p0 = (int64_t)group_of_block(io) * W;
-    for (int it = threadIdx.x; it < W * F; it += blockDim.x) {
-        const int w = it / F, n = it - w * F;
-        const int64_t p = p0 + w;
-        float2 v = make_float2(0.0f, 0.0f);
-        if (p < npairs && n < Z) {
-            const int64_t x = p / Yp;
-            const int y = 2 * (int)(p - x * Yp);
-            const float* row = T + (x * Y + y) * (int64_t)Z;
-            v.x = row[n];
-            if (y + 1 < Y)
-                v.y = row[Z + n];
+    const int w = threadIdx.x % W;
+    const int n0 = threadIdx.x / W;
+    constexpr int kStep = kFftThreads / W;
+    const LinePair lp =
+        line_pair((int64_t)group_of_block(io) * W + w, X, Y);
+    const bool two = lp.y + 1 < Y;
+    const float* row = T + (lp.x * Y + lp.y) * (int64_t)Z;
+    // every load of this thread first, then the LDS stores
+    float2 v[kPerThread];
+#pragma unroll
+    for (int i = 0; i < kPerThread; ++i) {
+        const int n = n0 + i * kStep;
+        v[i] = make_float2(0.0f, 0.0f);
+        if (lp.valid && n < Z) {
+            v[i].x = row[n];
+            if (two)
+                v[i].y = row[Z + n];
         }
-        s_a[w * LP + n] = v;
+    }
+#pragma unroll
+    for (int i = 0; i < kPerThread; ++i) {
+        const int n = n0 + i * kStep;
+        if (n < F)
+            s_a[n * W + w] = v[i];
     }
     __syncthreads();
-    const float2* r = lds_fft<false, false>(s_a, s_b, s_tw, L, W, 1, LP);
+    const float2* r = lds_fft<false, W>(s_a, s_b, s_tw, L);
+    if (!lp.valid)
+        return;
     const int K = F / 2 + 1;
-    for (int it = threadIdx.x; it < W * K; it += blockDim.x) {
-        const int w = it / K, k = it - w * K;
-        const int64_t p = p0 + w;
-        if (p >= npairs)
-            continue;
-        const int64_t x = p / Yp;
-        const int y = 2 * (int)(p - x * Yp);
-        const float2 c = r[w * LP + k];
-        const float2 cm = r[w * LP + (k == 0 ? 0 : F - k)];
-        float2* out = A + (x * a_rows + y) * (int64_t)kp;
+    float2* out = A + (lp.x * a_rows + lp.y) * (int64_t)kp;
+    for (int k = n0; k < K; k += kStep) {
+        const float2 c = r[k * W + w];
+        const float2 cm = r[(k == 0 ? 0 : F - k) * W + w];
         out[k] = make_float2(0.5f * (c.x + cm.x), 0.5f * (c.y - cm.y));
-        if (y + 1 < Y)
+        if (two)
             out[kp + k] =
                 make_float2(0.5f * (c.y + cm.y), -0.5f * (c.x - cm.x));
     }
@@ -330,8 +390,8 @@ __global__ __launch_bounds__(kFftThreads) void z_fwd_kernel(
 // Inverse z transforms: the half spectra of lines y and y + 1 (Hermitian
 // extension conj X[F-k] above F/2, imaginary parts of the k = 0 bin and,
 // for even F, the k = F/2 bin dropped) as C = X_y + i X_{y+1}; the inverse
-// FFT gives line y in the
-// real and line y + 1 in the imaginary part; z < Z is stored or added.
+// FFT gives line y in the real and line y + 1 in the imaginary part; z < Z
+// is stored or added.
 template <int W>
 __global__ __launch_bounds__(kFftThreads) void z_inv_kernel(
     BeamIO io, const float2* __restrict__ A, int32_t X, int32_t Y,
@@ -339,61 +399,66 @@ __global__ __launch_bounds__(kFftThreads) void z_inv_kernel(
     const float2* __restrict__ g_tw)
 {
     extern __shared__ float2 sm[];
-    const int F = L.n, LP = F + 1;
+    const int F = L.n;
     float2* s_tw = sm;
     float2* s_a = sm + F;
-    float2* s_b = s_a + W * LP;
+    float2* s_b = s_a + W * F;
     load_twiddles(s_tw, g_tw, F);
     const int beam = beam_of_block(io);
     float* __restrict__ D = io.d_tab ? io.d_tab[beam] : io.d;
     const int accumulate = io.acc_tab ? io.acc_tab[beam] : io.accumulate;
     A += beam * io.a_beam;
-    const int Yp = (Y + 1) / 2;
-    const int64_t npairs = (int64_t)X * Yp;
-    const int64_t p0 = (int64_t)group_of_block(io) * W;
+    const int w = threadIdx.x % W;
+    const int n0 = threadIdx.x / W;
+    constexpr int kStep = kFftThreads / W;
+    const LinePair lp =
+        line_pair((int64_t)group_of_block(io) * W + w, X, Y);
+    const bool two = lp.y + 1 < Y;
+    const float2* row = A + (lp.x * a_rows + lp.y) * (int64_t)kp;
     const int half = F / 2;
-    for (int it = threadIdx.x; it < W * F; it += blockDim.x) {
-        const int w = it / F, k = it - w * F;
-        const int64_t p = p0 + w;
-        float2 c = make_float2(0.0f, 0.0f);
-        if (p < npairs) {
-            const int64_t x = p / Yp;
-            const int y = 2 * (int)(p - x * Yp);
-            const float2* row = A + (x * a_rows + y) * (int64_t)kp;
-            const bool lo = k <= half;
-            const int kk = lo ? k : F - k;
-            float2 a = row[kk];
-            float2 b = (y + 1 < Y) ? row[kp + kk] : make_float2(0.0f, 0.0f);
-            if (!lo) {
-                a.y = -a.y;
-                b.y = -b.y;
-            }
-            if (k == 0 || 2 * k == F) {
-                a.y = 0.0f;
-                b.y = 0.0f;
-            }
-            c = make_float2(a.x - b.y, a.y + b.x);
+    float2 va[kPerThread], vb[kPerThread];
+#pragma unroll
+    for (int i = 0; i < kPerThread; ++i) {
+        const int k = n0 + i * kStep;
+        va[i] = make_float2(0.0f, 0.0f);
+        vb[i] = make_float2(0.0f, 0.0f);
+        if (lp.valid && k < F) {
+            const int kk = k <= half ? k : F - k;
+            va[i] = row[kk];
+            if (two)
+                vb[i] = row[kp + kk];
         }
-        s_a[w * LP + k] = c;
+    }
+#pragma unroll
+    for (int i = 0; i < kPerThread; ++i) {
+        const int k = n0 + i * kStep;
+        if (k >= F)
+            continue;
+        float2 a = va[i], b = vb[i];
+        if (k > half) {
+            a.y = -a.y;
+            b.y = -b.y;
+        }
+        if (k == 0 || 2 * k == F) {
+            a.y = 0.0f;
+            b.y = 0.0f;
+        }
+        s_a[k * W + w] = make_float2(a.x - b.y, a.y + b.x);
     }
     __syncthreads();
-    const float2* r = lds_fft<true, false>(s_a, s_b, s_tw, L, W, 1, LP);
-    for (int it = threadIdx.x; it < W * Z; it += blockDim.x) {
-        const int w = it / Z, n = it - w * Z;
-        const int64_t p = p0 + w;
-        if (p >= npairs)
-            continue;
-        const int64_t x = p / Yp;
-        const int y = 2 * (int)(p - x * Yp);
-        float* out = D + (x * Y + y) * (int64_t)Z;
-        const float2 c = r[w * LP + n];
+    const float2* r = lds_fft<true, W>(s_a, s_b, s_tw, L);
+    if (!lp.valid)
+        return;
+    float* out = D + (lp.x * Y + lp.y) * (int64_t)Z;
+    for (int n = n0; n < Z; n += kStep) {
+        const float2 c = r[n * W + w];
         if (accumulate) {
             out[n] += c.x;
-            if (y + 1 < Y)
+            if (two)
                 out[Z + n] += c.y;
         } else {
             out[n] = c.x;
-            if (y + 1 < Y)
+            if (two)
                 out[Z + n] = c.y;
         }
     }
@@ -403,9 +468,9 @@ enum ColMode { kColFwd = 0, kColInv = 1, kColConv = 2 };
 
 // Transforms along a strided axis: W adjacent columns (kz .. kz + W - 1) of
 // the rows n = 0 .. F-1 at base + n * row_stride, base = A + blockIdx.y *
-// outer_stride + W * blockIdx.x.  Rows n >= n_in read as zero; rows n <
-// n_out are written back in place.  kColConv multiplies by K^ (same
-// layout, its own strides) between the forward and the inverse transform.
+// outer_stride + W * group.  Rows n >= n_in read as zero; rows n < n_out
+// are written back in place.  kColConv multiplies by K^ (same layout, its
+// own strides) between the forward and the inverse transform.
 template <int W, int MODE>
 __global__ __launch_bounds__(kFftThreads) void col_kernel(
     BeamIO io, float2* __restrict__ A, int64_t row_stride,
@@ -421,38 +486,59 @@ __global__ __launch_bounds__(kFftThreads) void col_kernel(
     float2* s_b = s_a + F * W;
     load_twiddles(s_tw, g_tw, F);
     const int k0 = group_of_block(io) * W;
-    const int nv = min(W, n_cols - k0);
+    const int w = threadIdx.x % W;
+    const int n0 = threadIdx.x / W;
+    constexpr int kStep = kFftThreads / W;
+    const bool col_ok = w < n_cols - k0;
     float2* base = A + beam_of_block(io) * io.a_beam +
-                   blockIdx.y * outer_stride + k0;
-    for (int it = threadIdx.x; it < F * W; it += blockDim.x) {
-        const int n = it / W, w = it % W;
-        float2 v = make_float2(0.0f, 0.0f);
-        if (n < n_in && w < nv)
-            v = base[n * row_stride + w];
-        s_a[n * W + w] = v;
+                   blockIdx.y * outer_stride + k0 + w;
+    float2 v[kPerThread];
+#pragma unroll
+    for (int i = 0; i < kPerThread; ++i) {
+        const int n = n0 + i * kStep;
+        v[i] = make_float2(0.0f, 0.0f);
+        if (col_ok && n < n_in)
+            v[i] = base[n * row_stride];
+    }
+    // K^ for this thread's elements, loaded while the transform runs
+    float2 kh[MODE == kColConv ? kPerThread : 1];
+    if (MODE == kColConv) {
+        const float2* kb = khat + blockIdx.y * k_outer_stride + k0 + w;
+#pragma unroll
+        for (int i = 0; i < kPerThread; ++i) {
+            const int n = n0 + i * kStep;
+            kh[i] = make_float2(0.0f, 0.0f);
+            if (col_ok && n < F)
+                kh[i] = kb[n * k_row_stride];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < kPerThread; ++i) {
+        const int n = n0 + i * kStep;
+        if (n < F)
+            s_a[n * W + w] = v[i];
     }
     __syncthreads();
     float2* r;
     if (MODE == kColInv)
-        r = lds_fft<true, true>(s_a, s_b, s_tw, L, W, W, 1);
+        r = lds_fft<true, W>(s_a, s_b, s_tw, L);
     else
-        r = lds_fft<false, true>(s_a, s_b, s_tw, L, W, W, 1);
+        r = lds_fft<false, W>(s_a, s_b, s_tw, L);
     if (MODE == kColConv) {
-        const float2* kb = khat + blockIdx.y * k_outer_stride + k0;
-        for (int it = threadIdx.x; it < F * W; it += blockDim.x) {
-            const int n = it / W, w = it % W;
-            if (w < nv)
-                r[n * W + w] = cmul(r[n * W + w], kb[n * k_row_stride + w]);
+#pragma unroll
+        for (int i = 0; i < kPerThread; ++i) {
+            const int n = n0 + i * kStep;
+            if (n < F)
+                r[n * W + w] = cmul(r[n * W + w], kh[MODE == kColConv ? i : 0]);
         }
         __syncthreads();
         float2* o = (r == s_a) ? s_b : s_a;
-        r = lds_fft<true, true>(r, o, s_tw, L, W, W, 1);
+        r = lds_fft<true, W>(r, o, s_tw, L);
     }
-    for (int it = threadIdx.x; it < n_out * W; it += blockDim.x) {
-        const int n = it / W, w = it % W;
-        if (w < nv)
-            base[n * row_stride + w] = r[n * W + w];
-    }
+    if (!col_ok)
+        return;
+    for (int n = n0; n < n_out; n += kStep)
+        base[n * row_stride] = r[n * W + w];
 }
 
 __global__ void scatter_frame_kernel(float* __restrict__ frame,
@@ -466,17 +552,19 @@ __global__ void scatter_frame_kernel(float* __restrict__ frame,
 
 // ---------------------------------------------------------------- host --
 
+// Lines per workgroup: the most (up to 16) with F * W <= kFftLdsElems,
+// which is also what kPerThread elements per thread cover.
 int lines_per_group(int F)
 {
     int w = 16;
-    while (w > 4 && F * w > kFftLdsElems)
+    while (w > 2 && F * w > kFftLdsElems)
         w /= 2;
     return w;
 }
 
 size_t z_lds_bytes(int F, int W)
 {
-    return (size_t)(F + 2 * W * (F + 1)) * sizeof(float2);
+    return (size_t)(F + 2 * W * F) * sizeof(float2);
 }
 
 size_t col_lds_bytes(int F, int W)
@@ -498,7 +586,7 @@ bool fft_factor(int n, FftLen& L)
     L.n = n;
     L.nfac = 0;
     int m = n;
-    const int order[5] = {8, 4, 2, 3, 5};
+    const int order[6] = {16, 8, 4, 2, 3, 5};
     for (int r : order)
         while (m % r == 0 && L.nfac < 12) {
             L.fac[L.nfac++] = r;
@@ -580,8 +668,10 @@ ch_status z_fwd(const BeamIO& io, float2* A, int X, int Y, int Z, int a_rows,
         return launch_z_fwd<16>(io, A, X, Y, Z, a_rows, kp, L, tw, s);
     case 8:
         return launch_z_fwd<8>(io, A, X, Y, Z, a_rows, kp, L, tw, s);
-    default:
+    case 4:
         return launch_z_fwd<4>(io, A, X, Y, Z, a_rows, kp, L, tw, s);
+    default:
+        return launch_z_fwd<2>(io, A, X, Y, Z, a_rows, kp, L, tw, s);
     }
 }
 
@@ -594,8 +684,10 @@ ch_status z_inv(const BeamIO& io, const float2* A, int X, int Y, int Z,
         return launch_z_inv<16>(io, A, X, Y, Z, a_rows, kp, L, tw, s);
     case 8:
         return launch_z_inv<8>(io, A, X, Y, Z, a_rows, kp, L, tw, s);
-    default:
+    case 4:
         return launch_z_inv<4>(io, A, X, Y, Z, a_rows, kp, L, tw, s);
+    default:
+        return launch_z_inv<2>(io, A, X, Y, Z, a_rows, kp, L, tw, s);
     }
 }
 
@@ -615,8 +707,12 @@ ch_status col(const BeamIO& io, float2* A, int64_t row_stride,
         return launch_col<8, MODE>(io, A, row_stride, outer_stride, n_outer,
                                    n_cols, n_in, n_out, L, tw, khat,
                                    k_row_stride, k_outer_stride, s);
-    default:
+    case 4:
         return launch_col<4, MODE>(io, A, row_stride, outer_stride, n_outer,
+                                   n_cols, n_in, n_out, L, tw, khat,
+                                   k_row_stride, k_outer_stride, s);
+    default:
+        return launch_col<2, MODE>(io, A, row_stride, outer_stride, n_outer,
                                    n_cols, n_in, n_out, L, tw, khat,
                                    k_row_stride, k_outer_stride, s);
     }
@@ -656,8 +752,10 @@ size_t beam_points(const ch_plan* p)
 }
 
 // The five passes over the beams of `io` (all on plan p), work grids in A.
+// time_conv: the caller's launch events (and ch_set_timing) bracket the
+// convolution pass, the dominant kernel, instead of the whole call.
 ch_status spectral_passes(const ch_plan* p, BeamIO io, float2* A,
-                          hipStream_t s)
+                          hipStream_t s, bool time_conv)
 {
     const int X = p->shape[0], Y = p->shape[1], Z = p->shape[2];
     const int Fy = p->fft_n[1];
@@ -669,9 +767,17 @@ ch_status spectral_passes(const ch_plan* p, BeamIO io, float2* A,
     if (st == CH_OK)
         st = col<kColFwd>(io, A, kp, slab, X, kz, Y, Fy, Ly, p->d_tw[1],
                           nullptr, 0, 0, s);
-    if (st == CH_OK)
-        st = col<kColConv>(io, A, slab, kp, Fy, kz, X, X, Lx, p->d_tw[0],
-                           p->d_khat, slab, kp, s);
+    if (st == CH_OK) {
+        if (time_conv) {
+            LaunchTimer timer(s);
+            st = col<kColConv>(io, A, slab, kp, Fy, kz, X, X, Lx, p->d_tw[0],
+                               p->d_khat, slab, kp, s);
+            timer.stop();
+        } else {
+            st = col<kColConv>(io, A, slab, kp, Fy, kz, X, X, Lx, p->d_tw[0],
+                               p->d_khat, slab, kp, s);
+        }
+    }
     if (st == CH_OK)
         st = col<kColInv>(io, A, kp, slab, X, kz, Fy, Y, Ly, p->d_tw[1],
                           nullptr, 0, 0, s);
@@ -933,11 +1039,8 @@ ch_status ch_convolve_fft(const ch_plan* plan, const float* d_terma,
                                   beam_points(plan) * sizeof(float2), &work);
     if (st != CH_OK)
         return st;
-    LaunchTimer timer(s);
-    st = spectral_passes(plan, single_beam(d_terma, d_dose, accumulate),
-                         (float2*)work, s);
-    timer.stop();
-    return st;
+    return spectral_passes(plan, single_beam(d_terma, d_dose, accumulate),
+                           (float2*)work, s, true);
 }
 
 ch_status ch_convolve_fft_batch(int32_t n_jobs, const ch_conv_job* h_jobs,
@@ -1005,7 +1108,7 @@ ch_status ch_convolve_fft_batch(int32_t n_jobs, const ch_conv_job* h_jobs,
             io.d_tab = (float* const*)((char*)t->d + ptrs);
             io.acc_tab = (const int32_t*)((char*)t->d + 2 * ptrs);
             io.n_beams = nb;
-            st = spectral_passes(p, io, (float2*)work, s);
+            st = spectral_passes(p, io, (float2*)work, s, false);
             if (st != CH_OK)
                 return st;
         }

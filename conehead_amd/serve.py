@@ -22,20 +22,22 @@ import numpy as np
 class AdmissionQueue:
     """FIFO of pending requests with an HBM budget (no GPU calls)."""
 
-    def __init__(self, budget_bytes, max_batch_jobs=64):
+    def __init__(self, budget_bytes, max_batch_jobs=64, method="gather"):
         self.budget = int(budget_bytes)
         self.in_use = 0
         self.max_batch_jobs = max_batch_jobs
+        self.method = method      # convolution whose work space is counted
         self._q = collections.deque()
 
     def push(self, item):
         """item: (request, future, t_arrival).  Rejects requests that
         exceed the whole budget."""
         req, fut, _ = item
-        if req.device_bytes() > self.budget:
+        need = req.device_bytes(self.method)
+        if need > self.budget:
             fut.set_exception(MemoryError(
                 "request needs %d bytes of HBM; budget is %d" %
-                (req.device_bytes(), self.budget)))
+                (need, self.budget)))
             return False
         self._q.append(item)
         return True
@@ -48,7 +50,7 @@ class AdmissionQueue:
         batch, jobs = [], 0
         while self._q:
             req = self._q[0][0]
-            need = req.device_bytes()
+            need = req.device_bytes(self.method)
             if self.in_use + need > self.budget:
                 break
             if batch and jobs + req.n_jobs > self.max_batch_jobs:
@@ -60,7 +62,7 @@ class AdmissionQueue:
 
     def release(self, items):
         for req, _, _ in items:
-            self.in_use -= req.device_bytes()
+            self.in_use -= req.device_bytes(self.method)
 
 
 class DoseServer:
@@ -74,7 +76,8 @@ class DoseServer:
         if hbm_budget is None:
             free, _ = mem_info()
             hbm_budget = int(free * 0.8)
-        self.queue = AdmissionQueue(hbm_budget, max_batch_jobs)
+        self.queue = AdmissionQueue(hbm_budget, max_batch_jobs,
+                                    method=self.be.method)
         self.max_inflight = max_inflight
         # download=False delivers each result as its device dose grids
         # (the caller returns them with batch_engine.release([grids]))

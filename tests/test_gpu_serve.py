@@ -39,7 +39,7 @@ def test_budget_serialises_and_rejects(engine):
     with DoseServer(be, hbm_budget=one, max_inflight=2) as srv:
         futs = [srv.submit(r) for r in reqs]
         big = DoseRequest.from_case(cases[0])
-        big.device_bytes = lambda: one + 1
+        big.device_bytes = lambda method="gather": one + 1
         rej = srv.submit(big)
         got = [f.result(timeout=60) for f in futs]
     with pytest.raises(MemoryError):

@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 from conehead_amd import workloads as W
-from conehead_amd.batch import DoseRequest, group_beams
+from conehead_amd.batch import DoseRequest, group_beams, spectral_frame
 from conehead_amd.serve import AdmissionQueue, poisson_arrivals
 
 
@@ -15,7 +15,7 @@ class FakeRequest:
         self.nbytes = nbytes
         self.n_jobs = n_jobs
 
-    def device_bytes(self):
+    def device_bytes(self, method="gather"):
         return self.nbytes
 
 
@@ -36,6 +36,21 @@ def test_request_accounting():
                                     (2 * X * Y * Z + 8) + slots * padded)
     assert r.voxel_cone_updates() == X * Y * Z * c.cones.n_cones * \
         len(c.beams)
+    # FFT form: one complex X x Fy x Kp work grid per job instead
+    frame, kp = spectral_frame(c.shape)
+    assert all(f >= 2 * n - 1 for f, n in zip(frame, c.shape))
+    assert r.device_bytes("fft") == 4 * (X * Y * Z + r.n_jobs *
+                                         (2 * X * Y * Z + 8)) + \
+        8 * r.n_jobs * X * frame[1] * kp
+
+
+def test_spectral_frame():
+    """Smallest 2,3,5-smooth lengths >= 2n - 1 and the 16-aligned pitch of
+    the Fz / 2 + 1 half-spectrum bins."""
+    assert spectral_frame((128, 128, 128)) == ((256, 256, 256), 144)
+    assert spectral_frame((200, 200, 160)) == ((400, 400, 320), 176)
+    assert spectral_frame((24, 26, 22)) == ((48, 54, 45), 32)
+    assert spectral_frame((20, 13, 1)) == ((40, 25, 1), 16)
 
 
 def test_admission_fifo_and_budget():
