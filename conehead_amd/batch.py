@@ -202,19 +202,23 @@ class BatchEngine:
     beams that share a support box (arc control points a few degrees
     apart), not for beams at unrelated gantry angles.
 
-    method: "gather" (the direct stencil sum, ch_convolve_batch) or "fft"
-    (the same convolution through zero-padded FFTs, ch_convolve_fft_batch:
-    five HBM passes per beam whatever the cone count; beams on one plan
-    share each launch)."""
+    method: "fft" (the default: the convolution through zero-padded FFTs,
+    ch_convolve_fft_batch, five HBM passes per beam whatever the cone
+    count, beams on one plan sharing each launch) or "gather" (the direct
+    stencil sum, ch_convolve_batch; the default with group_beams)."""
 
     def __init__(self, engine=None, pool=None, group_beams=False,
-                 host_pool=None, method="gather"):
+                 host_pool=None, method=None):
+        if method is None:
+            method = "gather" if group_beams else "fft"
         if method not in ("gather", "fft"):
             raise ValueError("method must be 'gather' or 'fft'")
+        if group_beams and method != "gather":
+            raise ValueError("beam groups are a form of the gather")
         self.engine = engine or E.Engine()
         self.pool = pool or DevicePool()
         self.host_pool = host_pool or HostPool()
-        self.group_beams = group_beams and method == "gather"
+        self.group_beams = group_beams
         self.method = method
         self._resident = {}
         self._default = None

@@ -3,7 +3,8 @@
 The reference's flow -- density interpolation, hit test, effective depth,
 off-axis distance, fluence, softening and horn, TERMA, polyenergetic kernel,
 dose (conehead/conehead.py:522-773) -- runs as one fused TERMA launch and
-one collapsed-cone gather launch with every intermediate kept in HBM.  The
+the collapsed-cone convolution (five FFT passes, or one gather launch),
+with every intermediate kept in HBM.  The
 dose is returned and stored in self.dose_grid.dose (the reference plots it
 and discards it).
 
@@ -18,6 +19,11 @@ Settings keys are the reference's (example.py:27-58).  Optional keys:
       the radiological distance through the densities; without `kernel` the
       EDKnrc polyenergetic kernel of settings['energy_weights'] merged into
       settings.get('n_phi', 12) polar groups);
+  convolution -- "fft" (default for the convolve_c and cuda_dose
+      semantics: the cone stencil summed into one kernel and applied with
+      zero-padded FFTs, ch_convolve_fft) or "gather" (the direct stencil
+      sum, ch_convolve); physics mode always gathers, since its kernel
+      distance depends on the densities along each path;
   density_xy_swap -- True (default) reproduces the reference's density
       resampling, whose meshgrid(indexing='xy') evaluation at the grid nodes
       swaps the x and y axes (conehead/conehead.py:562-567); False uses the
@@ -128,9 +134,15 @@ class Conehead:
         _engine.terma([beam], shape, origin, spacing, stream=s)
         plan = eng.plan(shape, spacing, cones)
         d_dose = DeviceArray(shape, np.float32)
+        method = settings.get("convolution", "fft")
+        if method not in ("fft", "gather"):
+            raise ValueError("settings['convolution'] must be 'fft' or "
+                             "'gather'")
         if physics:
             _engine.convolve_rad(plan, d_terma, d_rho, d_dose, d_box,
                                  stream=s)
+        elif method == "fft":
+            _engine.convolve_fft(plan, d_terma, d_dose, stream=s)
         else:
             _engine.convolve(plan, d_terma, d_dose, d_box, stream=s)
         dose = d_dose.download(stream=s)

@@ -16,7 +16,8 @@ from conehead_amd.workloads import SETTINGS
 pytestmark = pytest.mark.gpu
 
 
-def test_calculate_matches_oracle_chain(oracle, engine):
+@pytest.mark.parametrize("method", ["fft", "gather"])
+def test_calculate_matches_oracle_chain(oracle, engine, method):
     n, s = 30, 0.5
     shape = (n, n, n)
     origin = np.array([-(n - 1) / 2 * s, s / 2, -(n - 1) / 2 * s])
@@ -28,7 +29,7 @@ def test_calculate_matches_oracle_chain(oracle, engine):
     blk.set_square(8)
     k = merged_kernel(PolyenergeticKernel(), 6)
     settings = dict(SETTINGS, kernel=k, thetas=default_thetas(8),
-                    phis=np.array(k.keys(), float))
+                    phis=np.array(k.keys(), float), convolution=method)
     ch = Conehead(engine)
     dose = ch.calculate(src, blk, phantom, settings)
     assert np.array_equal(ch.dose_grid.dose, dose)

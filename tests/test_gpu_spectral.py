@@ -165,7 +165,7 @@ def test_batch_engine_fft_method(engine):
                      n_phis=(4, 6))
     reqs = [DoseRequest.from_case(c) for c in cases]
     s = Stream()
-    gather = BatchEngine(engine).run(reqs, s)
+    gather = BatchEngine(engine, method="gather").run(reqs, s)
     fft_be = BatchEngine(engine, method="fft")
     fft = fft_be.run(reqs, s)
     alone = [fft_be.run([r], s)[0] for r in reqs]
