@@ -189,7 +189,9 @@ ch_status ch_plan_spectrum_info(const ch_plan* plan, int32_t* frame3,
  * as dose[v] (+)= sum_c sum_m k_cm T[v - o_cm]) computed as one linear
  * convolution of T with K through zero-padded FP32 FFTs: five HBM passes
  * per beam whatever the number of cones or taps.  Needs ch_plan_spectrum;
- * float dose; accumulate adds into d_dose. */
+ * float dose; accumulate adds into d_dose.  ch_set_launch_events and
+ * ch_set_timing bracket its convolution pass (the x transforms with the
+ * multiplication by K^), the dominant one of the five kernels. */
 ch_status ch_convolve_fft(const ch_plan* plan, const float* d_terma,
                           float* d_dose, int32_t accumulate, void* stream);
 /* ch_convolve_fft for many jobs (the ch_conv_job list of
