@@ -34,9 +34,15 @@ namespace {
 
 constexpr int kFftThreads = 256;
 constexpr int kFftMax = 1024;      // longest transform
-constexpr int kFftLdsElems = 2048; // F * W per ping-pong buffer
-// Elements of one line group per thread (F * W <= kFftLdsElems).
+// F * W per ping-pong buffer: the convolution pass takes up to 16 lines of
+// 256 (its radix-16 stages then keep all 256 threads busy and K^ is read in
+// 128-byte rows); the other passes run faster with half of that.
+constexpr int kFftLdsElemsConv = 4096;
+constexpr int kFftLdsElems = 2048;
+// Elements of one line group per thread in the unfused path (transforms
+// of one stage), for each limit.
 constexpr int kPerThread = kFftLdsElems / kFftThreads;
+constexpr int kPerThreadConv = kFftLdsElemsConv / kFftThreads;
 // Work grids one batched launch may use (beams beyond it take further
 // launches).
 constexpr size_t kSpectralWorkBytes = (size_t)4 << 30;
@@ -276,6 +282,161 @@ __device__ float2* lds_fft(float2* a, float2* b, const float2* tw,
     return a;
 }
 
+// ---- stages fed from, or draining to, registers -------------------------
+// A transform of factors f0 .. f(n-1) (n >= 2) runs its first stage from
+// global memory (no LDS store of the input) and its last stage to global
+// memory (no LDS read of the output); the convolution pass also fuses its
+// forward last stage, the product with K^ and its inverse first stage,
+// whose radices match because the inverse takes the factors in reverse.
+
+// First stage (Ns = 1): rows n = j + r * F / R come from load(n).
+template <int R, bool INV, int W, class Load>
+__device__ __forceinline__ void first_stage(const Load& load,
+                                            float2* __restrict__ dst, int F)
+{
+    const int nb = F / R;
+    const int w = threadIdx.x % W;
+    for (int j = threadIdx.x / W; j < nb; j += kFftThreads / W) {
+        float2 v[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            v[r] = load(j + r * nb);
+        dft<INV>(v);
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            dst[(j * R + r) * W + w] = v[r];
+    }
+}
+
+// Last stage (Ns = F / R): outputs k = j + r * F / R go to store(k, v).
+template <int R, bool INV, int W, class Store>
+__device__ __forceinline__ void last_stage(const float2* __restrict__ src,
+                                           int F,
+                                           const float2* __restrict__ tw,
+                                           const Store& store)
+{
+    const int nb = F / R;
+    const int w = threadIdx.x % W;
+    for (int j = threadIdx.x / W; j < nb; j += kFftThreads / W) {
+        float2 v[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            v[r] = src[(j + r * nb) * W + w];
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+            float2 t = tw[r * j];
+            if (INV)
+                t.y = -t.y;
+            v[r] = cmul(v[r], t);
+        }
+        dft<INV>(v);
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            store(j + r * nb, v[r]);
+    }
+}
+
+// Forward last stage, times K^ (kload(k) for output k), inverse first
+// stage: LDS src -> LDS dst.
+template <int R, int W, class KLoad>
+__device__ __forceinline__ void conv_stage(const float2* __restrict__ src,
+                                           float2* __restrict__ dst, int F,
+                                           const float2* __restrict__ tw,
+                                           const KLoad& kload)
+{
+    const int nb = F / R;
+    const int w = threadIdx.x % W;
+    for (int j = threadIdx.x / W; j < nb; j += kFftThreads / W) {
+        float2 kv[R], v[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            kv[r] = kload(j + r * nb);
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            v[r] = src[(j + r * nb) * W + w];
+#pragma unroll
+        for (int r = 1; r < R; ++r)
+            v[r] = cmul(v[r], tw[r * j]);
+        dft<false>(v);
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            v[r] = cmul(v[r], kv[r]);
+        dft<true>(v);
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            dst[(j * R + r) * W + w] = v[r];
+    }
+}
+
+template <bool INV, int W, class Load>
+__device__ __forceinline__ void first_any(int R, const Load& load,
+                                          float2* dst, int F)
+{
+    switch (R) {
+    case 16: first_stage<16, INV, W>(load, dst, F); break;
+    case 8: first_stage<8, INV, W>(load, dst, F); break;
+    case 4: first_stage<4, INV, W>(load, dst, F); break;
+    case 2: first_stage<2, INV, W>(load, dst, F); break;
+    case 3: first_stage<3, INV, W>(load, dst, F); break;
+    default: first_stage<5, INV, W>(load, dst, F); break;
+    }
+}
+
+template <bool INV, int W, class Store>
+__device__ __forceinline__ void last_any(int R, const float2* src, int F,
+                                         const float2* tw, const Store& st)
+{
+    switch (R) {
+    case 16: last_stage<16, INV, W>(src, F, tw, st); break;
+    case 8: last_stage<8, INV, W>(src, F, tw, st); break;
+    case 4: last_stage<4, INV, W>(src, F, tw, st); break;
+    case 2: last_stage<2, INV, W>(src, F, tw, st); break;
+    case 3: last_stage<3, INV, W>(src, F, tw, st); break;
+    default: last_stage<5, INV, W>(src, F, tw, st); break;
+    }
+}
+
+template <int W, class KLoad>
+__device__ __forceinline__ void conv_any(int R, const float2* src,
+                                         float2* dst, int F, const float2* tw,
+                                         const KLoad& kload)
+{
+    switch (R) {
+    case 16: conv_stage<16, W>(src, dst, F, tw, kload); break;
+    case 8: conv_stage<8, W>(src, dst, F, tw, kload); break;
+    case 4: conv_stage<4, W>(src, dst, F, tw, kload); break;
+    case 2: conv_stage<2, W>(src, dst, F, tw, kload); break;
+    case 3: conv_stage<3, W>(src, dst, F, tw, kload); break;
+    default: conv_stage<5, W>(src, dst, F, tw, kload); break;
+    }
+}
+
+// LDS stages fac[s] for s = s0, s0 + step, ... (stopping before s1), Ns
+// updated; each ends with a barrier.  Returns the buffer holding the data.
+template <bool INV, int W>
+__device__ float2* stages_range(float2* a, float2* b, const float2* tw,
+                                int F, const int32_t* fac, int s0, int s1,
+                                int step, int& Ns)
+{
+    for (int s = s0; s != s1; s += step) {
+        const int R = fac[s];
+        switch (R) {
+        case 16: fft_stage<16, INV, W>(a, b, F, Ns, tw); break;
+        case 8: fft_stage<8, INV, W>(a, b, F, Ns, tw); break;
+        case 4: fft_stage<4, INV, W>(a, b, F, Ns, tw); break;
+        case 2: fft_stage<2, INV, W>(a, b, F, Ns, tw); break;
+        case 3: fft_stage<3, INV, W>(a, b, F, Ns, tw); break;
+        default: fft_stage<5, INV, W>(a, b, F, Ns, tw); break;
+        }
+        __syncthreads();
+        Ns *= R;
+        float2* t = a;
+        a = b;
+        b = t;
+    }
+    return a;
+}
+
 __device__ __forceinline__ void load_twiddles(float2* s_tw, const float2* g,
                                               int F)
 {
@@ -353,6 +514,35 @@ __global__ __launch_bounds__(kFftThreads) void z_fwd_kernel(
         line_pair((int64_t)group_of_block(io) * W + w, X, Y);
     const bool two = lp.y + 1 < Y;
     const float* row = T + (lp.x * Y + lp.y) * (int64_t)Z;
+    const int K = F / 2 + 1;
+    float2* out = A + (lp.x * a_rows + lp.y) * (int64_t)kp;
+    if (L.nfac >= 2) {
+        auto load = [&](int n) -> float2 {
+            float2 v = make_float2(0.0f, 0.0f);
+            if (lp.valid && n < Z) {
+                v.x = row[n];
+                if (two)
+                    v.y = row[Z + n];
+            }
+            return v;
+        };
+        first_any<false, W>(L.fac[0], load, s_a, F);
+        __syncthreads();
+        int Ns = L.fac[0];
+        const float2* r =
+            stages_range<false, W>(s_a, s_b, s_tw, F, L.fac, 1, L.nfac, 1, Ns);
+        if (!lp.valid)
+            return;
+        for (int k = n0; k < K; k += kStep) {
+            const float2 c = r[k * W + w];
+            const float2 cm = r[(k == 0 ? 0 : F - k) * W + w];
+            out[k] = make_float2(0.5f * (c.x + cm.x), 0.5f * (c.y - cm.y));
+            if (two)
+                out[kp + k] =
+                    make_float2(0.5f * (c.y + cm.y), -0.5f * (c.x - cm.x));
+        }
+        return;
+    }
     // every load of this thread first, then the LDS stores
     float2 v[kPerThread];
 #pragma unroll
@@ -375,8 +565,6 @@ __global__ __launch_bounds__(kFftThreads) void z_fwd_kernel(
     const float2* r = lds_fft<false, W>(s_a, s_b, s_tw, L);
     if (!lp.valid)
         return;
-    const int K = F / 2 + 1;
-    float2* out = A + (lp.x * a_rows + lp.y) * (int64_t)kp;
     for (int k = n0; k < K; k += kStep) {
         const float2 c = r[k * W + w];
         const float2 cm = r[(k == 0 ? 0 : F - k) * W + w];
@@ -416,6 +604,47 @@ __global__ __launch_bounds__(kFftThreads) void z_inv_kernel(
     const bool two = lp.y + 1 < Y;
     const float2* row = A + (lp.x * a_rows + lp.y) * (int64_t)kp;
     const int half = F / 2;
+    float* out = D + (lp.x * Y + lp.y) * (int64_t)Z;
+    if (L.nfac >= 2) {
+        auto load = [&](int k) -> float2 {
+            float2 a = make_float2(0.0f, 0.0f), b = a;
+            if (lp.valid) {
+                const int kk = k <= half ? k : F - k;
+                a = row[kk];
+                if (two)
+                    b = row[kp + kk];
+            }
+            if (k > half) {
+                a.y = -a.y;
+                b.y = -b.y;
+            }
+            if (k == 0 || 2 * k == F) {
+                a.y = 0.0f;
+                b.y = 0.0f;
+            }
+            return make_float2(a.x - b.y, a.y + b.x);
+        };
+        first_any<true, W>(L.fac[0], load, s_a, F);
+        __syncthreads();
+        int Ns = L.fac[0];
+        const float2* cur = stages_range<true, W>(s_a, s_b, s_tw, F, L.fac, 1,
+                                                  L.nfac - 1, 1, Ns);
+        auto store = [&](int n, float2 c) {
+            if (!lp.valid || n >= Z)
+                return;
+            if (accumulate) {
+                out[n] += c.x;
+                if (two)
+                    out[Z + n] += c.y;
+            } else {
+                out[n] = c.x;
+                if (two)
+                    out[Z + n] = c.y;
+            }
+        };
+        last_any<true, W>(L.fac[L.nfac - 1], cur, F, s_tw, store);
+        return;
+    }
     float2 va[kPerThread], vb[kPerThread];
 #pragma unroll
     for (int i = 0; i < kPerThread; ++i) {
@@ -449,7 +678,6 @@ __global__ __launch_bounds__(kFftThreads) void z_inv_kernel(
     const float2* r = lds_fft<true, W>(s_a, s_b, s_tw, L);
     if (!lp.valid)
         return;
-    float* out = D + (lp.x * Y + lp.y) * (int64_t)Z;
     for (int n = n0; n < Z; n += kStep) {
         const float2 c = r[n * W + w];
         if (accumulate) {
@@ -479,6 +707,7 @@ __global__ __launch_bounds__(kFftThreads) void col_kernel(
     const float2* __restrict__ khat, int64_t k_row_stride,
     int64_t k_outer_stride)
 {
+    constexpr int kPer = MODE == kColConv ? kPerThreadConv : kPerThread;
     extern __shared__ float2 sm[];
     const int F = L.n;
     float2* s_tw = sm;
@@ -492,20 +721,58 @@ __global__ __launch_bounds__(kFftThreads) void col_kernel(
     const bool col_ok = w < n_cols - k0;
     float2* base = A + beam_of_block(io) * io.a_beam +
                    blockIdx.y * outer_stride + k0 + w;
-    float2 v[kPerThread];
+    if (L.nfac >= 2) {
+        const int nf = L.nfac;
+        auto load = [&](int n) -> float2 {
+            return (col_ok && n < n_in) ? base[n * row_stride]
+                                        : make_float2(0.0f, 0.0f);
+        };
+        auto store = [&](int n, float2 v) {
+            if (col_ok && n < n_out)
+                base[n * row_stride] = v;
+        };
+        first_any<MODE == kColInv, W>(L.fac[0], load, s_a, F);
+        __syncthreads();
+        int Ns = L.fac[0];
+        float2* cur;
+        if (MODE == kColConv) {
+            cur = stages_range<false, W>(s_a, s_b, s_tw, F, L.fac, 1, nf - 1,
+                                         1, Ns);
+            float2* oth = (cur == s_a) ? s_b : s_a;
+            const float2* kb = khat + blockIdx.y * k_outer_stride + k0 + w;
+            auto kload = [&](int k) -> float2 {
+                return col_ok ? kb[k * k_row_stride]
+                              : make_float2(0.0f, 0.0f);
+            };
+            conv_any<W>(L.fac[nf - 1], cur, oth, F, s_tw, kload);
+            __syncthreads();
+            // the inverse takes the factors in reverse: its first stage
+            // (radix fac[nf-1]) was fused above
+            Ns = L.fac[nf - 1];
+            cur = stages_range<true, W>(oth, cur, s_tw, F, L.fac, nf - 2, 0,
+                                        -1, Ns);
+            last_any<true, W>(L.fac[0], cur, F, s_tw, store);
+        } else {
+            cur = stages_range<MODE == kColInv, W>(s_a, s_b, s_tw, F, L.fac,
+                                                   1, nf - 1, 1, Ns);
+            last_any<MODE == kColInv, W>(L.fac[nf - 1], cur, F, s_tw, store);
+        }
+        return;
+    }
+    float2 v[kPer];
 #pragma unroll
-    for (int i = 0; i < kPerThread; ++i) {
+    for (int i = 0; i < kPer; ++i) {
         const int n = n0 + i * kStep;
         v[i] = make_float2(0.0f, 0.0f);
         if (col_ok && n < n_in)
             v[i] = base[n * row_stride];
     }
     // K^ for this thread's elements, loaded while the transform runs
-    float2 kh[MODE == kColConv ? kPerThread : 1];
+    float2 kh[MODE == kColConv ? kPer : 1];
     if (MODE == kColConv) {
         const float2* kb = khat + blockIdx.y * k_outer_stride + k0 + w;
 #pragma unroll
-        for (int i = 0; i < kPerThread; ++i) {
+        for (int i = 0; i < kPer; ++i) {
             const int n = n0 + i * kStep;
             kh[i] = make_float2(0.0f, 0.0f);
             if (col_ok && n < F)
@@ -513,7 +780,7 @@ __global__ __launch_bounds__(kFftThreads) void col_kernel(
         }
     }
 #pragma unroll
-    for (int i = 0; i < kPerThread; ++i) {
+    for (int i = 0; i < kPer; ++i) {
         const int n = n0 + i * kStep;
         if (n < F)
             s_a[n * W + w] = v[i];
@@ -526,7 +793,7 @@ __global__ __launch_bounds__(kFftThreads) void col_kernel(
         r = lds_fft<false, W>(s_a, s_b, s_tw, L);
     if (MODE == kColConv) {
 #pragma unroll
-        for (int i = 0; i < kPerThread; ++i) {
+        for (int i = 0; i < kPer; ++i) {
             const int n = n0 + i * kStep;
             if (n < F)
                 r[n * W + w] = cmul(r[n * W + w], kh[MODE == kColConv ? i : 0]);
@@ -552,12 +819,12 @@ __global__ void scatter_frame_kernel(float* __restrict__ frame,
 
 // ---------------------------------------------------------------- host --
 
-// Lines per workgroup: the most (up to 16) with F * W <= kFftLdsElems,
-// which is also what kPerThread elements per thread cover.
-int lines_per_group(int F)
+// Lines per workgroup: the most (up to 16) with F * W <= limit, which
+// kPerThread elements per thread always cover.
+int lines_per_group(int F, int limit = kFftLdsElems)
 {
     int w = 16;
-    while (w > 2 && F * w > kFftLdsElems)
+    while (w > 2 && F * w > limit)
         w /= 2;
     return w;
 }
@@ -698,7 +965,8 @@ ch_status col(const BeamIO& io, float2* A, int64_t row_stride,
               const float2* khat, int64_t k_row_stride,
               int64_t k_outer_stride, hipStream_t s)
 {
-    switch (lines_per_group(L.n)) {
+    switch (lines_per_group(
+        L.n, MODE == kColConv ? kFftLdsElemsConv : kFftLdsElems)) {
     case 16:
         return launch_col<16, MODE>(io, A, row_stride, outer_stride, n_outer,
                                     n_cols, n_in, n_out, L, tw, khat,

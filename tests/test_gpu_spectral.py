@@ -67,6 +67,7 @@ def run_fft(plan, T, accumulate_onto=None):
     ((17, 9, 31), (0.3, 0.5, 0.25), "convolve_c"),    # odd y: 36, 18, 64
     ((20, 13, 1), (0.4, 0.4, 0.4), "convolve_c"),     # one z voxel: 40, 25, 1
     ((30, 28, 40), (0.35, 0.3, 0.3), "cuda_dose"),    # 60, 60, 80
+    ((6, 5, 3), (0.5, 0.5, 0.5), "convolve_c"),       # 12, 9, 5: one-stage z
 ])
 def test_matches_oracle(oracle, engine, kern, shape, vs, semantics):
     rng = np.random.default_rng(sum(shape))
