@@ -1139,8 +1139,8 @@ static ch_status convolve_one(const ch_plan* p, const float* d_terma,
     const unsigned blocks =
         (unsigned)((n_tiles + k.tiles_per_block - 1) / k.tiles_per_block);
     LaunchTimer timer(s);
-    hipLaunchKernelGGL(k.single, dim3(blocks), dim3(k.threads), 0, s, j,
-                       (int)n_tiles);
+    timer.launch(k.single, dim3(blocks), dim3(k.threads), 0, j,
+                 (int)n_tiles);
     CH_HIP(hipGetLastError());
     timer.stop();
     return CH_OK;
@@ -1292,8 +1292,8 @@ ch_status ch_convolve_batch(int32_t n_jobs, const ch_conv_job* h_jobs,
             (unsigned)((tiles + k.batch_tiles_per_block - 1) /
                        k.batch_tiles_per_block);
         LaunchTimer timer(s);
-        hipLaunchKernelGGL(k.batch, dim3(blocks), dim3(k.batch_threads), 0, s,
-                           (const ConvJob*)w->d_tab, ng, (int)tiles);
+        timer.launch(k.batch, dim3(blocks), dim3(k.batch_threads), 0,
+                     (const ConvJob*)w->d_tab, ng, (int)tiles);
         CH_HIP(hipGetLastError());
         timer.stop();
     }
@@ -1412,9 +1412,8 @@ ch_status ch_convolve_groups(int32_t n_groups, const ch_conv_group* h_groups,
                               hipMemcpyHostToDevice, s));
         CH_HIP(hipEventRecord(w->jobs_done, s));
         LaunchTimer timer(s);
-        hipLaunchKernelGGL(kern, dim3((unsigned)((tiles + 3) / 4)),
-                           dim3(256), 0, s, (const GroupJob*)w->d_tab, ng,
-                           (int)tiles);
+        timer.launch(kern, dim3((unsigned)((tiles + 3) / 4)), dim3(256), 0,
+                     (const GroupJob*)w->d_tab, ng, (int)tiles);
         CH_HIP(hipGetLastError());
         timer.stop();
     }

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <cstdint>
 #include <vector>
@@ -121,39 +122,61 @@ bool capturing(hipStream_t s);
 // taken (and cleared) by the next gather or scatter launch.
 bool take_launch_events(hipEvent_t* start, hipEvent_t* stop);
 
-// Also records the caller's launch-event pair, if one is pending, right
-// around the launch (no wait).
+// Timing of one launch (or of a span of launches).  The caller's pending
+// launch-event pair (ch_set_launch_events), or a pair of its own when
+// ch_set_timing is on, is attached to the kernel launched through
+// launch(): hipExtLaunchKernelGGL records the events at the kernel's own
+// start and end, so they measure the kernel and not the dispatch gap
+// before it.  A span timer (span = true) records the pair around
+// everything queued between its construction and stop() instead.  stop()
+// waits only when ch_set_timing is on, and then records the duration for
+// ch_last_kernel_ms().  Nothing is recorded while the stream is captured.
 struct LaunchTimer {
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    hipEvent_t u0 = nullptr, u1 = nullptr;
     hipStream_t s;
-    bool on, user;
-    explicit LaunchTimer(hipStream_t st)
-        : s(st), on(timing_enabled() && !capturing(st)), user(false)
+    bool on, user, span, own;
+    explicit LaunchTimer(hipStream_t st, bool span_ = false)
+        : s(st), on(timing_enabled() && !capturing(st)), user(false),
+          span(span_), own(false)
     {
         if (!capturing(st))
-            user = take_launch_events(&u0, &u1);
-        if (user)
-            (void)hipEventRecord(u0, s);
-        if (on) {
+            user = take_launch_events(&e0, &e1);
+        if (on && !user) {
             (void)hipEventCreate(&e0);
             (void)hipEventCreate(&e1);
-            (void)hipEventRecord(e0, s);
+            own = true;
         }
+        if (span && (user || on))
+            (void)hipEventRecord(e0, s);
+    }
+    bool armed() const { return user || on; }
+    template <typename... Args, typename F = void (*)(Args...)>
+    void launch(F kernel, const dim3& grid, const dim3& block,
+                std::uint32_t lds, Args... args)
+    {
+        if (armed() && !span)
+            hipExtLaunchKernelGGL(kernel, grid, block, lds, s, e0, e1, 0,
+                                  args...);
+        else
+            hipLaunchKernelGGL(kernel, grid, block, lds, s, args...);
     }
     void stop()
     {
-        if (user)
-            (void)hipEventRecord(u1, s);
-        if (!on)
-            return;
-        (void)hipEventRecord(e1, s);
-        (void)hipEventSynchronize(e1);
-        float ms = -1.0f;
-        (void)hipEventElapsedTime(&ms, e0, e1);
-        record_kernel_ms(ms);
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
+        if (span && armed())
+            (void)hipEventRecord(e1, s);
+        if (on) {
+            (void)hipEventSynchronize(e1);
+            float ms = -1.0f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            record_kernel_ms(ms);
+        }
+        if (own) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+        }
+        own = false;
+        on = false;
+        user = false;
     }
 };
 

@@ -268,8 +268,8 @@ ch_status ch_convolve_rad(const ch_plan* plan, const float* d_terma,
     const unsigned blocks =
         (unsigned)((n_tiles + kRadWaves - 1) / kRadWaves);
     LaunchTimer timer(s);
-    hipLaunchKernelGGL(gather_rad, dim3(blocks), dim3(64 * kRadWaves), 0, s,
-                       j, (int)n_tiles);
+    timer.launch(gather_rad, dim3(blocks), dim3(64 * kRadWaves), 0, j,
+                 (int)n_tiles);
     CH_HIP(hipGetLastError());
     timer.stop();
     return CH_OK;

@@ -81,11 +81,10 @@ ch_status ch_convolve_scatter(const float* d_terma, float* d_dose,
     CH_REQUIRE(n < (int64_t)INT32_MAX * 256, "ch_convolve_scatter: too big");
     hipStream_t s = (hipStream_t)stream;
     LaunchTimer timer(s);
-    hipLaunchKernelGGL(scatter_kernel, dim3((unsigned)((n + 255) / 256)),
-                       dim3(256), 0, s, d_terma, d_dose, h_shape[0],
-                       h_shape[1], h_shape[2], h_vsize[0], h_vsize[1],
-                       h_vsize[2], n_cones, d_dirs, d_rows, d_cum, row_len,
-                       semantics);
+    timer.launch(scatter_kernel, dim3((unsigned)((n + 255) / 256)),
+                 dim3(256), 0, d_terma, d_dose, h_shape[0], h_shape[1],
+                 h_shape[2], h_vsize[0], h_vsize[1], h_vsize[2], n_cones,
+                 d_dirs, d_rows, d_cum, row_len, semantics);
     CH_HIP(hipGetLastError());
     timer.stop();
     return CH_OK;

@@ -469,6 +469,28 @@ __device__ __forceinline__ int group_of_block(const BeamIO& io)
     return (int)(blockIdx.x / (unsigned)io.n_beams);
 }
 
+// Workgroups go to the 8 XCDs round-robin by linear id, and each XCD has
+// its own L2.  The column passes take their work in XCD-major order
+// instead: XCD k runs a contiguous range of logical blocks, so the two
+// halves of a 128-byte line (neighbouring 8-column groups) and the beams
+// that read the same K^ chunk meet in one L2.  The work per block is
+// uniform, so the ranges stay balanced.
+constexpr unsigned kXcds = 8;
+struct BlockXY {
+    unsigned x, y;
+};
+__device__ __forceinline__ BlockXY xcd_major_block(bool on)
+{
+    const unsigned nx = gridDim.x, total = gridDim.x * gridDim.y;
+    unsigned l = blockIdx.x + nx * blockIdx.y;
+    if (on && total % kXcds == 0)
+        l = (l % kXcds) * (total / kXcds) + l / kXcds;
+    BlockXY b;
+    b.x = l % nx;
+    b.y = l / nx;
+    return b;
+}
+
 // Line pair w of a z-pass workgroup: lines (x, y) and (x, y + 1) of the
 // grid, y even; valid = false past the last pair.
 struct LinePair {
@@ -714,13 +736,16 @@ __global__ __launch_bounds__(kFftThreads) void col_kernel(
     float2* s_a = sm + F;
     float2* s_b = s_a + F * W;
     load_twiddles(s_tw, g_tw, F);
-    const int k0 = group_of_block(io) * W;
+    // (the convolution pass reads whole 128-byte lines and measured no
+    // faster in XCD-major order)
+    const BlockXY blk = xcd_major_block(MODE != kColConv);
+    const int beam = (int)(blk.x % (unsigned)io.n_beams);
+    const int k0 = (int)(blk.x / (unsigned)io.n_beams) * W;
     const int w = threadIdx.x % W;
     const int n0 = threadIdx.x / W;
     constexpr int kStep = kFftThreads / W;
     const bool col_ok = w < n_cols - k0;
-    float2* base = A + beam_of_block(io) * io.a_beam +
-                   blockIdx.y * outer_stride + k0 + w;
+    float2* base = A + beam * io.a_beam + blk.y * outer_stride + k0 + w;
     if (L.nfac >= 2) {
         const int nf = L.nfac;
         auto load = [&](int n) -> float2 {
@@ -739,7 +764,7 @@ __global__ __launch_bounds__(kFftThreads) void col_kernel(
             cur = stages_range<false, W>(s_a, s_b, s_tw, F, L.fac, 1, nf - 1,
                                          1, Ns);
             float2* oth = (cur == s_a) ? s_b : s_a;
-            const float2* kb = khat + blockIdx.y * k_outer_stride + k0 + w;
+            const float2* kb = khat + blk.y * k_outer_stride + k0 + w;
             auto kload = [&](int k) -> float2 {
                 return col_ok ? kb[k * k_row_stride]
                               : make_float2(0.0f, 0.0f);
@@ -770,7 +795,7 @@ __global__ __launch_bounds__(kFftThreads) void col_kernel(
     // K^ for this thread's elements, loaded while the transform runs
     float2 kh[MODE == kColConv ? kPer : 1];
     if (MODE == kColConv) {
-        const float2* kb = khat + blockIdx.y * k_outer_stride + k0 + w;
+        const float2* kb = khat + blk.y * k_outer_stride + k0 + w;
 #pragma unroll
         for (int i = 0; i < kPer; ++i) {
             const int n = n0 + i * kStep;
@@ -910,18 +935,25 @@ ch_status launch_col(const BeamIO& io, float2* A, int64_t row_stride,
                      int64_t outer_stride, int n_outer, int n_cols, int n_in,
                      int n_out, const FftLen& L, const float2* tw,
                      const float2* khat, int64_t k_row_stride,
-                     int64_t k_outer_stride, hipStream_t s)
+                     int64_t k_outer_stride, hipStream_t s,
+                     LaunchTimer* timer)
 {
     const size_t lds = col_lds_bytes(L.n, W);
     ch_status st = allow_lds(col_kernel<W, MODE>, lds);
     if (st != CH_OK)
         return st;
     const int64_t groups = (n_cols + W - 1) / W;
-    hipLaunchKernelGGL((col_kernel<W, MODE>),
-                       dim3((unsigned)(groups * io.n_beams), (unsigned)n_outer),
-                       dim3(kFftThreads), lds, s, io, A, row_stride,
-                       outer_stride, n_cols, n_in, n_out, L, tw, khat,
-                       k_row_stride, k_outer_stride);
+    const dim3 grid((unsigned)(groups * io.n_beams), (unsigned)n_outer);
+    if (timer)
+        timer->launch(col_kernel<W, MODE>, grid, dim3(kFftThreads),
+                      (std::uint32_t)lds, io, A, row_stride, outer_stride,
+                      n_cols, n_in, n_out, L, tw, khat, k_row_stride,
+                      k_outer_stride);
+    else
+        hipLaunchKernelGGL((col_kernel<W, MODE>), grid, dim3(kFftThreads),
+                           lds, s, io, A, row_stride, outer_stride, n_cols,
+                           n_in, n_out, L, tw, khat, k_row_stride,
+                           k_outer_stride);
     CH_HIP(hipGetLastError());
     return CH_OK;
 }
@@ -963,26 +995,27 @@ ch_status col(const BeamIO& io, float2* A, int64_t row_stride,
               int64_t outer_stride, int n_outer, int n_cols, int n_in,
               int n_out, const FftLen& L, const float2* tw,
               const float2* khat, int64_t k_row_stride,
-              int64_t k_outer_stride, hipStream_t s)
+              int64_t k_outer_stride, hipStream_t s,
+              LaunchTimer* timer = nullptr)
 {
     switch (lines_per_group(
         L.n, MODE == kColConv ? kFftLdsElemsConv : kFftLdsElems)) {
     case 16:
         return launch_col<16, MODE>(io, A, row_stride, outer_stride, n_outer,
                                     n_cols, n_in, n_out, L, tw, khat,
-                                    k_row_stride, k_outer_stride, s);
+                                    k_row_stride, k_outer_stride, s, timer);
     case 8:
         return launch_col<8, MODE>(io, A, row_stride, outer_stride, n_outer,
                                    n_cols, n_in, n_out, L, tw, khat,
-                                   k_row_stride, k_outer_stride, s);
+                                   k_row_stride, k_outer_stride, s, timer);
     case 4:
         return launch_col<4, MODE>(io, A, row_stride, outer_stride, n_outer,
                                    n_cols, n_in, n_out, L, tw, khat,
-                                   k_row_stride, k_outer_stride, s);
+                                   k_row_stride, k_outer_stride, s, timer);
     default:
         return launch_col<2, MODE>(io, A, row_stride, outer_stride, n_outer,
                                    n_cols, n_in, n_out, L, tw, khat,
-                                   k_row_stride, k_outer_stride, s);
+                                   k_row_stride, k_outer_stride, s, timer);
     }
 }
 
@@ -1039,7 +1072,7 @@ ch_status spectral_passes(const ch_plan* p, BeamIO io, float2* A,
         if (time_conv) {
             LaunchTimer timer(s);
             st = col<kColConv>(io, A, slab, kp, Fy, kz, X, X, Lx, p->d_tw[0],
-                               p->d_khat, slab, kp, s);
+                               p->d_khat, slab, kp, s, &timer);
             timer.stop();
         } else {
             st = col<kColConv>(io, A, slab, kp, Fy, kz, X, X, Lx, p->d_tw[0],
@@ -1341,7 +1374,7 @@ ch_status ch_convolve_fft_batch(int32_t n_jobs, const ch_conv_job* h_jobs,
         members[g].push_back(i);
     }
     std::lock_guard<std::mutex> lock(g_tab_mu);
-    LaunchTimer timer(s);
+    LaunchTimer timer(s, true);
     for (size_t g = 0; g < plans.size(); ++g) {
         const ch_plan* p = plans[g];
         const size_t per = beam_points(p) * sizeof(float2);

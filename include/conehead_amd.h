@@ -190,7 +190,7 @@ ch_status ch_plan_spectrum_info(const ch_plan* plan, int32_t* frame3,
  * convolution of T with K through zero-padded FP32 FFTs: five HBM passes
  * per beam whatever the number of cones or taps.  Needs ch_plan_spectrum;
  * float dose; accumulate adds into d_dose.  ch_set_launch_events and
- * ch_set_timing bracket its convolution pass (the x transforms with the
+ * ch_set_timing time its convolution pass (the x transforms with the
  * multiplication by K^), the dominant one of the five kernels. */
 ch_status ch_convolve_fft(const ch_plan* plan, const float* d_terma,
                           float* d_dose, int32_t accumulate, void* stream);
@@ -401,19 +401,23 @@ ch_status ch_mlc_table(int32_t* d_table, int32_t n_pairs,
 ch_status ch_raster_mlc(float* d_raster, const int32_t* d_table,
                         void* stream);
 
-/* Kernel time of the last gather or scatter launch, measured with HIP
- * events around the launch when timing is enabled (the call then waits for
- * the launch). */
+/* Kernel time of the last timed launch (the gather, the FFT form's
+ * convolution pass, the physics gather or the scatter; a whole
+ * ch_convolve_fft_batch call), measured with HIP events when timing is
+ * enabled (the call then waits for the launch). */
 ch_status ch_set_timing(int32_t enabled);
 ch_status ch_last_kernel_ms(float* ms);
 
 /* Arms a pair of events (ch_event_create) for the calling thread: the next
- * gather or scatter launch of this thread records `start` on its stream
- * right before the kernel and `stop` right after it, without waiting, and
+ * timed launch of this thread (as for ch_set_timing) takes them as the
+ * kernel's own start and end timestamps (hipExtLaunchKernelGGL), so their
+ * interval is the kernel's duration without the dispatch gap; a batched
+ * FFT call records them around the whole call.  It does not wait, and it
  * disarms the pair (read it with ch_event_elapsed once `stop` completed).
  * Pass two nulls to disarm.  Launches that prepare the gather (padding,
- * support and slab boxes) fall outside the pair.  No counterpart in the
- * reference: it is the measurement hook of the benchmark. */
+ * support and slab boxes) and the FFT form's other passes fall outside the
+ * pair.  No counterpart in the reference: it is the measurement hook of
+ * the benchmark. */
 ch_status ch_set_launch_events(void* start, void* stop);
 
 #ifdef __cplusplus
