@@ -1,6 +1,6 @@
-"""The gather at the benchmark's full size (128^3 at 2.5 mm, 12 x 12 cones)
-through properties that hold at any size, where the FP64 oracle would take
-minutes:
+"""The gather and the FFT form at the benchmark's full size (128^3 at
+2.5 mm, 12 x 12 cones) through properties that hold at any size, where the
+FP64 oracle would take minutes:
   - scaling T by 2 scales every product and sum exactly (FP32), so the dose
     doubles bit for bit;
   - linearity: dose(T1 + T2) = dose(T1) + dose(T2) within FP32 rounding;
@@ -81,3 +81,36 @@ def test_box_or_none(engine, case_terma):
     a = gather(engine, case, T, box=True)
     b = gather(engine, case, T, box=False)
     assert np.abs(a - b).max() / a.max() < 1e-6
+
+
+def fft(engine, case, T):
+    plan = engine.plan(case.shape, case.spacing, case.cones)
+    d_t = DeviceArray.from_host(T)
+    d_d = DeviceArray(case.shape, np.float32)
+    E.convolve_fft(plan, d_t, d_d)
+    return d_d.download()
+
+
+def test_fft_form_fullsize(engine, case_terma):
+    """The FFT form at 128^3 (frame 256^3): the same exact doubling (every
+    FFT operation commutes with a power-of-two scale), linearity, batch ==
+    single bitwise, and agreement with the independent gather within 1e-5
+    of Dmax (both are within 5e-7 of the FP64 oracle in bench.py)."""
+    case, T = case_terma
+    d1 = fft(engine, case, T)
+    assert d1.max() > 0
+    assert np.array_equal(fft(engine, case, 2.0 * T), 2.0 * d1)
+    rng = np.random.default_rng(17)
+    T2 = np.zeros_like(T)
+    T2[40:80, 20:60, 50:70] = rng.random((40, 40, 20)).astype(np.float32)
+    b = fft(engine, case, T2)
+    ab = fft(engine, case, T + T2)
+    assert np.abs(ab - (d1 + b)).max() / ab.max() < 1e-6
+    plan = engine.plan(case.shape, case.spacing, case.cones)
+    d_t = DeviceArray.from_host(T)
+    outs = [DeviceArray(case.shape, np.float32) for _ in range(2)]
+    E.convolve_fft_batch([(plan, d_t, o, None, False) for o in outs])
+    for o in outs:
+        assert np.array_equal(o.download(), d1)
+    g = gather(engine, case, T)
+    assert np.abs(d1 - g).max() / g.max() < 1e-5

@@ -1,4 +1,5 @@
-"""HIP-graph capture of the single-beam pipeline (zero, TERMA, gather).
+"""HIP-graph capture of the single-beam pipeline (zero, TERMA, then the
+gather, or the FFT form's five passes accumulating onto the zeroed dose).
 
 Every replay is checked bit for bit against the eager result, with eager
 work and host synchronisation between replays; a replay reads the buffers'
@@ -29,21 +30,25 @@ def setup(engine, s):
     return case, rho, t, d, box, beam, plan
 
 
-def run(case, t, d, box, beam, plan, s):
+def run(case, t, d, box, beam, plan, s, method="gather"):
     d.zero(s)
     E.terma([beam], case.shape, case.origin, case.spacing, stream=s,
             check=False)
-    E.convolve(plan, t, d, box, stream=s)
+    if method == "fft":
+        E.convolve_fft(plan, t, d, accumulate=True, stream=s)
+    else:
+        E.convolve(plan, t, d, box, stream=s)
 
 
-def test_graph_replays_match_eager(engine):
+@pytest.mark.parametrize("method", ["gather", "fft"])
+def test_graph_replays_match_eager(engine, method):
     s = Stream()
     case, rho, t, d, box, beam, plan = setup(engine, s)
-    run(case, t, d, box, beam, plan, s)           # eager, warms scratch
+    run(case, t, d, box, beam, plan, s, method)   # eager, warms scratch
     ref = d.download(stream=s)
     assert ref.max() > 0
     with E.Graph.capture(s) as cap:
-        run(case, t, d, box, beam, plan, s)
+        run(case, t, d, box, beam, plan, s, method)
     g = cap.graph
     for _ in range(4):
         d.zero(s)                                  # eager work in between
@@ -55,7 +60,7 @@ def test_graph_replays_match_eager(engine):
     rho.upload(case.densities * 1.1, s)
     g.launch(s)
     replay = d.download(stream=s)
-    run(case, t, d, box, beam, plan, s)
+    run(case, t, d, box, beam, plan, s, method)
     assert np.array_equal(replay, d.download(stream=s))
     assert not np.array_equal(replay, ref)
     g.close()
