@@ -263,9 +263,12 @@ def run_hetero(args, eng):
     }
     traffic_file = {"gather": "gather_hetero_pmc.md",
                     "fft": "spectral_hetero_pmc.md"}
+    lines = 16                      # the convolution pass's line group
+    while lines > 2 and frame[0] * lines > 4096:
+        lines //= 2
     kernel_name = {"gather": "gather_single (direct stencil gather)",
-                   "fft": "col_kernel<8, kColConv> (x transforms, "
-                          "times K^, inverse x transforms)"}
+                   "fft": "col_kernel<%d, 2> (kColConv: x transforms, "
+                          "times K^, inverse x transforms)" % lines}
     for method, r in runs.items():
         k_ms = float(r["kernel_ms"].mean())
         achieved = kernel_bytes[method] / (k_ms * 1e-3) / 1e9
