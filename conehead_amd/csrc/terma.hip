@@ -284,19 +284,43 @@ __global__ __launch_bounds__(256) void terma_kernel(TermaParams P)
             Dda d;
             d.init(dir[0], dir[1], dir[2], P.spacing[0], P.spacing[1],
                    P.spacing[2]);
+            // Only the axis that steps can leave the grid: keep the linear
+            // index and each axis's steps left to the boundary instead of
+            // recomputing the index and testing all six bounds per voxel
+            // (the same t-values, voxels and sums).
             int32_t c0 = x, c1 = y, c2 = z;
+            int64_t lin = ((int64_t)x * P.Y + y) * P.Z + z;
+            const int64_t st0 = (int64_t)d.s0 * P.Y * P.Z;
+            const int64_t st1 = (int64_t)d.s1 * P.Z;
+            const int64_t st2 = d.s2;
+            int32_t left0 = d.s0 > 0 ? P.X - 1 - x : x;
+            int32_t left1 = d.s1 > 0 ? P.Y - 1 - y : y;
+            int32_t left2 = d.s2 > 0 ? P.Z - 1 - z : z;
             double tprev = 0.0;
             float de = 0.0f, dc = 0.0f;
-            while (c0 >= 0 && c0 < P.X && c1 >= 0 && c1 < P.Y && c2 >= 0 &&
-                   c2 < P.Z) {
-                const float rho = P.density[((int64_t)c0 * P.Y + c1) * P.Z + c2];
-                const double t = d.advance(d.axis(), c0, c1, c2);
+            while (true) {
+                const float rho = P.density[lin];
+                const int a = d.axis();
+                const double t = d.advance(a, c0, c1, c2);
                 const float term = rho * (float)(t - tprev);
                 tprev = t;
                 const float yk = term - dc;
                 const float tk = de + yk;
                 dc = (tk - de) - yk;
                 de = tk;
+                int32_t left;
+                if (a == 0) {
+                    left = --left0;
+                    lin += st0;
+                } else if (a == 1) {
+                    left = --left1;
+                    lin += st1;
+                } else {
+                    left = --left2;
+                    lin += st2;
+                }
+                if (left < 0)
+                    break;
             }
 
             // off-axis distance (conehead.py:286-314)
