@@ -167,9 +167,12 @@ def test_batch_engine_fft_method(engine):
     reqs = [DoseRequest.from_case(c) for c in cases]
     s = Stream()
     gather = BatchEngine(engine, method="gather").run(reqs, s)
-    fft_be = BatchEngine(engine, method="fft")
+    fft_be = BatchEngine(engine, method="fft", pipeline=2)
     fft = fft_be.run(reqs, s)
     alone = [fft_be.run([r], s)[0] for r in reqs]
-    for g, f, a in zip(gather, fft, alone):
+    serial = BatchEngine(engine, method="fft", pipeline=0).run(reqs, s)
+    for g, f, a, o in zip(gather, fft, alone, serial):
         assert rel_err(f, g) < TOL
         assert np.array_equal(f, a)
+        assert np.array_equal(f, o)   # chunks on a companion stream
+    fft_be.close()
