@@ -49,6 +49,15 @@ def spectral_frame(shape):
     return frame, (kz + 15) // 16 * 16
 
 
+SPECTRAL_MAX_FRAME = 1024   # longest transform of ch_plan_spectrum
+
+
+def spectral_ok(shape):
+    """True when the FFT form takes this grid (every frame <= 1024, i.e.
+    axes of at most 512 voxels)."""
+    return max(spectral_frame(shape)[0]) <= SPECTRAL_MAX_FRAME
+
+
 class DoseRequest:
     """One dose calculation: phantom + beams + cones."""
 
@@ -85,7 +94,7 @@ class DoseRequest:
         X, Y, Z = self.shape
         nv = self.n_voxels
         base = 4 * (nv + self.n_jobs * (2 * nv + 8))
-        if method == "fft":
+        if method == "fft" and spectral_ok(self.shape):
             frame, kp = spectral_frame(self.shape)
             return base + 8 * self.n_jobs * X * frame[1] * kp
         padded = X * (Y + 2 * PAD_Y) * (Z + 2 * PAD_Z)
@@ -312,7 +321,9 @@ class BatchEngine:
                 beams.append(bs)
             for j in range(n_jobs):
                 outs.append(self.pool.get(req.shape))
-            if s2 is not None:
+            # grids beyond the FFT form's frames take the gather
+            fft_here = self.method == "fft" and spectral_ok(req.shape)
+            if s2 is not None and fft_here:
                 # a chunk's convolutions start as soon as its TERMA is done,
                 # beside the next chunk's TERMA
                 step = self.pipeline if req.per_beam else len(beams)
@@ -329,11 +340,16 @@ class BatchEngine:
             else:
                 E.terma(beams, req.shape, req.origin, req.spacing, stream=s,
                         check=False)
-                pending.append((plan, t_bufs, outs, boxes))
+                pending.append((plan, t_bufs, outs, boxes, fft_here))
             temps.extend(t_bufs)
             temps.extend(boxes)
             outputs.append(outs)
-        for plan, t_bufs, outs, boxes in pending:
+        fft_jobs = []
+        for plan, t_bufs, outs, boxes, fft_here in pending:
+            if fft_here:
+                fft_jobs.extend((plan, t_bufs[j], outs[j], None, False)
+                                for j in range(len(t_bufs)))
+                continue
             parts = [[j] for j in range(len(t_bufs))]
             if self.group_beams and len(t_bufs) >= 3:
                 # one sync for the TERMA boxes of this request
@@ -350,11 +366,9 @@ class BatchEngine:
             ev = Event().record(s2)
             s.wait(ev)
             events.append(ev)
-        elif self.method == "fft":
-            E.convolve_fft_batch(jobs, stream=s)
-        else:
-            E.convolve_batch(jobs, stream=s)
-            E.convolve_groups(groups, stream=s)
+        E.convolve_fft_batch(fft_jobs, stream=s)
+        E.convolve_batch(jobs, stream=s)
+        E.convolve_groups(groups, stream=s)
         staged = None
         if stage:
             staged = []

@@ -176,3 +176,28 @@ def test_batch_engine_fft_method(engine):
         assert np.array_equal(f, a)
         assert np.array_equal(f, o)   # chunks on a companion stream
     fft_be.close()
+
+
+def test_batch_engine_large_axis_takes_the_gather(engine, kern):
+    """A grid whose frame would exceed 1024 points (an axis of 520 voxels)
+    is convolved by the gather inside an FFT-form batch: the same bits as a
+    gather engine, next to a request the FFT form takes."""
+    from conehead_amd import workloads as W
+    from conehead_amd.batch import BatchEngine, DoseRequest
+    from conehead_amd.device import Stream
+    small = W.uneven(n_phantoms=1, seed=3, max_beams=1, xy=(16, 20),
+                     z=(14, 16), n_theta=4, n_phis=(4,))[0]
+    big = W.uneven(n_phantoms=1, seed=4, max_beams=1, xy=(16, 20),
+                   z=(14, 16), n_theta=4, n_phis=(4,))[0]
+    dens = np.ones((520, 12, 10), np.float32)
+    req_big = DoseRequest(dens, big.origin, big.spacing, big.beams,
+                          big.cones)
+    req_small = DoseRequest.from_case(small)
+    s = Stream()
+    fft_be = BatchEngine(engine, method="fft")
+    got = fft_be.run([req_small, req_big], s)
+    want_big = BatchEngine(engine, method="gather").run([req_big], s)[0]
+    want_small = fft_be.run([req_small], s)[0]
+    assert np.array_equal(got[1], want_big)
+    assert np.array_equal(got[0], want_small)
+    fft_be.close()

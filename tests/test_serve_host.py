@@ -6,7 +6,8 @@ import numpy as np
 import pytest
 
 from conehead_amd import workloads as W
-from conehead_amd.batch import DoseRequest, group_beams, spectral_frame
+from conehead_amd.batch import (DoseRequest, group_beams, spectral_frame,
+                                 spectral_ok)
 from conehead_amd.serve import AdmissionQueue, poisson_arrivals
 
 
@@ -51,6 +52,7 @@ def test_spectral_frame():
     assert spectral_frame((200, 200, 160)) == ((400, 400, 320), 176)
     assert spectral_frame((24, 26, 22)) == ((48, 54, 45), 32)
     assert spectral_frame((20, 13, 1)) == ((40, 25, 1), 16)
+    assert spectral_ok((512, 64, 64)) and not spectral_ok((513, 64, 64))
 
 
 def test_admission_fifo_and_budget():
