@@ -368,6 +368,118 @@ __device__ __forceinline__ void conv_stage(const float2* __restrict__ src,
     }
 }
 
+// In-place forms for a single LDS buffer, when every stage gives each
+// thread at most one butterfly group ((F / R) * W <= kFftThreads): read
+// into registers, barrier, write back.  Every thread reaches the barrier.
+template <int R, bool INV, int W>
+__device__ __forceinline__ void fft_stage_ip(float2* __restrict__ buf, int F,
+                                             int Ns,
+                                             const float2* __restrict__ tw)
+{
+    const int nb = F / R;
+    const int twstep = F / (Ns * R);
+    const int w = threadIdx.x % W;
+    const int j = threadIdx.x / W;
+    const bool act = j < nb;
+    const int jm = (Ns & (Ns - 1)) == 0 ? (j & (Ns - 1)) : (j % Ns);
+    float2 v[R];
+    if (act) {
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            v[r] = buf[(j + r * nb) * W + w];
+        if (Ns > 1) {
+            const int step = jm * twstep;
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                float2 t = tw[r * step];
+                if (INV)
+                    t.y = -t.y;
+                v[r] = cmul(v[r], t);
+            }
+        }
+        dft<INV>(v);
+    }
+    __syncthreads();
+    if (act) {
+        const int d0 = (j - jm) * R + jm;
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            buf[(d0 + r * Ns) * W + w] = v[r];
+    }
+}
+
+template <int R, int W, class KLoad>
+__device__ __forceinline__ void conv_stage_ip(float2* __restrict__ buf, int F,
+                                              const float2* __restrict__ tw,
+                                              const KLoad& kload)
+{
+    const int nb = F / R;
+    const int w = threadIdx.x % W;
+    const int j = threadIdx.x / W;
+    const bool act = j < nb;
+    float2 v[R];
+    if (act) {
+        float2 kv[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            kv[r] = kload(j + r * nb);
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            v[r] = buf[(j + r * nb) * W + w];
+#pragma unroll
+        for (int r = 1; r < R; ++r)
+            v[r] = cmul(v[r], tw[r * j]);
+        dft<false>(v);
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            v[r] = cmul(v[r], kv[r]);
+        dft<true>(v);
+    }
+    __syncthreads();
+    if (act) {
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            buf[(j * R + r) * W + w] = v[r];
+    }
+}
+
+template <int W, class KLoad>
+__device__ __forceinline__ void conv_any_ip(int R, float2* buf, int F,
+                                            const float2* tw,
+                                            const KLoad& kload)
+{
+    switch (R) {
+    case 16: conv_stage_ip<16, W>(buf, F, tw, kload); break;
+    case 8: conv_stage_ip<8, W>(buf, F, tw, kload); break;
+    case 4: conv_stage_ip<4, W>(buf, F, tw, kload); break;
+    case 2: conv_stage_ip<2, W>(buf, F, tw, kload); break;
+    case 3: conv_stage_ip<3, W>(buf, F, tw, kload); break;
+    default: conv_stage_ip<5, W>(buf, F, tw, kload); break;
+    }
+}
+
+// In-place LDS stages fac[s], s = s0, s0 + step, ... (stopping before s1);
+// each ends with a barrier.
+template <bool INV, int W>
+__device__ void stages_range_ip(float2* buf, const float2* tw, int F,
+                                const int32_t* fac, int s0, int s1, int step,
+                                int& Ns)
+{
+    for (int s = s0; s != s1; s += step) {
+        const int R = fac[s];
+        switch (R) {
+        case 16: fft_stage_ip<16, INV, W>(buf, F, Ns, tw); break;
+        case 8: fft_stage_ip<8, INV, W>(buf, F, Ns, tw); break;
+        case 4: fft_stage_ip<4, INV, W>(buf, F, Ns, tw); break;
+        case 2: fft_stage_ip<2, INV, W>(buf, F, Ns, tw); break;
+        case 3: fft_stage_ip<3, INV, W>(buf, F, Ns, tw); break;
+        default: fft_stage_ip<5, INV, W>(buf, F, Ns, tw); break;
+        }
+        __syncthreads();
+        Ns *= R;
+    }
+}
+
 template <bool INV, int W, class Load>
 __device__ __forceinline__ void first_any(int R, const Load& load,
                                           float2* dst, int F)
@@ -518,7 +630,8 @@ __device__ __forceinline__ LinePair line_pair(int64_t p, int32_t X,
 template <int W>
 __global__ __launch_bounds__(kFftThreads) void z_fwd_kernel(
     BeamIO io, float2* __restrict__ A, int32_t X, int32_t Y, int32_t Z,
-    int32_t a_rows, int32_t kp, FftLen L, const float2* __restrict__ g_tw)
+    int32_t a_rows, int32_t kp, FftLen L, const float2* __restrict__ g_tw,
+    int32_t inplace)
 {
     extern __shared__ float2 sm[];
     const int F = L.n;
@@ -551,8 +664,12 @@ __global__ __launch_bounds__(kFftThreads) void z_fwd_kernel(
         first_any<false, W>(L.fac[0], load, s_a, F);
         __syncthreads();
         int Ns = L.fac[0];
-        const float2* r =
-            stages_range<false, W>(s_a, s_b, s_tw, F, L.fac, 1, L.nfac, 1, Ns);
+        const float2* r = s_a;
+        if (inplace)
+            stages_range_ip<false, W>(s_a, s_tw, F, L.fac, 1, L.nfac, 1, Ns);
+        else
+            r = stages_range<false, W>(s_a, s_b, s_tw, F, L.fac, 1, L.nfac, 1,
+                                       Ns);
         if (!lp.valid)
             return;
         for (int k = n0; k < K; k += kStep) {
@@ -606,7 +723,7 @@ template <int W>
 __global__ __launch_bounds__(kFftThreads) void z_inv_kernel(
     BeamIO io, const float2* __restrict__ A, int32_t X, int32_t Y,
     int32_t Z, int32_t a_rows, int32_t kp, FftLen L,
-    const float2* __restrict__ g_tw)
+    const float2* __restrict__ g_tw, int32_t inplace)
 {
     extern __shared__ float2 sm[];
     const int F = L.n;
@@ -649,8 +766,13 @@ __global__ __launch_bounds__(kFftThreads) void z_inv_kernel(
         first_any<true, W>(L.fac[0], load, s_a, F);
         __syncthreads();
         int Ns = L.fac[0];
-        const float2* cur = stages_range<true, W>(s_a, s_b, s_tw, F, L.fac, 1,
-                                                  L.nfac - 1, 1, Ns);
+        const float2* cur = s_a;
+        if (inplace)
+            stages_range_ip<true, W>(s_a, s_tw, F, L.fac, 1, L.nfac - 1, 1,
+                                     Ns);
+        else
+            cur = stages_range<true, W>(s_a, s_b, s_tw, F, L.fac, 1,
+                                        L.nfac - 1, 1, Ns);
         auto store = [&](int n, float2 c) {
             if (!lp.valid || n >= Z)
                 return;
@@ -727,7 +849,7 @@ __global__ __launch_bounds__(kFftThreads) void col_kernel(
     int64_t outer_stride, int32_t n_cols, int32_t n_in, int32_t n_out,
     FftLen L, const float2* __restrict__ g_tw,
     const float2* __restrict__ khat, int64_t k_row_stride,
-    int64_t k_outer_stride)
+    int64_t k_outer_stride, int32_t inplace)
 {
     constexpr int kPer = MODE == kColConv ? kPerThreadConv : kPerThread;
     extern __shared__ float2 sm[];
@@ -760,6 +882,31 @@ __global__ __launch_bounds__(kFftThreads) void col_kernel(
         __syncthreads();
         int Ns = L.fac[0];
         float2* cur;
+        if (inplace) {
+            // one LDS buffer (the host allotted F * W elements)
+            if (MODE == kColConv) {
+                stages_range_ip<false, W>(s_a, s_tw, F, L.fac, 1, nf - 1, 1,
+                                          Ns);
+                const float2* kb =
+                    khat + blk.y * k_outer_stride + k0 + w;
+                auto kload = [&](int k) -> float2 {
+                    return col_ok ? kb[k * k_row_stride]
+                                  : make_float2(0.0f, 0.0f);
+                };
+                conv_any_ip<W>(L.fac[nf - 1], s_a, F, s_tw, kload);
+                __syncthreads();
+                Ns = L.fac[nf - 1];
+                stages_range_ip<true, W>(s_a, s_tw, F, L.fac, nf - 2, 0, -1,
+                                         Ns);
+                last_any<true, W>(L.fac[0], s_a, F, s_tw, store);
+            } else {
+                stages_range_ip<MODE == kColInv, W>(s_a, s_tw, F, L.fac, 1,
+                                                    nf - 1, 1, Ns);
+                last_any<MODE == kColInv, W>(L.fac[nf - 1], s_a, F, s_tw,
+                                             store);
+            }
+            return;
+        }
         if (MODE == kColConv) {
             cur = stages_range<false, W>(s_a, s_b, s_tw, F, L.fac, 1, nf - 1,
                                          1, Ns);
@@ -864,6 +1011,16 @@ size_t col_lds_bytes(int F, int W)
     return (size_t)(F + 2 * F * W) * sizeof(float2);
 }
 
+// One LDS buffer suffices when the fused path runs (two stages or more)
+// and every stage gives each thread at most one butterfly group.
+bool single_buffer(const FftLen& L, int W)
+{
+    bool ip = L.nfac >= 2;
+    for (int i = 0; i < L.nfac; ++i)
+        ip = ip && (L.n / L.fac[i]) * W <= kFftThreads;
+    return ip;
+}
+
 template <typename K>
 ch_status allow_lds(K kernel, size_t bytes)
 {
@@ -899,15 +1056,17 @@ ch_status launch_z_fwd(const BeamIO& io, float2* A, int X, int Y, int Z,
                        int a_rows, int kp, const FftLen& L, const float2* tw,
                        hipStream_t s)
 {
-    const size_t lds = z_lds_bytes(L.n, W);
-    ch_status st = allow_lds(z_fwd_kernel<W>, lds);
+    const bool ip = single_buffer(L, W);
+    const size_t lds = ip ? (size_t)(L.n + L.n * W) * sizeof(float2)
+                          : z_lds_bytes(L.n, W);
+    ch_status st = allow_lds(z_fwd_kernel<W>, z_lds_bytes(L.n, W));
     if (st != CH_OK)
         return st;
     const int64_t groups = ((int64_t)X * ((Y + 1) / 2) + W - 1) / W;
     hipLaunchKernelGGL(z_fwd_kernel<W>,
                        dim3((unsigned)(groups * io.n_beams)),
                        dim3(kFftThreads), lds, s, io, A, X, Y, Z, a_rows, kp,
-                       L, tw);
+                       L, tw, (int32_t)ip);
     CH_HIP(hipGetLastError());
     return CH_OK;
 }
@@ -917,15 +1076,17 @@ ch_status launch_z_inv(const BeamIO& io, const float2* A, int X, int Y,
                        int Z, int a_rows, int kp, const FftLen& L,
                        const float2* tw, hipStream_t s)
 {
-    const size_t lds = z_lds_bytes(L.n, W);
-    ch_status st = allow_lds(z_inv_kernel<W>, lds);
+    const bool ip = single_buffer(L, W);
+    const size_t lds = ip ? (size_t)(L.n + L.n * W) * sizeof(float2)
+                          : z_lds_bytes(L.n, W);
+    ch_status st = allow_lds(z_inv_kernel<W>, z_lds_bytes(L.n, W));
     if (st != CH_OK)
         return st;
     const int64_t groups = ((int64_t)X * ((Y + 1) / 2) + W - 1) / W;
     hipLaunchKernelGGL(z_inv_kernel<W>,
                        dim3((unsigned)(groups * io.n_beams)),
                        dim3(kFftThreads), lds, s, io, A, X, Y, Z, a_rows, kp,
-                       L, tw);
+                       L, tw, (int32_t)ip);
     CH_HIP(hipGetLastError());
     return CH_OK;
 }
@@ -938,8 +1099,12 @@ ch_status launch_col(const BeamIO& io, float2* A, int64_t row_stride,
                      int64_t k_outer_stride, hipStream_t s,
                      LaunchTimer* timer)
 {
-    const size_t lds = col_lds_bytes(L.n, W);
-    ch_status st = allow_lds(col_kernel<W, MODE>, lds);
+    // one LDS buffer when the fused path runs and every stage gives each
+    // thread at most one butterfly group
+    const bool ip = single_buffer(L, W);
+    const size_t lds = ip ? (size_t)(L.n + L.n * W) * sizeof(float2)
+                          : col_lds_bytes(L.n, W);
+    ch_status st = allow_lds(col_kernel<W, MODE>, col_lds_bytes(L.n, W));
     if (st != CH_OK)
         return st;
     const int64_t groups = (n_cols + W - 1) / W;
@@ -948,12 +1113,12 @@ ch_status launch_col(const BeamIO& io, float2* A, int64_t row_stride,
         timer->launch(col_kernel<W, MODE>, grid, dim3(kFftThreads),
                       (std::uint32_t)lds, io, A, row_stride, outer_stride,
                       n_cols, n_in, n_out, L, tw, khat, k_row_stride,
-                      k_outer_stride);
+                      k_outer_stride, (int32_t)ip);
     else
         hipLaunchKernelGGL((col_kernel<W, MODE>), grid, dim3(kFftThreads),
                            lds, s, io, A, row_stride, outer_stride, n_cols,
                            n_in, n_out, L, tw, khat, k_row_stride,
-                           k_outer_stride);
+                           k_outer_stride, (int32_t)ip);
     CH_HIP(hipGetLastError());
     return CH_OK;
 }
