@@ -34,9 +34,10 @@ namespace {
 
 constexpr int kFftThreads = 256;
 constexpr int kFftMax = 1024;      // longest transform
-// F * W per ping-pong buffer: the convolution pass takes up to 16 lines of
-// 256 (its radix-16 stages then keep all 256 threads busy and K^ is read in
-// 128-byte rows); the other passes run faster with half of that.
+// F * W per LDS buffer: the convolution pass takes up to 16 lines of 256
+// (its radix-16 stages then keep all 256 threads busy and K^ is read in
+// 128-byte rows); the other passes run faster with half of that.  Two
+// buffers (ping-pong) unless single_buffer() allows one.
 constexpr int kFftLdsElemsConv = 4096;
 constexpr int kFftLdsElems = 2048;
 // Elements of one line group per thread in the unfused path (transforms
@@ -582,11 +583,10 @@ __device__ __forceinline__ int group_of_block(const BeamIO& io)
 }
 
 // Workgroups go to the 8 XCDs round-robin by linear id, and each XCD has
-// its own L2.  The column passes take their work in XCD-major order
-// instead: XCD k runs a contiguous range of logical blocks, so the two
-// halves of a 128-byte line (neighbouring 8-column groups) and the beams
-// that read the same K^ chunk meet in one L2.  The work per block is
-// uniform, so the ranges stay balanced.
+// its own L2.  The y passes take their work in XCD-major order instead:
+// XCD k runs a contiguous range of logical blocks, so the two halves of a
+// 128-byte line (neighbouring 8-column groups) meet in one L2.  The work
+// per block is uniform, so the ranges stay balanced.
 constexpr unsigned kXcds = 8;
 struct BlockXY {
     unsigned x, y;
@@ -839,10 +839,12 @@ __global__ __launch_bounds__(kFftThreads) void z_inv_kernel(
 enum ColMode { kColFwd = 0, kColInv = 1, kColConv = 2 };
 
 // Transforms along a strided axis: W adjacent columns (kz .. kz + W - 1) of
-// the rows n = 0 .. F-1 at base + n * row_stride, base = A + blockIdx.y *
-// outer_stride + W * group.  Rows n >= n_in read as zero; rows n < n_out
-// are written back in place.  kColConv multiplies by K^ (same layout, its
-// own strides) between the forward and the inverse transform.
+// the rows n = 0 .. F-1 at base + n * row_stride, base = A + outer *
+// outer_stride + W * group, with (group, beam, outer) from the block id
+// (XCD-major for the y passes).  Rows n >= n_in read as zero; rows n <
+// n_out are written back in place.  kColConv multiplies by K^ (same
+// layout, its own strides) between the forward and the inverse transform.
+// `inplace`: the host allotted one LDS buffer (single_buffer()).
 template <int W, int MODE>
 __global__ __launch_bounds__(kFftThreads) void col_kernel(
     BeamIO io, float2* __restrict__ A, int64_t row_stride,
