@@ -183,6 +183,9 @@ def run_hetero(args, eng):
         tile = tuple(int(v) for v in args.tile.split("x"))
         plan.set_tile(*tile)
 
+    # the FFT form needs no support box: its TERMA skips computing one
+    beam_fft = E.beam_struct(src, st, d_r, d_rho, d_t, None)
+
     def conv_fn(method):
         if method == "gather":
             return lambda: E.convolve(plan, d_t, d_d, d_box, stream=s)
@@ -196,8 +199,9 @@ def run_hetero(args, eng):
         the gather, or the FFT form's convolution pass; then the same step
         captured once as a HIP graph and replayed K times."""
         conv = conv_fn(method)
+        bm = beam_fft if method == "fft" else beam
         for _ in range(args.warmup):
-            E.terma([beam], shape, case.origin, case.spacing, stream=s,
+            E.terma([bm], shape, case.origin, case.spacing, stream=s,
                     check=False)
             conv()
         s.sync()
@@ -211,7 +215,7 @@ def run_hetero(args, eng):
         t_start = time.perf_counter()
         ev[0].record(s)
         for i in range(K):
-            E.terma([beam], shape, case.origin, case.spacing, stream=s,
+            E.terma([bm], shape, case.origin, case.spacing, stream=s,
                     check=False)
             g0[i].record(s)
             arm_launch_events(k0[i], k1[i])
@@ -231,7 +235,7 @@ def run_hetero(args, eng):
              "gpu_total_ms": ev[0].elapsed_ms(ev[K]),
              "dose": d_d.download(stream=s)}
         with E.Graph.capture(s) as cap:
-            E.terma([beam], shape, case.origin, case.spacing, stream=s,
+            E.terma([bm], shape, case.origin, case.spacing, stream=s,
                     check=False)
             conv()
         graph = cap.graph

@@ -311,8 +311,12 @@ class BatchEngine:
             plan = eng.plan(req.shape, req.spacing, req.cones)
             beams, outs = [], []
             n_jobs = req.n_jobs
+            # grids beyond the FFT form's frames take the gather
+            fft_here = self.method == "fft" and spectral_ok(req.shape)
             t_bufs = [self.pool.get(req.shape) for _ in range(n_jobs)]
-            boxes = [self.pool.get((8,), np.int32) for _ in range(n_jobs)]
+            # support boxes only for the gather, which clips to them
+            boxes = [None if fft_here else self.pool.get((8,), np.int32)
+                     for _ in range(n_jobs)]
             for b, (src, blk, st) in enumerate(req.beams):
                 j = b if req.per_beam else 0
                 bs = E.beam_struct(src, st, eng.aperture(blk, s), d_rho,
@@ -321,8 +325,6 @@ class BatchEngine:
                 beams.append(bs)
             for j in range(n_jobs):
                 outs.append(self.pool.get(req.shape))
-            # grids beyond the FFT form's frames take the gather
-            fft_here = self.method == "fft" and spectral_ok(req.shape)
             if s2 is not None and fft_here:
                 # a chunk's convolutions start as soon as its TERMA is done,
                 # beside the next chunk's TERMA
@@ -342,7 +344,7 @@ class BatchEngine:
                         check=False)
                 pending.append((plan, t_bufs, outs, boxes, fft_here))
             temps.extend(t_bufs)
-            temps.extend(boxes)
+            temps.extend(b for b in boxes if b is not None)
             outputs.append(outs)
         fft_jobs = []
         for plan, t_bufs, outs, boxes, fft_here in pending:

@@ -33,6 +33,7 @@ import numpy as np
 
 from . import engine as _engine
 from .cones import ConeSet, beam_frame, default_thetas, kernel_phis
+from .batch import spectral_ok
 from .device import DeviceArray, Stream
 from .dosegrid import DoseGrid
 from .kernel import EdkSet, merge_polar
@@ -126,22 +127,25 @@ class Conehead:
         cones = ConeSet(thetas, phis, kernel, semantics=semantics,
                         frame=beam_frame(source) if physics else None)
 
+        method = settings.get("convolution", "fft")
+        if method not in ("fft", "gather"):
+            raise ValueError("settings['convolution'] must be 'fft' or "
+                             "'gather'")
+        # the FFT form takes axes of up to 512 voxels; beyond, the gather
+        use_fft = method == "fft" and not physics and spectral_ok(shape)
         d_terma = DeviceArray(shape, np.float32)
-        d_box = DeviceArray((6,), np.int32)
+        # the support box only serves the gathers, which clip to it
+        d_box = None if use_fft else DeviceArray((6,), np.int32)
         d_raster = eng.aperture(block, s)
         beam = _engine.beam_struct(source, settings, d_raster, d_rho,
                                    d_terma, d_box)
         _engine.terma([beam], shape, origin, spacing, stream=s)
         plan = eng.plan(shape, spacing, cones)
         d_dose = DeviceArray(shape, np.float32)
-        method = settings.get("convolution", "fft")
-        if method not in ("fft", "gather"):
-            raise ValueError("settings['convolution'] must be 'fft' or "
-                             "'gather'")
         if physics:
             _engine.convolve_rad(plan, d_terma, d_rho, d_dose, d_box,
                                  stream=s)
-        elif method == "fft":
+        elif use_fft:
             _engine.convolve_fft(plan, d_terma, d_dose, stream=s)
         else:
             _engine.convolve(plan, d_terma, d_dose, d_box, stream=s)
