@@ -546,7 +546,9 @@ def run_uneven(args, eng):
     r = {"metric": METRIC, "value": nb / (ms * 1e-3), "unit": "beams/s",
          "n_gpus": 1, "steps": steps, "warmup": 1,
          "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
-         "vs_baseline": None, "dtype": "f32 (f64 DDA alphas)",
+         "vs_baseline": None, "dtype": "f32",
+         "precision": "FP32 arithmetic; FP64 DDA alphas and TERMA "
+                      "geometry",
          "data": "synthetic (seeded phantoms; EDKnrc varian_6 kernel)",
          "config": {"workload": "uneven batch: 32 phantoms (seed 7), dims "
                                 "64-200 x 64-200 x 64-160, 2-5 mm voxels, "
@@ -623,7 +625,9 @@ def run_serve(args, eng):
             "unit": "beams/s", "n_gpus": 1, "steps": n_req * len(sweep),
             "warmup": 1, "ms_per_step": None, "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None,
-            "dtype": "f32 (f64 DDA alphas)",
+            "dtype": "f32",
+            "precision": "FP32 arithmetic; FP64 DDA alphas and TERMA "
+                         "geometry",
             "data": "synthetic (seeded phantoms; EDKnrc varian_6 kernel)",
             "config": {"workload": "serving: Poisson arrivals (seed 99), "
                                    "64 requests per lambda drawn from the "
@@ -665,7 +669,9 @@ def main():
                 "unit": "beams/s", "n_gpus": 1, "steps": args.steps,
                 "warmup": args.warmup, "ms_per_step": h.pop("ms_per_step"),
                 "higher_is_better": True, "scaling": "weak",
-                "vs_baseline": None, "dtype": "f32 (f64 DDA alphas)",
+                "vs_baseline": None, "dtype": "f32",
+                "precision": "FP32 arithmetic; FP64 DDA alphas and TERMA "
+                             "geometry",
                 "data": "synthetic (seeded phantom; EDKnrc varian_6 kernel)",
                 "config": h.pop("config")}
         line.update(h)
