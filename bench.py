@@ -603,11 +603,11 @@ def run_serve(args, eng):
     # beyond the sweep: an arrival rate the server cannot keep up with, so
     # beams/s is its capacity, with the doses delivered to the host and
     # with them left in HBM (consumers on the GPU)
-    lam = 2000
+    lam, n_sat = 2000, 256      # a longer burst: steadier than 64 requests
     saturation = {}
     for download in (True, False):
-        reqs = [pool[int(i)] for i in rng.integers(0, len(pool), n_req)]
-        arr = poisson_arrivals(float(lam), n_req, seed=99 + lam)
+        reqs = [pool[int(i)] for i in rng.integers(0, len(pool), n_sat)]
+        arr = poisson_arrivals(float(lam), n_sat, seed=99 + lam)
         with DoseServer(be, max_inflight=2, max_batch_jobs=64,
                         download=download) as srv:
             futs = replay(srv, reqs, arr)
