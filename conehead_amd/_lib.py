@@ -167,6 +167,7 @@ SIGNATURES = {
                                      c_vp]),
     "ch_convolve_f64": (ctypes.c_int32, [c_vp, c_vp, c_vp, c_vp, c_vp]),
     "ch_plan_spectrum": (ctypes.c_int32, [c_vp, c_vp]),
+    "ch_plan_spectrum_release": (ctypes.c_int32, [c_vp]),
     "ch_plan_spectrum_info": (ctypes.c_int32, [c_vp, c_i32p, c_i32p,
                                                c_i64p]),
     "ch_convolve_fft": (ctypes.c_int32, [c_vp, c_vp, c_vp, ctypes.c_int32,

@@ -224,7 +224,8 @@ class BatchEngine:
     0 runs all TERMA, then all convolutions, on the batch's stream."""
 
     def __init__(self, engine=None, pool=None, group_beams=False,
-                 host_pool=None, method=None, pipeline=4):
+                 host_pool=None, method=None, pipeline=4,
+                 spectrum_budget=32 << 30):
         if method is None:
             method = "gather" if group_beams else "fft"
         if method not in ("gather", "fft"):
@@ -240,6 +241,9 @@ class BatchEngine:
         # stream, each chunk's convolution on a companion stream behind it
         # (0: all TERMA, then all convolutions, on the one stream)
         self.pipeline = int(pipeline) if method == "fft" else 0
+        # HBM the cached plans' kernel spectra may hold (least recently used
+        # ones are freed first, never those of the batch being launched)
+        self.spectrum_budget = int(spectrum_budget)
         self._companions = {}
         self._resident = {}
         self._default = None
@@ -304,6 +308,9 @@ class BatchEngine:
         jobs, groups, outputs, temps, pending = [], [], [], [], []
         s2 = self._companion(s) if self.pipeline > 0 else None
         events = []
+        if self.method == "fft":
+            keep = [eng.plan(r.shape, r.spacing, r.cones) for r in requests]
+            eng.trim_spectra(self.spectrum_budget, keep=keep)
         for req in requests:
             d_rho, own = self.densities(req, s)
             if own:

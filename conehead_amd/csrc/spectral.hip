@@ -1479,6 +1479,19 @@ ch_status ch_plan_spectrum(ch_plan* plan, void* stream)
     return st;
 }
 
+ch_status ch_plan_spectrum_release(ch_plan* plan)
+{
+    CH_REQUIRE(plan, "ch_plan_spectrum_release: null plan");
+    if (!plan->d_khat)
+        return CH_OK;
+    // kernels queued on any stream may still read K^
+    CH_HIP(hipDeviceSynchronize());
+    spectrum_free(plan);
+    plan->fft_n[0] = plan->fft_n[1] = plan->fft_n[2] = 0;
+    plan->fft_kp = 0;
+    return CH_OK;
+}
+
 ch_status ch_plan_spectrum_info(const ch_plan* plan, int32_t* frame3,
                                 int32_t* pitch, int64_t* work_bytes)
 {

@@ -5,6 +5,7 @@ Everything that is reused across beams and requests stays in HBM: cone
 plans (tap tables) keyed by (grid shape, voxel size, cone set), aperture
 rasters keyed by their recipe, phantom densities keyed by the caller.
 """
+import collections
 import ctypes
 import hashlib
 import threading
@@ -85,7 +86,13 @@ class Plan:
         work = ctypes.c_int64()
         _lib.check(_lib.lib().ch_plan_spectrum_info(
             self.handle, f, ctypes.byref(pitch), ctypes.byref(work)))
+        self.spectrum_bytes = 8 * f[0] * f[1] * pitch.value
         return tuple(f), pitch.value, work.value
+
+    def release_spectrum(self):
+        """Frees the kernel spectrum (HBM); the next FFT use rebuilds it."""
+        _lib.check(_lib.lib().ch_plan_spectrum_release(self.handle))
+        self.spectrum_bytes = 0
 
     def close(self):
         if self.handle:
@@ -337,9 +344,27 @@ class Engine:
 
     def __init__(self, device=0):
         self.info = _lib.init(device)
-        self._plans = {}
+        self._plans = collections.OrderedDict()   # least recently used first
         self._rasters = {}
         self._lock = threading.RLock()
+
+    def trim_spectra(self, budget_bytes, keep=()):
+        """Frees the kernel spectra of the least recently used plans, other
+        than `keep`, until the cached spectra hold at most budget_bytes of
+        HBM (75 MB per 128^3 plan, about 225 MB at 200 x 200 x 160).
+        Returns the bytes still held."""
+        with self._lock:
+            keep = {id(p) for p in keep}
+            held = sum(getattr(p, "spectrum_bytes", 0)
+                       for p in self._plans.values())
+            for p in list(self._plans.values()):
+                if held <= budget_bytes:
+                    break
+                b = getattr(p, "spectrum_bytes", 0)
+                if b and id(p) not in keep:
+                    p.release_spectrum()
+                    held -= b
+            return held
 
     # -- plans -----------------------------------------------------------
     def plan(self, shape, voxel_size, cones, tile=None, accum=None):
@@ -351,8 +376,10 @@ class Engine:
                 from .autotune import default_tile
                 p.set_tile(*(tile or default_tile(p)))
                 self._plans[key] = p
-            elif tile is not None:
-                p.set_tile(*tile)
+            else:
+                self._plans.move_to_end(key)
+                if tile is not None:
+                    p.set_tile(*tile)
             if accum is not None:
                 p.set_accum(accum)
             return p

@@ -181,6 +181,10 @@ ch_status ch_convolve_batch(int32_t n_jobs, const ch_conv_job* h_jobs,
  * are refused: their kernel distance depends on the densities.  No
  * reference counterpart: it precomputes what ch_convolve_fft needs. */
 ch_status ch_plan_spectrum(ch_plan* plan, void* stream);
+/* Frees the plan's kernel spectrum (waits for the device first: queued
+ * kernels may read it); ch_plan_spectrum builds it again.  No reference
+ * counterpart: it bounds the HBM held by cached plans. */
+ch_status ch_plan_spectrum_release(ch_plan* plan);
 /* Frame lengths, half-spectrum row pitch and the work-grid bytes one beam
  * uses (held per stream) of a plan whose spectrum is built. */
 ch_status ch_plan_spectrum_info(const ch_plan* plan, int32_t* frame3,

@@ -224,3 +224,45 @@ def test_uneven_workload_fft_equals_gather(engine):
             for b in range(len(f)):
                 worst = max(worst, rel_err(f[b], g[b]))
     assert worst < TOL, worst
+
+
+def test_spectrum_budget(engine, kern):
+    """Cached plans' spectra are freed least recently used first, never
+    those a batch is about to use, and rebuilt on the next use with the same
+    bits."""
+    from conehead_amd import engine as Emod
+    from conehead_amd.batch import BatchEngine, DoseRequest
+    from conehead_amd.device import Stream
+    from conehead_amd import workloads as W
+    eng = Emod.Engine()
+    cs = cone_set(kern, 4, 4)
+    rng = np.random.default_rng(12)
+    shapes = [(20, 22, 24), (24, 20, 22), (22, 24, 20)]
+    plans, Ts, first = [], [], []
+    for shp in shapes:
+        p = eng.plan(shp, (0.4, 0.4, 0.4), cs)
+        T = beam_like(shp, rng)
+        plans.append(p)
+        Ts.append(T)
+        first.append(run_fft(p, T))
+    one = plans[0].spectrum_bytes
+    assert one > 0
+    eng.plan(shapes[0], (0.4, 0.4, 0.4), cs)        # plan 0 most recent
+    # LRU order: plan 1, plan 2, plan 0; the budget of one plan 0 frees
+    # plan 1, skips plan 2 (kept) and frees plan 0
+    held = eng.trim_spectra(one, keep=[plans[2]])
+    assert plans[1].spectrum_bytes == 0 and plans[0].spectrum_bytes == 0
+    assert plans[2].spectrum_bytes > 0              # kept despite the budget
+    assert held == plans[2].spectrum_bytes
+    for p, T, d in zip(plans, Ts, first):           # rebuilt, same bits
+        assert np.array_equal(run_fft(p, T), d)
+    c = W.uneven(n_phantoms=2, seed=5, max_beams=2, xy=(16, 20), z=(14, 16),
+                 n_theta=4, n_phis=(4,))
+    reqs = [DoseRequest.from_case(x) for x in c]
+    s = Stream()
+    tight = BatchEngine(eng, method="fft", spectrum_budget=1)
+    got = tight.run(reqs, s)
+    want = BatchEngine(eng, method="fft").run(reqs, s)
+    for a, b in zip(got, want):
+        assert np.array_equal(a, b)
+    tight.close()
