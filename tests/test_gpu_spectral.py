@@ -192,6 +192,8 @@ def test_batch_engine_large_axis_takes_the_gather(engine, kern):
     dens = np.ones((520, 12, 10), np.float32)
     req_big = DoseRequest(dens, big.origin, big.spacing, big.beams,
                           big.cones)
+    with pytest.raises(RuntimeError, match="transform"):
+        engine.plan(dens.shape, big.spacing, big.cones).spectrum()
     req_small = DoseRequest.from_case(small)
     s = Stream()
     fft_be = BatchEngine(engine, method="fft")
