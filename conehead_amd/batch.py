@@ -1,21 +1,28 @@
 """Batched dose engine: many phantoms, beams and cone sets per launch.
 
 A DoseRequest is one phantom (densities on its dose grid), a list of beams
-and a cone set.  A batch of requests runs as one TERMA launch per beam and
-one gather launch over every convolution job of the batch (jobs of
-different grids, voxel sizes and cone sets share it; ch_convolve_batch),
-plus one launch for the beams convolved in interleaved groups (beams of a
-request whose support boxes nearly coincide; ch_convolve_groups).
+and a cone set.  A batch of requests runs one TERMA launch per beam and
+then the convolutions:
+  method="fft" (default)  the FFT form, ch_convolve_fft_batch: five passes
+                          per beam, the jobs on one plan sharing each
+                          launch; TERMA chunks overlap the previous chunk's
+                          passes on a companion stream;
+  method="gather"         one gather launch over every job of the batch
+                          (jobs of different grids, voxel sizes and cone
+                          sets share it; ch_convolve_batch), plus one
+                          launch for the beams convolved in interleaved
+                          groups (beams of a request whose support boxes
+                          nearly coincide; ch_convolve_groups).
 Two output modes:
-  per_beam=True   one dose grid per beam (each beam is its own gather job);
+  per_beam=True   one dose grid per beam (each beam is its own job);
   per_beam=False  the plan dose of all beams: their TERMA grids are summed
                   (ch_terma accumulate mode) and convolved once -- exact,
                   because convolve_c is linear in TERMA and its cones are
                   fixed to the grid (conehead/convolve_c.pyx:101).
 Device buffers come from a size-keyed pool, densities can stay resident
-under a caller key, the gather is deterministic and groups are chosen per
-request from its own beams, so a request's result is bitwise the same
-alone or inside any batch.
+under a caller key, both convolutions are deterministic and groups are
+chosen per request from its own beams, so a request's result is bitwise
+the same alone or inside any batch.
 """
 import threading
 
@@ -297,12 +304,13 @@ class BatchEngine:
             c.close()
 
     def launch(self, requests, stream=None, stage=False):
-        """Queue TERMA and one gather launch for `requests` on `stream` (the
-        engine's default stream if None); returns a BatchHandle (outputs
-        valid once handle.done()).  stage=True also queues the copies of
-        the doses into pinned host buffers behind the gather, so they run
-        on the DMA engine while other streams compute and collect() hands
-        them out without a copy of its own."""
+        """Queue TERMA and the convolutions of `requests` on `stream` (the
+        engine's default stream if None; FFT-form chunks run on its
+        companion stream, joined back before the batch's event); returns a
+        BatchHandle (outputs valid once handle.done()).  stage=True also
+        queues the copies of the doses into pinned host buffers behind the
+        convolutions, so they run on the DMA engine while other streams
+        compute and collect() hands them out without a copy of its own."""
         s = stream or self._default_stream()
         eng = self.engine
         jobs, groups, outputs, temps, pending = [], [], [], [], []
