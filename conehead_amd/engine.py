@@ -1,9 +1,11 @@
 """Device-resident dose engine: plan store, aperture rasters, phantoms,
-TERMA and the collapsed-cone gather, all through the C-ABI.
+TERMA and the collapsed-cone convolution (the direct gather and the FFT
+form), all through the C-ABI.
 
 Everything that is reused across beams and requests stays in HBM: cone
-plans (tap tables) keyed by (grid shape, voxel size, cone set), aperture
-rasters keyed by their recipe, phantom densities keyed by the caller.
+plans (tap tables, and the kernel spectrum once the FFT form needs it)
+keyed by (grid shape, voxel size, cone set), aperture rasters keyed by
+their recipe, phantom densities keyed by the caller.
 """
 import collections
 import ctypes

@@ -8,8 +8,8 @@ HBM budget are admitted as the next batch -- so requests that arrive while
 others run join the next free slot and their work overlaps on the device.
 Admission is first-come first-served by the budget; a request that could
 never fit is rejected.  Every result is bitwise identical to the same
-request computed alone (the gather has no atomics and a fixed summation
-order), which tests/test_gpu_serve.py checks.
+request computed alone (both convolutions have no atomics and a fixed
+order of operations per voxel), which tests/test_gpu_serve.py checks.
 """
 import collections
 import threading
