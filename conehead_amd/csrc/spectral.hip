@@ -1254,20 +1254,45 @@ ch_status spectral_passes(const ch_plan* p, BeamIO io, float2* A,
     return st;
 }
 
-// Per-stream device table of a batch (TERMA pointers, dose pointers,
-// accumulate flags), staged through pinned memory that is rewritten only
-// after the event of its last copy.
-struct BeamTable {
-    hipStream_t stream;
+// Per-stream device tables of a batch (TERMA pointers, dose pointers,
+// accumulate flags), staged through pinned memory. Each stream owns a ring of
+// kTableSlots slots; a slot's pinned half is rewritten only after the event of
+// its previous copy, so the host waits only when it runs kTableSlots calls
+// ahead of the stream. The device half needs no wait: the next copy into it
+// is queued on the same stream behind the kernels that read it.
+constexpr int kTableSlots = 64;
+
+struct TableSlot {
     void* h;
     void* d;
-    size_t bytes;
     hipEvent_t copied;
+};
+
+// The slots of one stream share one pinned and one device block of
+// kTableSlots x slot_bytes, allocated on first use and regrown (after the
+// stream drains) only when a batch needs a larger table.
+struct BeamTable {
+    hipStream_t stream;
+    char* h = nullptr;
+    char* d = nullptr;
+    size_t slot_bytes = 0;
+    hipEvent_t copied[kTableSlots] = {};
+    int next = 0;
 };
 std::vector<BeamTable*> g_tabs;
 std::mutex g_tab_mu;
 
-ch_status beam_table(hipStream_t s, size_t bytes, BeamTable** out)
+void table_free(BeamTable* t)
+{
+    (void)hipFree(t->d);
+    (void)hipHostFree(t->h);
+    for (hipEvent_t e : t->copied)
+        if (e)
+            (void)hipEventDestroy(e);
+    delete t;
+}
+
+ch_status beam_table(hipStream_t s, size_t bytes, TableSlot* out)
 {
     BeamTable* t = nullptr;
     for (auto* x : g_tabs)
@@ -1276,26 +1301,30 @@ ch_status beam_table(hipStream_t s, size_t bytes, BeamTable** out)
     if (!t) {
         t = new BeamTable();
         t->stream = s;
-        t->h = nullptr;
-        t->d = nullptr;
-        t->bytes = 0;
-        CH_HIP(hipEventCreateWithFlags(&t->copied, hipEventDisableTiming));
         g_tabs.push_back(t);
+        for (hipEvent_t& e : t->copied)
+            CH_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     }
-    CH_HIP(hipEventSynchronize(t->copied));
-    if (t->bytes < bytes) {
+    if (t->slot_bytes < bytes) {
+        // queued copies and kernels of this stream may use the old blocks
         CH_HIP(hipStreamSynchronize(s));
         CH_HIP(hipFree(t->d));
         CH_HIP(hipHostFree(t->h));
         t->d = nullptr;
         t->h = nullptr;
-        t->bytes = 0;
-        const size_t cap = std::max(bytes, (size_t)4096);
-        CH_HIP(hipMalloc(&t->d, cap));
-        CH_HIP(hipHostMalloc(&t->h, cap, hipHostMallocDefault));
-        t->bytes = cap;
+        t->slot_bytes = 0;
+        const size_t cap = (std::max(bytes, (size_t)4096) + 255) & ~(size_t)255;
+        CH_HIP(hipMalloc((void**)&t->d, cap * kTableSlots));
+        CH_HIP(hipHostMalloc((void**)&t->h, cap * kTableSlots,
+                             hipHostMallocDefault));
+        t->slot_bytes = cap;
     }
-    *out = t;
+    const int k = t->next;
+    t->next = (k + 1) % kTableSlots;
+    CH_HIP(hipEventSynchronize(t->copied[k]));
+    out->h = t->h + (size_t)k * t->slot_bytes;
+    out->d = t->d + (size_t)k * t->slot_bytes;
+    out->copied = t->copied[k];
     return CH_OK;
 }
 
@@ -1312,10 +1341,7 @@ void ch::release_spectral_tables(hipStream_t s)
         if (t->stream != s)
             continue;
         (void)hipStreamSynchronize(s);
-        (void)hipFree(t->d);
-        (void)hipHostFree(t->h);
-        (void)hipEventDestroy(t->copied);
-        delete t;
+        table_free(t);
         g_tabs.erase(g_tabs.begin() + i);
         return;
     }
@@ -1567,9 +1593,10 @@ ch_status ch_convolve_fft_batch(int32_t n_jobs, const ch_conv_job* h_jobs,
                 stream_scratch(s, kScratchSpectrum, per * nb, &work);
             if (st != CH_OK)
                 return st;
-            BeamTable* t = nullptr;
+            TableSlot slot;
+            TableSlot* t = &slot;
             const size_t ptrs = (size_t)nb * sizeof(void*);
-            st = beam_table(s, 2 * ptrs + (size_t)nb * sizeof(int32_t), &t);
+            st = beam_table(s, 2 * ptrs + (size_t)nb * sizeof(int32_t), t);
             if (st != CH_OK)
                 return st;
             const float** ht = (const float**)t->h;
