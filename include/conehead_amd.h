@@ -201,7 +201,9 @@ ch_status ch_convolve_fft(const ch_plan* plan, const float* d_terma,
 /* ch_convolve_fft for many jobs (the ch_conv_job list of
  * ch_convolve_batch; d_box is not used): jobs on one plan share each pass's
  * launch, up to 4 GiB of work grids per launch.  Dose buffers must be
- * distinct within a call.  Stages a host table, so it cannot be captured. */
+ * distinct within a call.  Stages a host table, so it cannot be captured;
+ * the tables of a stream rotate through 64 pinned slots, so the call waits
+ * on the host only when it runs 64 calls ahead of the stream. */
 ch_status ch_convolve_fft_batch(int32_t n_jobs, const ch_conv_job* h_jobs,
                                 void* stream);
 
