@@ -28,7 +28,6 @@ import threading
 
 import numpy as np
 
-from . import _lib
 from . import engine as E
 from .device import DeviceArray, Event, HostPool, Stream, _h
 
@@ -407,8 +406,7 @@ class BatchEngine:
         batch view pinned buffers, which return to the host pool when the
         arrays are dropped."""
         handle.event.sync()
-        from . import _lib
-        _lib.check(_lib.lib().ch_terma_check(_h(handle.stream)))
+        E.terma_check(handle.stream)
         results = []
         staged = handle.staged or [None] * len(handle.requests)
         handle.staged = None

@@ -456,12 +456,17 @@ ch_status ch_terma(int32_t n_beams, const ch_beam* h_beams,
     CH_REQUIRE(nv < (1LL << 31), "ch_terma: grid too large");
     hipStream_t s = (hipStream_t)stream;
     // The geometry flag is per stream and covers every beam of this call.
+    // The geometry flag stays set until ch_terma_check reads it, so one
+    // check covers every call queued since the last one; it is cleared
+    // here only when the stream's flag is first allocated.
+    const bool fresh = stream_scratch_peek(s, kScratchStatus) == nullptr;
     void* status = nullptr;
     ch_status sst = stream_scratch(s, kScratchStatus, 256, &status);
     if (sst != CH_OK)
         return sst;
-    hipLaunchKernelGGL(terma_prep_kernel, dim3(1), dim3(64), 0, s,
-                       (int32_t*)status);
+    if (fresh)
+        hipLaunchKernelGGL(terma_prep_kernel, dim3(1), dim3(64), 0, s,
+                           (int32_t*)status);
     for (int32_t b = 0; b < n_beams; ++b) {
         const ch_beam& B = h_beams[b];
         const ch_aperture& A = B.aperture;
@@ -537,8 +542,7 @@ ch_status ch_terma_rays(const ch_beam* h_beam, const int32_t h_shape[3],
 
 ch_status ch_terma_check(void* stream)
 {
-    const void* status =
-        stream_scratch_peek((hipStream_t)stream, kScratchStatus);
+    void* status = stream_scratch_peek((hipStream_t)stream, kScratchStatus);
     if (!status)
         return CH_OK;
     int32_t flag = 0;
@@ -546,6 +550,9 @@ ch_status ch_terma_check(void* stream)
                           hipMemcpyDeviceToHost, (hipStream_t)stream));
     CH_HIP(hipStreamSynchronize((hipStream_t)stream));
     if (flag) {
+        hipLaunchKernelGGL(terma_prep_kernel, dim3(1), dim3(64), 0,
+                           (hipStream_t)stream, (int32_t*)status);
+        CH_HIP(hipStreamSynchronize((hipStream_t)stream));
         set_error("no intersection or line is within plane");
         return CH_ERR_GEOMETRY;
     }

@@ -324,7 +324,10 @@ def spectrum(settings):
 
 def terma(beams, shape, origin, spacing, debug=None, stream=None,
           check=True):
-    """Run the fused TERMA kernel for a list of ch_beam structs."""
+    """Run the fused TERMA kernel for a list of ch_beam structs.  With
+    check=False the geometry error of conehead/conehead.py:34-35 (a ray
+    parallel to the isocentre plane) is left for terma_check, which reports
+    it for every call queued on the stream since the last check."""
     n = len(beams)
     if n == 0:
         return
@@ -337,7 +340,14 @@ def terma(beams, shape, origin, spacing, debug=None, stream=None,
                           debug.ptr if debug is not None else None,
                           _h(stream)))
     if check:
-        _lib.check(L.ch_terma_check(_h(stream)))
+        terma_check(stream)
+
+
+def terma_check(stream=None):
+    """Synchronises `stream`; raises RuntimeError if a ray of any ch_terma
+    call on it since the last check lay in the plane through the source
+    perpendicular to the beam axis, and clears that flag."""
+    _lib.check(_lib.lib().ch_terma_check(_h(stream)))
 
 
 class Engine:

@@ -376,9 +376,11 @@ ch_status ch_terma_rays(const ch_beam* h_beam, const int32_t h_shape[3],
                         double* d_dirs, int32_t* d_vox, double* d_t,
                         int32_t* d_count, void* stream);
 
-/* Synchronises `stream` and returns CH_ERR_GEOMETRY when a ray of the last
- * ch_terma call was parallel to the isocentre plane (the reference raises
- * RuntimeError there, conehead/conehead.py:34-35). */
+/* Synchronises `stream` and returns CH_ERR_GEOMETRY when a ray of any
+ * ch_terma call queued on it since the last check was parallel to the
+ * isocentre plane (the reference raises RuntimeError there,
+ * conehead/conehead.py:34-35); the flag is then cleared.  One check after a
+ * batch of ch_terma calls covers all of them. */
 ch_status ch_terma_check(void* stream);
 
 /* Opens a centred square of side `length` cm (sets it to 1, leaves the

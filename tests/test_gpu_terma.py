@@ -119,3 +119,31 @@ def test_dicom_plan_mlc(oracle, engine):
     dens = hetero_densities(n, s)
     T, ref = compare(oracle, engine, shape, [s] * 3, origin, dens, 0.0, blk)
     assert T.max() > 0 and (T == 0).any()
+
+
+def test_geometry_error_sticky_until_checked(engine):
+    """The geometry error of conehead/conehead.py:34-35 (the grid of the
+    oracle test that raises: the source plane y = -100 cuts it) is reported
+    by the next check after any number of TERMA calls, and cleared by it."""
+    from conehead_amd.device import Stream
+    shape, spacing = (8, 8, 8), (0.5, 0.5, 0.5)
+    bad_origin, good_origin = (2.0, -102.0, -2.0), (-2.0, -2.0, -2.0)
+    src = Source("varian_clinac_6MV")
+    blk = Block()
+    blk.set_square(10.0)
+    d_rho = DeviceArray.from_host(np.ones(shape, np.float32))
+    d_t = DeviceArray(shape, np.float32)
+    beam = E.beam_struct(src, SETTINGS, engine.aperture(blk), d_rho, d_t,
+                         None)
+    s = Stream()
+    with pytest.raises(RuntimeError, match="within plane"):
+        E.terma([beam], shape, bad_origin, spacing, stream=s)
+    E.terma([beam], shape, good_origin, spacing, stream=s)    # cleared
+    # a failing call followed by a good one: one check reports it
+    E.terma([beam], shape, bad_origin, spacing, stream=s, check=False)
+    E.terma([beam], shape, good_origin, spacing, stream=s, check=False)
+    with pytest.raises(RuntimeError, match="within plane"):
+        E.terma_check(s)
+    E.terma_check(s)
+    s.sync()
+    s.close()

@@ -130,3 +130,17 @@ def test_rotated_gantry_is_symmetric(oracle):
     rot = np.transpose(a["terma"], (1, 0, 2))[::-1, :, :]
     np.testing.assert_allclose(b["terma"], rot, rtol=1e-9,
                                atol=1e-12 * a["terma"].max())
+
+
+def test_ray_in_source_plane_raises(oracle):
+    """A grid cut by the plane through the source perpendicular to the beam
+    axis (y = -100 cm at gantry 0, source at x = 0 outside the grid) has
+    sub-sample rays with n.u = 0: the reference raises there
+    (conehead/conehead.py:34-35). The same grid moved to the isocentre
+    does not."""
+    shape, s = (8, 8, 8), [0.5] * 3
+    dens = np.ones(shape, np.float32)
+    with pytest.raises(RuntimeError, match="within plane"):
+        run(oracle, shape, s, (2.0, -102.0, -2.0), dens)
+    out, _, _ = run(oracle, shape, s, (-2.0, -2.0, -2.0), dens)
+    assert np.isfinite(out["terma"]).all() and out["terma"].max() > 0
