@@ -14,7 +14,10 @@ Settings keys are the reference's (example.py:27-58).  Optional keys:
       of the reference's convolve_c call, conehead/conehead.py:780-794);
   semantics -- "convolve_c" (default), "cuda_dose" (y-polar cones and the
       index arithmetic of conehead/conehead.py:385-488; `kernel` is then a
-      [n_phi, >=5996] cumulative table and `phis` its angles) or "physics"
+      [n_phi, >=5996] cumulative table and `phis` its angles; without them
+      the reference's live call: the EDKnrc KernelPoly of
+      settings['energy_weights'], its 48 polar angles and 12 float32
+      azimuths, conehead/conehead.py:686-738) or "physics"
       (the cuda_dose cones aligned with the beam and the kernel looked up at
       the radiological distance through the densities; without `kernel` the
       EDKnrc polyenergetic kernel of settings['energy_weights'] merged into
@@ -62,7 +65,16 @@ class Conehead:
 
     def __init__(self, engine=None):
         self._engine = engine
+        self._stream = None
         self.dose_grid = None
+
+    @property
+    def stream(self):
+        """The instance's stream, kept across calls so its TERMA flag, FFT
+        work grids and gather workspace are allocated once."""
+        if self._stream is None:
+            self._stream = Stream()
+        return self._stream
 
     @property
     def engine(self):
@@ -81,7 +93,7 @@ class Conehead:
         eng = self.engine
         swap = settings.get("density_xy_swap", True)
         grid = settings.get("dose_grid")
-        s = Stream()
+        s = self.stream
         if grid is None:
             self.dose_grid = DoseGrid(phantom.size, phantom.origin,
                                       phantom.spacing)
@@ -116,6 +128,16 @@ class Conehead:
             poly = EdkSet.load().poly(settings["energy_weights"])
             phis, kernel = merge_polar(poly.angles, poly.kernel_cum,
                                        int(settings.get("n_phi", 12)))
+        elif semantics == "cuda_dose" and kernel is None:
+            # the reference's live dose call (conehead/conehead.py:686-738):
+            # KernelPoly of settings['energy_weights'] (48 x 5996), its 48
+            # float32 polar angles and 12 float32 azimuths 0, 30, ..., 330
+            if settings.get("phis") is not None:
+                raise ValueError("cuda_dose semantics: settings['phis'] "
+                                 "needs settings['kernel'] too")
+            poly = EdkSet.load().poly(settings["energy_weights"])
+            kernel = poly.kernel_cum
+            phis = np.asarray(poly.angles)
         else:
             if kernel is None or settings.get("phis") is None:
                 raise ValueError("%s semantics needs settings['kernel'] "
@@ -123,7 +145,9 @@ class Conehead:
                                  % semantics)
             phis = settings["phis"]
         thetas = settings.get("thetas")
-        thetas = default_thetas(12) if thetas is None else thetas
+        if thetas is None:
+            thetas = (np.linspace(0, 360 - 360 / 12, 12, dtype=np.float32)
+                      if semantics == "cuda_dose" else default_thetas(12))
         cones = ConeSet(thetas, phis, kernel, semantics=semantics,
                         frame=beam_frame(source) if physics else None)
 

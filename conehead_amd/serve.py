@@ -25,6 +25,8 @@ class AdmissionQueue:
     def __init__(self, budget_bytes, max_batch_jobs=64, method="gather"):
         self.budget = int(budget_bytes)
         self.in_use = 0
+        self.peak = 0         # largest in_use seen
+        self.deferred = 0     # admissions held back by the budget
         self.max_batch_jobs = max_batch_jobs
         self.method = method      # convolution whose work space is counted
         self._q = collections.deque()
@@ -52,11 +54,13 @@ class AdmissionQueue:
             req = self._q[0][0]
             need = req.device_bytes(self.method)
             if self.in_use + need > self.budget:
+                self.deferred += 1
                 break
             if batch and jobs + req.n_jobs > self.max_batch_jobs:
                 break
             batch.append(self._q.popleft())
             self.in_use += need
+            self.peak = max(self.peak, self.in_use)
             jobs += req.n_jobs
         return batch
 

@@ -117,3 +117,40 @@ def test_reference_example_config(oracle, engine):
                          case.cones.rows, case.cones.cum, n_threads=8)
     assert ref.max() > 0
     assert np.abs(dose - ref).max() / ref.max() < 1e-4
+
+
+@pytest.mark.parametrize("method", ["fft", "gather"])
+def test_calculate_cuda_dose_reference_settings(oracle, engine, method):
+    """The reference's live dose call (conehead/conehead.py:686-738) from
+    its own settings alone: semantics="cuda_dose" builds the EDKnrc
+    KernelPoly of settings['energy_weights'] (48 x 5996), its 48 polar
+    angles and 12 float32 azimuths, y-polar cones and the |floor(100 t - 5)|
+    index clamped at 5995.  Checked on the reference's example grid (50^3
+    water at 4 mm, 10 x 10 cm field) against the oracle chain: oracle TERMA,
+    then the oracle's geometric gather with the cuda_dose index
+    (convolve_rad, radiological=False), within 1e-4 of Dmax."""
+    from conehead_amd.cones import ConeSet
+    from conehead_amd.kernel import EdkSet
+    from conehead_amd.workloads import plain_cpu
+    case = plain_cpu()
+    src, blk, st = case.beams[0]
+    phantom = Phantom(case.densities, case.origin, case.spacing)
+    settings = dict(SETTINGS, semantics="cuda_dose", convolution=method)
+    dose = Conehead(engine).calculate(src, blk, phantom, settings)
+    poly = EdkSet.load().poly(SETTINGS["energy_weights"])
+    assert poly.kernel_cum.shape == (48, 5996)
+    cs = ConeSet(np.linspace(0, 330, 12, dtype=np.float32), poly.angles,
+                 poly.kernel_cum, semantics="cuda_dose")
+    assert cs.n_cones == 576
+    e, w, mu = E.spectrum(SETTINGS)
+    rb = oracle.make_beam(src.position, src.v_y, src.transform, src.sad,
+                          SETTINGS, e, w, mu)
+    dd = dose_grid_densities(phantom)
+    ref_t = oracle.terma(rb, case.shape, case.origin, case.spacing, dd,
+                         blk.block_values, n_threads=16)["terma"]
+    ref = np.zeros(case.shape)
+    oracle.convolve_rad(ref_t, None, ref, case.spacing, cs.dirs, cs.rows,
+                        cs.cum, radiological=False, n_threads=16)
+    assert ref.max() > 0
+    err = np.abs(dose - ref).max() / ref.max()
+    assert err < 1e-4, err

@@ -17,15 +17,15 @@ pytestmark = pytest.mark.gpu
 
 
 def compare(oracle, engine, shape, spacing, origin, dens, gantry, block,
-            collimator=0.0):
+            collimator=0.0, settings=SETTINGS):
     src = Source("varian_clinac_6MV")
     if gantry:
         src.gantry = gantry
     if collimator:
         src.collimator = collimator
-    e, w, mu = E.spectrum(SETTINGS)
+    e, w, mu = E.spectrum(settings)
     rb = oracle.make_beam(src.position, src.v_y, src.transform, src.sad,
-                          SETTINGS, e, w, mu)
+                          settings, e, w, mu)
     ref = oracle.terma(rb, shape, origin, spacing, dens, block.block_values,
                        n_threads=8)
     d_rho = DeviceArray.from_host(dens.astype(np.float32))
@@ -35,13 +35,13 @@ def compare(oracle, engine, shape, spacing, origin, dens, gantry, block,
     # analytic aperture (rectangles / leaf table) with the bounding-box
     # early out, and the materialised raster: identical TERMA bits
     ap = engine.aperture(block)
-    beam = E.beam_struct(src, SETTINGS, ap, d_rho, d_t, d_box)
+    beam = E.beam_struct(src, settings, ap, d_rho, d_t, d_box)
     E.terma([beam], shape, origin, spacing, debug=d_dbg)
     T = d_t.download()
     dbg = d_dbg.download()
     box = d_box.download()
     d_t2 = DeviceArray(shape, np.float32)
-    beam2 = E.beam_struct(src, SETTINGS, engine.raster(block), d_rho, d_t2,
+    beam2 = E.beam_struct(src, settings, engine.raster(block), d_rho, d_t2,
                           None)
     E.terma([beam2], shape, origin, spacing)
     assert np.array_equal(d_t2.download(), T)
@@ -58,6 +58,24 @@ def compare(oracle, engine, shape, spacing, origin, dens, gantry, block,
     assert box[:3].tolist() == nz.min(0).tolist()
     assert box[3:].tolist() == nz.max(0).tolist()
     return T, ref
+
+
+def test_off_axis_boundaries(oracle, engine):
+    """The annular window (rOuter moved to 0.4 so both ends lie in the
+    grid), the softening limit (8 cm) and the exponential clamp switch on
+    the same voxels as in the oracle: voxel (30, j, 20) images exactly at
+    OAD 5 (OAD zAnn / SAD == rInner), (40, 0, 20) at OAD 10 (== rOuter),
+    (36, 0, 20) at OAD 8 (== softLimit).  The oracle's off-axis arithmetic
+    is pinned by closed forms in test_oracle_terma_offaxis."""
+    shape = (41, 9, 41)
+    origin = np.array([-10.0, 0.0, -10.0])
+    rng = np.random.default_rng(11)
+    dens = (1.0 + 0.3 * rng.standard_normal(shape)).clip(0.1, None)
+    blk = Block()
+    blk.set_square(40)
+    st = dict(SETTINGS, rOuter=0.4, softLimit=8.0)
+    compare(oracle, engine, shape, [0.5] * 3, origin,
+            dens.astype(np.float32), 0.0, blk, settings=st)
 
 
 def test_water_square_field(oracle, engine):
