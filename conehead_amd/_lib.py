@@ -89,6 +89,7 @@ class Beam(ctypes.Structure):
         ("d_box", c_vp),
         ("accumulate", ctypes.c_int32),
         ("reserved", ctypes.c_int32),
+        ("d_status", c_vp),
     ]
 
 
@@ -159,6 +160,7 @@ SIGNATURES = {
                                         ctypes.c_int32, c_f64p,
                                         ctypes.c_int32, c_vpp]),
     "ch_plan_destroy": (ctypes.c_int32, [c_vp]),
+    "ch_plan_graphs": (ctypes.c_int32, [c_vp, c_i32p]),
     "ch_plan_info": (ctypes.c_int32, [c_vp, c_i32p, c_i64p, c_i32p]),
     "ch_plan_cone_taps": (ctypes.c_int32, [c_vp, ctypes.c_int32, c_i32p,
                                            c_f64p, ctypes.c_int32, c_i32p]),

@@ -28,6 +28,7 @@ import threading
 
 import numpy as np
 
+from . import _lib
 from . import engine as E
 from .device import DeviceArray, Event, HostPool, Stream, _h
 
@@ -153,16 +154,21 @@ class BatchHandle:
     """A launched batch: its completion event and device outputs."""
 
     def __init__(self, requests, outputs, temps, event, stream,
-                 staged=None):
+                 staged=None, flags=None):
         self.requests = requests
         self.outputs = outputs     # per request: list of dose DeviceArrays
         self.temps = temps         # buffers to return to the pool
         self.event = event
         self.stream = stream
         self.staged = staged       # per request: pinned HostBuffer, or None
+        self.flags = flags         # per request: TERMA geometry flag (int32)
 
     def done(self):
         return self.event.done()
+
+
+def _lib_error(message):
+    return _lib.CHError(_lib.CH_ERR_GEOMETRY, message)
 
 
 # Measured cost of one interleaved group gather relative to one single-beam
@@ -318,7 +324,11 @@ class BatchEngine:
         if self.method == "fft":
             keep = [eng.plan(r.shape, r.spacing, r.cones) for r in requests]
             eng.trim_spectra(self.spectrum_budget, keep=keep)
-        for req in requests:
+        # one geometry flag per request, so a ray in the source plane fails
+        # only the request whose beam has it
+        flags = self.pool.get((max(1, len(requests)),), np.int32)
+        flags.zero(s)
+        for ri, req in enumerate(requests):
             d_rho, own = self.densities(req, s)
             if own:
                 temps.append(d_rho)
@@ -336,6 +346,7 @@ class BatchEngine:
                 bs = E.beam_struct(src, st, eng.aperture(blk, s), d_rho,
                                    t_bufs[j], boxes[j])
                 bs.accumulate = 0 if (req.per_beam or b == 0) else 1
+                bs.d_status = flags.ptr + 4 * ri
                 beams.append(bs)
             for j in range(n_jobs):
                 outs.append(self.pool.get(req.shape))
@@ -395,41 +406,67 @@ class BatchEngine:
                     d.download_async(hb, i * per, stream=s)
                 staged.append(hb)
         ev = Event().record(s)
-        h = BatchHandle(list(requests), outputs, temps, ev, s, staged)
+        h = BatchHandle(list(requests), outputs, temps, ev, s, staged, flags)
         h.events = events          # kept alive until the batch is collected
         return h
 
-    def collect(self, handle, download=True):
+    def collect(self, handle, download=True, errors="raise"):
         """Wait for a batch; returns per request a float32 array
         [n_jobs][X][Y][Z] (or the device arrays if download=False) and
         returns the batch's temporaries to the pool.  Arrays of a staged
         batch view pinned buffers, which return to the host pool when the
-        arrays are dropped."""
+        arrays are dropped.  A request with a TERMA ray in the plane through
+        the source perpendicular to the beam (the reference's RuntimeError,
+        conehead/conehead.py:34-35) fails alone: errors="raise" raises a
+        CHError naming the failed requests once the batch is cleaned up,
+        errors="return" puts the CHError in that request's place."""
         handle.event.sync()
-        E.terma_check(handle.stream)
         results = []
         staged = handle.staged or [None] * len(handle.requests)
         handle.staged = None
-        for req, outs, hb in zip(handle.requests, handle.outputs, staged):
-            if download and hb is not None:
-                results.append(self.host_pool.as_array(
-                    hb, (len(outs),) + req.shape))
-                for d in outs:
-                    self.pool.put(d)
-            elif download:
-                arr = np.empty((len(outs),) + req.shape, np.float32)
-                for i, d in enumerate(outs):
-                    d.download(arr[i], stream=handle.stream)
-                    self.pool.put(d)
-                results.append(arr)
-            else:
-                if hb is not None:
-                    self.host_pool.put(hb)
-                results.append(outs)
-        for t in handle.temps:
-            self.pool.put(t)
-        handle.temps = []
-        handle.events = []
+        failed = []
+        try:
+            bad = np.zeros(len(handle.requests), bool)
+            if handle.flags is not None:
+                bad = handle.flags.download(
+                    stream=handle.stream)[:len(handle.requests)] != 0
+            for i, (req, outs, hb) in enumerate(
+                    zip(handle.requests, handle.outputs, staged)):
+                if bad[i]:
+                    failed.append(i)
+                    results.append(_lib_error(
+                        "no intersection or line is within plane "
+                        "(request %d of the batch)" % i))
+                    for d in outs:
+                        self.pool.put(d)
+                    if hb is not None:
+                        self.host_pool.put(hb)
+                elif download and hb is not None:
+                    results.append(self.host_pool.as_array(
+                        hb, (len(outs),) + req.shape))
+                    for d in outs:
+                        self.pool.put(d)
+                elif download:
+                    arr = np.empty((len(outs),) + req.shape, np.float32)
+                    for j, d in enumerate(outs):
+                        d.download(arr[j], stream=handle.stream)
+                        self.pool.put(d)
+                    results.append(arr)
+                else:
+                    if hb is not None:
+                        self.host_pool.put(hb)
+                    results.append(outs)
+        finally:
+            for t in handle.temps:
+                self.pool.put(t)
+            if handle.flags is not None:
+                self.pool.put(handle.flags)
+                handle.flags = None
+            handle.temps = []
+            handle.events = []
+        if failed and errors == "raise":
+            raise _lib_error("no intersection or line is within plane "
+                             "(requests %s of the batch)" % failed)
         return results
 
     def release(self, results):

@@ -34,51 +34,44 @@ __device__ __forceinline__ float leaf_value(int rr, int cc, int w, int h)
 
 // Leaf-table layout (int32): [0] n_pairs, [1] W (bank width), [2] roff
 // (raster row of bank row 0), [3] jx0, [4] jx1 (column jaws, tenths of mm),
-// [5] jy0, [6] jy1 (bank-row jaws), [7] unused, then bound[n_pairs + 1],
-// ends of bank A [n_pairs], ends of bank B [n_pairs].
+// [5] jy0, [6] jy1 (bank-row jaws), [7] start of the row table, then
+// bound[n_pairs + 1], ends of bank A [n_pairs], ends of bank B [n_pairs],
+// and at [7] (16-byte aligned) the row table: per raster row 4 int32
+// {kind, column of bank A's tip, column of bank B's tip, (row within the
+// leaf << 16) | leaf width}, kind 0 = closed (outside the bank or the
+// Y jaws), 1 = open (no leaf), 2 = covered by a leaf pair.
 constexpr int kMlcHeader = 8;
+constexpr int kMlcRows = 4000;
+constexpr int kRowClosed = 0, kRowOpen = 1, kRowLeaf = 2;
+
+__host__ __device__ __forceinline__ int mlc_row_table_start(int n_pairs)
+{
+    return (kMlcHeader + 3 * n_pairs + 1 + 3) & ~3;
+}
 
 // Leaf bank and jaws of conehead/block.py:130-196 (bank centred on raster
-// row 2000; rows outside the bank are blocked).  Leaves are contiguous and
-// ordered, so at most one contains a row: it is found by bisection.
+// row 2000; rows outside the bank are blocked) at one cell: the row's
+// entry of the row table (one 16-byte load) gives its leaf pair's tips and
+// the row's place in the leaf; the leaf that contains a row is the one
+// whose boundaries bracket it (leaves are contiguous and ordered).
 __device__ __forceinline__ float mlc_value(const int32_t* __restrict__ t,
                                            int row, int col)
 {
-    const int n_pairs = t[0], W = t[1];
-    const int rr = row - t[2];
-    if (rr < 0 || rr >= W)
+    const int4 e = reinterpret_cast<const int4*>(t + t[7])[row];
+    if (e.x == kRowClosed)
         return 0.0f;
-    const int32_t* bound = t + kMlcHeader;
-    const int32_t* ea = bound + n_pairs + 1;
-    const int32_t* eb = ea + n_pairs;
-    const int shift = abs(bound[0]);
     float v = 1.0f;
-    // largest n with bound[n] + shift <= rr
-    int lo = 0, hi = n_pairs - 1;
-    if (bound[0] + shift <= rr) {
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (bound[mid] + shift <= rr)
-                lo = mid;
-            else
-                hi = mid - 1;
-        }
-        const int rmin = bound[lo] + shift;
-        const int w = abs(bound[lo + 1] - bound[lo]);
-        if (rr < rmin + w) {
-            const int cmax = 2000 + ea[lo];       // bank A, :187-188
-            if (col < cmax)
-                v = leaf_value(rr - rmin, col, w, cmax);
-            const int cmin = 2000 + eb[lo];       // bank B, :189-190
-            const int h = 3999 - cmin;
-            if (col >= cmin && col < 3999)
-                v = leaf_value(w - 1 - (rr - rmin), h - 1 - (col - cmin), w,
-                               h);
-        }
+    if (e.x == kRowLeaf) {
+        const int w = e.w & 0xffff, rl = e.w >> 16;
+        const int cmax = e.y;                     // bank A, :187-188
+        if (col < cmax)
+            v = leaf_value(rl, col, w, cmax);
+        const int cmin = e.z;                     // bank B, :189-190
+        const int h = 3999 - cmin;
+        if (col >= cmin && col < 3999)
+            v = leaf_value(w - 1 - rl, h - 1 - (col - cmin), w, h);
     }
     if (col < 2000 + t[3] || col >= 2000 + t[4])  // jaws, :193-196
-        v = 0.0f;
-    if (rr < t[5] || rr >= t[6])
         v = 0.0f;
     return v;
 }
@@ -115,14 +108,22 @@ __device__ __forceinline__ float mlc_block_value(const int32_t* __restrict__ t,
         else
             hi = mid - 1;
     }
-    const int rmin = bound[lo] + shift;
-    const int w = abs(bound[lo + 1] - bound[lo]);
-    if (r1 >= rmin + w)
-        return -1.0f;  // spans two leaf pairs
-    // open gap between the tips: value 1 (mlc_value leaves v = 1 there)
-    if (j0 >= 2000 + ea[lo] && j1 < 2000 + eb[lo])
-        return 1.0f;
-    return -1.0f;
+    // Walk the leaf pairs the rows r0..r1 cross (pairs are contiguous):
+    // the block is open (value 1, mlc_value leaves v = 1 there) when its
+    // columns lie in the gap between the tips of every one of them.
+    for (int p = lo; p < n_pairs; ++p) {
+        const int rmin = bound[p] + shift;
+        const int w = abs(bound[p + 1] - bound[p]);
+        if (rmin > r1)
+            return 1.0f;          // every row of the block was covered
+        if (!(j0 >= 2000 + ea[p] && j1 < 2000 + eb[p]))
+            return -1.0f;         // a tip or a leaf inside the block
+        if (r1 < rmin + w)
+            return 1.0f;
+        if (p + 1 < n_pairs && bound[p + 1] + shift != rmin + w)
+            return -1.0f;         // not contiguous: rows between pairs
+    }
+    return -1.0f;                 // rows beyond the last pair
 }
 
 __device__ __forceinline__ float rects_block_value(const int32_t* __restrict__ t,

@@ -1013,6 +1013,12 @@ static ch_status workspace(hipStream_t s, size_t pad_elems, size_t tab_bytes,
                         "capturing it");
         return CH_ERR_ARG;
     }
+    if (w->pad_elems < pad_elems && stream_graph_holds(s) > 0) {
+        set_error("the gather workspace must grow, but a live graph "
+                  "captured on this stream reads it: destroy the graph or "
+                  "use another stream");
+        return CH_ERR_ARG;
+    }
     if (w->pad_elems < pad_elems) {
         // the old buffer may still be read by queued kernels of this stream
         CH_HIP(hipStreamSynchronize(s));
@@ -1125,6 +1131,7 @@ static ch_status convolve_one(const ch_plan* p, const float* d_terma,
     st = workspace(s, np + slab_ints(p->shape[0]), 0, &w);
     if (st != CH_OK)
         return st;
+    capture_pin(s, p);
     st = pad_into(d_terma, w->pad, p, s);
     if (st != CH_OK)
         return st;

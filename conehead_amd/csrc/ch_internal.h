@@ -189,11 +189,29 @@ enum ScratchSlot {
     kScratchSpectrum = 2,     // complex work grids of the spectral passes
     kScratchSlots = 3
 };
-ch_status stream_scratch(hipStream_t s, int slot, size_t bytes, void** out);
+// zero_new: a newly allocated buffer is zero-filled (synchronously, under
+// the allocator's lock) before it is handed out.
+ch_status stream_scratch(hipStream_t s, int slot, size_t bytes, void** out,
+                         bool zero_new = false);
 // The slot's buffer if this stream has one, else nullptr (no allocation).
 void* stream_scratch_peek(hipStream_t s, int slot);
 // Free a stream's scratch and gather workspace (before it is destroyed).
 void release_stream_scratch(hipStream_t s);
+// What a live captured graph reads stays valid while the graph lives:
+//  capture_pin(s, p) -- while `s` is being captured, records plan p in the
+//    capture (once) and pins it: ch_plan_destroy of a pinned plan is
+//    deferred to the destruction of its last graph, and its spectrum cannot
+//    be released;
+//  stream_graph_holds(s) -- live graphs captured on `s`: while there are
+//    any, the stream's scratch and gather workspace must not be
+//    reallocated and the stream must not be destroyed.
+void capture_pin(hipStream_t s, const ch_plan* p);
+int stream_graph_holds(hipStream_t s);
+int plan_pins(const ch_plan* p);
+// True when `p` is pinned: it is then marked to be freed with its last
+// graph (plan_release is called then).
+bool plan_defer_free(ch_plan* p);
+void plan_release(ch_plan* p);
 void release_gather_workspace(hipStream_t s);
 void release_spectral_tables(hipStream_t s);
 // Pads two grids of plan p's shape into the stream's gather workspace
@@ -291,6 +309,10 @@ struct ch_plan {
     int32_t max_taps;
     int32_t tile[3];
     int32_t accum_mode;
+    // live graphs that read this plan, and whether ch_plan_destroy was
+    // called while there were some (guarded by the graph registry's lock)
+    int32_t pins;
+    bool doomed;
     int32_t* d_lin;
     float* d_k;
     double* d_k64;

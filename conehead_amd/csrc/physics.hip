@@ -242,6 +242,7 @@ ch_status ch_convolve_rad(const ch_plan* plan, const float* d_terma,
     ch_status st = pad_pair(plan, d_terma, d_density, s, &pt, &pr);
     if (st != CH_OK)
         return st;
+    capture_pin(s, plan);
     RadJob j;
     j.terma = pt;
     j.density = pr;

@@ -113,7 +113,7 @@ __global__ __launch_bounds__(64) void build_taps_kernel(
     meta[c] = m;
 }
 
-static void plan_free(ch_plan* p)
+void plan_release(ch_plan* p)
 {
     if (!p)
         return;
@@ -355,7 +355,7 @@ ch_status ch_plan_create(const int32_t h_shape[3], const double h_vsize[3],
     hipFree(d_rows);
     hipFree(d_cum);
     if (e != hipSuccess) {
-        plan_free(p);
+        plan_release(p);
         return hip_fail(e, "ch_plan_create", __FILE__, __LINE__);
     }
     p->n_taps_total = 0;
@@ -371,7 +371,17 @@ ch_status ch_plan_create(const int32_t h_shape[3], const double h_vsize[3],
 
 ch_status ch_plan_destroy(ch_plan* plan)
 {
-    plan_free(plan);
+    // a plan read by a live graph is freed with the last such graph
+    if (plan && plan_defer_free(plan))
+        return CH_OK;
+    plan_release(plan);
+    return CH_OK;
+}
+
+ch_status ch_plan_graphs(const ch_plan* plan, int32_t* n_graphs)
+{
+    CH_REQUIRE(plan && n_graphs, "ch_plan_graphs: null pointer");
+    *n_graphs = plan_pins(plan);
     return CH_OK;
 }
 

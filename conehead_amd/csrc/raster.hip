@@ -58,7 +58,7 @@ ch_status ch_raster_square(float* d_raster, double length, void* stream)
 
 int32_t ch_mlc_table_len(int32_t n_pairs)
 {
-    return kMlcHeader + 3 * n_pairs + 1;
+    return mlc_row_table_start(n_pairs) + 4 * kMlcRows;
 }
 
 ch_status ch_mlc_table(int32_t* d_table, int32_t n_pairs,
@@ -80,7 +80,8 @@ ch_status ch_mlc_table(int32_t* d_table, int32_t n_pairs,
                    "ch_mlc_table: leaf %d tips %d/%d outside the raster",
                    n, h_a[n], h_b[n]);
     }
-    std::vector<int32_t> t(kMlcHeader + 3 * n_pairs + 1, 0);
+    const int rt = mlc_row_table_start(n_pairs);
+    std::vector<int32_t> t(rt + 4 * kMlcRows, 0);
     t[0] = n_pairs;
     t[1] = W;
     t[2] = 2000 - W / 2;                         // bank centred on row 2000
@@ -88,11 +89,39 @@ ch_status ch_mlc_table(int32_t* d_table, int32_t n_pairs,
     t[4] = h_jaw_x[1];
     t[5] = (int)(W / 2.0 + h_jaw_y[0]);          // int(mlc_width/2 + jaw_y)
     t[6] = (int)(W / 2.0 + h_jaw_y[1]);
+    t[7] = rt;
     for (int n = 0; n <= n_pairs; ++n)
         t[kMlcHeader + n] = h_bound[n];
     for (int n = 0; n < n_pairs; ++n) {
         t[kMlcHeader + n_pairs + 1 + n] = h_a[n];
         t[kMlcHeader + 2 * n_pairs + 1 + n] = h_b[n];
+    }
+    // Row table: the bank rows of conehead/block.py:130-196 in raster rows
+    // -- outside the bank or the Y jaws closed, otherwise the leaf pair
+    // whose rows [bound[n] + shift, + width) hold the row (the last pair
+    // starting at or before it), or open when none does.
+    const int shift = abs(h_bound[0]);
+    for (int row = 0; row < kMlcRows; ++row) {
+        int32_t* e = &t[rt + 4 * row];
+        const int rr = row - t[2];
+        if (rr < 0 || rr >= W || rr < t[5] || rr >= t[6]) {
+            e[0] = kRowClosed;
+            continue;
+        }
+        e[0] = kRowOpen;
+        if (h_bound[0] + shift > rr)
+            continue;
+        int lo = 0;
+        while (lo + 1 < n_pairs && h_bound[lo + 1] + shift <= rr)
+            ++lo;
+        const int rmin = h_bound[lo] + shift;
+        const int w = abs(h_bound[lo + 1] - h_bound[lo]);
+        if (rr < rmin + w) {
+            e[0] = kRowLeaf;
+            e[1] = 2000 + h_a[lo];
+            e[2] = 2000 + h_b[lo];
+            e[3] = ((rr - rmin) << 16) | w;
+        }
     }
     if (h_bbox_out) {
         // Every nonzero cell lies inside the bank rows, the Y-jaw rows and

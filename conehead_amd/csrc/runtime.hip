@@ -149,7 +149,8 @@ struct StreamScratch {
 static std::vector<StreamScratch> g_scratch;
 static std::mutex g_scratch_mu;
 
-ch_status stream_scratch(hipStream_t s, int slot, size_t bytes, void** out)
+ch_status stream_scratch(hipStream_t s, int slot, size_t bytes, void** out,
+                         bool zero_new)
 {
     std::lock_guard<std::mutex> lock(g_scratch_mu);
     StreamScratch* e = nullptr;
@@ -172,6 +173,12 @@ ch_status stream_scratch(hipStream_t s, int slot, size_t bytes, void** out)
                       "captured: run the pipeline once before capturing it");
             return CH_ERR_ARG;
         }
+        if (stream_graph_holds(s) > 0) {
+            set_error("stream scratch must grow, but a live graph captured "
+                      "on this stream reads it: destroy the graph or use "
+                      "another stream");
+            return CH_ERR_ARG;
+        }
         // queued kernels of this stream may still use the old buffer
         CH_HIP(hipStreamSynchronize(s));
         CH_HIP(hipFree(e->ptr[slot]));
@@ -179,6 +186,12 @@ ch_status stream_scratch(hipStream_t s, int slot, size_t bytes, void** out)
         e->bytes[slot] = 0;
         CH_HIP(hipMalloc(&e->ptr[slot], bytes));
         e->bytes[slot] = bytes;
+        if (zero_new) {
+            // cleared before any caller (of any thread) gets the pointer
+            std::vector<char> zeros(bytes, 0);
+            CH_HIP(hipMemcpy(e->ptr[slot], zeros.data(), bytes,
+                             hipMemcpyHostToDevice));
+        }
     }
     *out = e->ptr[slot];
     return CH_OK;
@@ -205,6 +218,81 @@ void* stream_scratch_peek(hipStream_t s, int slot)
         if (x.stream == s)
             return x.ptr[slot];
     return nullptr;
+}
+
+// Graphs and what they read.  A capture records every plan an entry point
+// uses on the capturing stream (capture_pin); the graph made from it pins
+// those plans and holds the capture stream's scratch until it is
+// destroyed.
+struct ActiveCapture {
+    hipStream_t stream;
+    std::vector<ch_plan*> plans;
+};
+static std::vector<ActiveCapture> g_active;
+static std::vector<std::pair<hipStream_t, int>> g_holds;
+static std::mutex g_graph_mu;
+
+static void unpin_locked(const std::vector<ch_plan*>& plans)
+{
+    for (ch_plan* p : plans)
+        if (--p->pins == 0 && p->doomed)
+            plan_release(p);
+}
+
+void capture_pin(hipStream_t s, const ch_plan* cp)
+{
+    if (!cp || !capturing(s))
+        return;
+    ch_plan* p = const_cast<ch_plan*>(cp);
+    std::lock_guard<std::mutex> lock(g_graph_mu);
+    for (auto& a : g_active) {
+        if (a.stream != s)
+            continue;
+        for (ch_plan* q : a.plans)
+            if (q == p)
+                return;
+        a.plans.push_back(p);
+        ++p->pins;
+        return;
+    }
+}
+
+int stream_graph_holds(hipStream_t s)
+{
+    std::lock_guard<std::mutex> lock(g_graph_mu);
+    for (auto& h : g_holds)
+        if (h.first == s)
+            return h.second;
+    return 0;
+}
+
+int plan_pins(const ch_plan* p)
+{
+    std::lock_guard<std::mutex> lock(g_graph_mu);
+    return p->pins;
+}
+
+bool plan_defer_free(ch_plan* p)
+{
+    std::lock_guard<std::mutex> lock(g_graph_mu);
+    if (p->pins == 0)
+        return false;
+    p->doomed = true;
+    return true;
+}
+
+// Takes the capture record of `s` (its plans stay pinned).
+static std::vector<ch_plan*> take_capture(hipStream_t s)
+{
+    std::lock_guard<std::mutex> lock(g_graph_mu);
+    std::vector<ch_plan*> plans;
+    for (size_t i = 0; i < g_active.size(); ++i)
+        if (g_active[i].stream == s) {
+            plans.swap(g_active[i].plans);
+            g_active.erase(g_active.begin() + i);
+            break;
+        }
+    return plans;
 }
 
 }  // namespace ch
@@ -365,6 +453,9 @@ ch_status ch_stream_destroy(void* stream)
 {
     if (stream) {
         hipStream_t s = (hipStream_t)stream;
+        CH_REQUIRE(stream_graph_holds(s) == 0,
+                   "ch_stream_destroy: a live graph captured on this stream "
+                   "reads its scratch; destroy the graph first");
         release_gather_workspace(s);
         release_spectral_tables(s);
         release_stream_scratch(s);
@@ -376,33 +467,78 @@ ch_status ch_stream_destroy(void* stream)
 ch_status ch_graph_begin(void* stream)
 {
     CH_REQUIRE(stream, "ch_graph_begin: capture needs a created stream");
-    CH_HIP(hipStreamBeginCapture((hipStream_t)stream,
-                                 hipStreamCaptureModeThreadLocal));
+    hipStream_t s = (hipStream_t)stream;
+    {
+        std::vector<ch_plan*> stale = take_capture(s);
+        std::lock_guard<std::mutex> lock(g_graph_mu);
+        unpin_locked(stale);
+        ActiveCapture a;
+        a.stream = s;
+        g_active.push_back(a);
+    }
+    const hipError_t e =
+        hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        std::vector<ch_plan*> plans = take_capture(s);
+        std::lock_guard<std::mutex> lock(g_graph_mu);
+        unpin_locked(plans);
+        return hip_fail(e, "hipStreamBeginCapture", __FILE__, __LINE__);
+    }
     return CH_OK;
 }
 
 struct ch_graph {
     hipGraph_t graph;
     hipGraphExec_t exec;
+    hipStream_t stream;            // capture stream (its scratch is held)
+    std::vector<ch_plan*> plans;   // pinned plans
 };
 
 ch_status ch_graph_end(void* stream, ch_graph** graph_out)
 {
     CH_REQUIRE(stream && graph_out, "ch_graph_end: null argument");
     *graph_out = nullptr;
+    hipStream_t s = (hipStream_t)stream;
+    std::vector<ch_plan*> plans = take_capture(s);
+    auto unpin = [&plans]() {
+        std::lock_guard<std::mutex> lock(g_graph_mu);
+        unpin_locked(plans);
+    };
     hipGraph_t g = nullptr;
     // always ends the capture, also when a call inside it failed
-    CH_HIP(hipStreamEndCapture((hipStream_t)stream, &g));
-    CH_REQUIRE(g, "ch_graph_end: the capture produced no graph");
+    hipError_t e = hipStreamEndCapture(s, &g);
+    if (e != hipSuccess) {
+        unpin();
+        return hip_fail(e, "hipStreamEndCapture", __FILE__, __LINE__);
+    }
+    if (!g) {
+        unpin();
+        set_error("ch_graph_end: the capture produced no graph");
+        return CH_ERR_ARG;
+    }
     hipGraphExec_t x = nullptr;
-    const hipError_t e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
     if (e != hipSuccess) {
         (void)hipGraphDestroy(g);
+        unpin();
         return hip_fail(e, "hipGraphInstantiate", __FILE__, __LINE__);
     }
     ch_graph* out = new ch_graph();
     out->graph = g;
     out->exec = x;
+    out->stream = s;
+    out->plans.swap(plans);
+    {
+        std::lock_guard<std::mutex> lock(g_graph_mu);
+        bool found = false;
+        for (auto& h : g_holds)
+            if (h.first == s) {
+                ++h.second;
+                found = true;
+            }
+        if (!found)
+            g_holds.emplace_back(s, 1);
+    }
     *graph_out = out;
     return CH_OK;
 }
@@ -418,8 +554,17 @@ ch_status ch_graph_destroy(ch_graph* graph)
 {
     if (!graph)
         return CH_OK;
+    // a replay may still be queued: wait for it before plans are freed
+    (void)hipDeviceSynchronize();
     (void)hipGraphExecDestroy(graph->exec);
     (void)hipGraphDestroy(graph->graph);
+    {
+        std::lock_guard<std::mutex> lock(g_graph_mu);
+        unpin_locked(graph->plans);
+        for (size_t i = 0; i < g_holds.size(); ++i)
+            if (g_holds[i].first == graph->stream && --g_holds[i].second == 0)
+                g_holds.erase(g_holds.begin() + i);
+    }
     delete graph;
     return CH_OK;
 }

@@ -1510,6 +1510,9 @@ ch_status ch_plan_spectrum_release(ch_plan* plan)
     CH_REQUIRE(plan, "ch_plan_spectrum_release: null plan");
     if (!plan->d_khat)
         return CH_OK;
+    CH_REQUIRE(plan_pins(plan) == 0,
+               "ch_plan_spectrum_release: %d live graph(s) read this plan's "
+               "spectrum; destroy them first", plan_pins(plan));
     // kernels queued on any stream may still read K^
     CH_HIP(hipDeviceSynchronize());
     spectrum_free(plan);
@@ -1546,6 +1549,7 @@ ch_status ch_convolve_fft(const ch_plan* plan, const float* d_terma,
                                   beam_points(plan) * sizeof(float2), &work);
     if (st != CH_OK)
         return st;
+    capture_pin(s, plan);
     return spectral_passes(plan, single_beam(d_terma, d_dose, accumulate),
                            (float2*)work, s, true);
 }

@@ -91,8 +91,15 @@ class Plan:
         self.spectrum_bytes = 8 * f[0] * f[1] * pitch.value
         return tuple(f), pitch.value, work.value
 
+    def graphs(self):
+        """Live captured graphs that read this plan (ch_plan_graphs)."""
+        n = ctypes.c_int32()
+        _lib.check(_lib.lib().ch_plan_graphs(self.handle, ctypes.byref(n)))
+        return n.value
+
     def release_spectrum(self):
-        """Frees the kernel spectrum (HBM); the next FFT use rebuilds it."""
+        """Frees the kernel spectrum (HBM); the next FFT use rebuilds it.
+        Refused while a live graph reads the plan."""
         _lib.check(_lib.lib().ch_plan_spectrum_release(self.handle))
         self.spectrum_bytes = 0
 
@@ -362,9 +369,9 @@ class Engine:
 
     def trim_spectra(self, budget_bytes, keep=()):
         """Frees the kernel spectra of the least recently used plans, other
-        than `keep`, until the cached spectra hold at most budget_bytes of
-        HBM (75 MB per 128^3 plan, about 225 MB at 200 x 200 x 160).
-        Returns the bytes still held."""
+        than `keep` and those a live captured graph reads, until the cached
+        spectra hold at most budget_bytes of HBM (75 MB per 128^3 plan,
+        about 225 MB at 200 x 200 x 160).  Returns the bytes still held."""
         with self._lock:
             keep = {id(p) for p in keep}
             held = sum(getattr(p, "spectrum_bytes", 0)
@@ -373,7 +380,7 @@ class Engine:
                 if held <= budget_bytes:
                     break
                 b = getattr(p, "spectrum_bytes", 0)
-                if b and id(p) not in keep:
+                if b and id(p) not in keep and p.graphs() == 0:
                     p.release_spectrum()
                     held -= b
             return held

@@ -127,9 +127,13 @@ class DoseServer:
             if handle.done():
                 try:
                     results = self.be.collect(handle,
-                                              download=self.download)
+                                              download=self.download,
+                                              errors="return")
                     t = time.perf_counter()
                     for (req, fut, t0), res in zip(items, results):
+                        if isinstance(res, Exception):
+                            fut.set_exception(res)   # this request only
+                            continue
                         self.records.append((t0, t, len(req.beams),
                                              req.voxel_cone_updates()))
                         fut.set_result(res)

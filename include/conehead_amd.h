@@ -123,6 +123,10 @@ ch_status ch_plan_create(const int32_t h_shape[3], const double h_vsize[3],
                          int32_t row_len, const double* h_cum,
                          int32_t semantics, ch_plan** plan_out);
 ch_status ch_plan_destroy(ch_plan* plan);
+/* Live graphs (ch_graph_end) that read the plan.  While there are any,
+ * ch_plan_spectrum_release refuses and ch_plan_destroy is deferred to the
+ * destruction of the last of them.  No reference counterpart. */
+ch_status ch_plan_graphs(const ch_plan* plan, int32_t* n_graphs);
 /* Totals of the plan: number of cones, total taps, longest tap list. */
 ch_status ch_plan_info(const ch_plan* plan, int32_t* n_cones,
                        int64_t* n_taps, int32_t* max_taps);
@@ -182,8 +186,9 @@ ch_status ch_convolve_batch(int32_t n_jobs, const ch_conv_job* h_jobs,
  * reference counterpart: it precomputes what ch_convolve_fft needs. */
 ch_status ch_plan_spectrum(ch_plan* plan, void* stream);
 /* Frees the plan's kernel spectrum (waits for the device first: queued
- * kernels may read it); ch_plan_spectrum builds it again.  No reference
- * counterpart: it bounds the HBM held by cached plans. */
+ * kernels may read it); ch_plan_spectrum builds it again.  Refused
+ * (CH_ERR_ARG) while a live graph reads the plan (ch_plan_graphs).  No
+ * reference counterpart: it bounds the HBM held by cached plans. */
 ch_status ch_plan_spectrum_release(ch_plan* plan);
 /* Frame lengths, half-spectrum row pitch and the work-grid bytes one beam
  * uses (held per stream) of a plan whose spectrum is built. */
@@ -299,8 +304,14 @@ ch_status ch_resample_linear(const float* d_src, const int32_t src_shape[3],
  * so its stream scratch and gather workspace exist; a call that would have
  * to allocate or synchronise fails with CH_ERR_ARG instead, and the batch
  * entry points (host job tables) always do.  ch_graph_end ends the capture
- * in every case.  No counterpart in the reference (its flow re-launches
- * every kernel from Python, conehead/conehead.py:536-682). */
+ * in every case.  A graph holds what it reads until ch_graph_destroy: the
+ * plans used in the capture are pinned (ch_plan_graphs), and the capture
+ * stream's scratch and gather workspace are never reallocated (a call on
+ * that stream that would need them larger fails with CH_ERR_ARG) nor freed
+ * (ch_stream_destroy refuses).  Device buffers passed by the caller stay
+ * the caller's to keep alive.  ch_graph_destroy waits for the device.  No
+ * counterpart in the reference (its flow re-launches every kernel from
+ * Python, conehead/conehead.py:536-682). */
 typedef struct ch_graph ch_graph;
 ch_status ch_graph_begin(void* stream);
 ch_status ch_graph_end(void* stream, ch_graph** graph_out);
@@ -350,6 +361,11 @@ typedef struct ch_beam {
     int32_t* d_box;             /* output support box (6 ints), nullable */
     int32_t accumulate;         /* 1: d_terma += this beam, d_box widened */
     int32_t reserved;
+    int32_t* d_status;          /* nullable: this beam's own geometry flag
+                                   (one int32 the caller zeroes; set to 1
+                                   when a ray lies in the source plane)
+                                   instead of the stream's flag that
+                                   ch_terma_check reads */
 } ch_beam;
 
 /* TERMA for n beams on one dose grid, queued in order on `stream` (one

@@ -57,3 +57,40 @@ def test_library_is_gfx950_only():
     data = open(_lib.LIB_PATH, "rb").read()
     assert b"gfx950" in data
     assert b"gfx942" not in data and b"gfx90a" not in data
+
+
+def test_struct_mirrors_match_the_header(tmp_path):
+    """The ctypes mirrors have the C structs' sizes and field offsets, read
+    from the header by the C compiler: a field added to one mirror and not
+    the other shifts every array element after the first."""
+    import ctypes
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    mirrors = {"ch_beam": _lib.Beam, "ch_conv_job": _lib.ConvJob,
+               "ch_aperture": _lib.Aperture,
+               "ch_conv_group": _lib.ConvGroup}
+    lines = ['#include <stdio.h>', '#include <stddef.h>',
+             '#include "conehead_amd.h"', 'int main(void) {']
+    for cname, py in mirrors.items():
+        lines.append('printf("%s size %%zu\\n", sizeof(%s));'
+                     % (cname, cname))
+        for f, _ in py._fields_:
+            lines.append('printf("%s %s %%zu\\n", offsetof(%s, %s));'
+                         % (cname, f, cname, f))
+    lines += ['return 0;', '}']
+    src = tmp_path / "sizes.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "sizes"
+    subprocess.run([cc, "-I", os.path.join(ROOT, "include"), str(src), "-o",
+                    str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True,
+                         text=True).stdout.split("\n")
+    got = {tuple(line.split()[:2]): int(line.split()[2])
+           for line in out if line.strip()}
+    for cname, py in mirrors.items():
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert got[(cname, f)] == getattr(py, f).offset, (cname, f)

@@ -155,3 +155,39 @@ def test_staged_results(be):
     reused = be.collect(be.launch([reqs[1]], s, stage=True))[0]
     assert reused.__array_interface__["data"][0] == ptr
     assert np.array_equal(reused, plain[1])
+
+
+def test_geometry_error_fails_only_its_request(engine):
+    """A request whose grid the source plane cuts (the reference raises
+    there, conehead/conehead.py:34-35) fails alone: the batch's other
+    requests are delivered, bitwise equal to their results alone, and the
+    server fails only that request's future."""
+    from conehead_amd.serve import DoseServer
+    good = small_cases()[:2]
+    # gantry 0: the source plane is y = -100 cm, which cuts this grid
+    shape = (8, 8, 8)
+    bad = DoseRequest(np.ones(shape, np.float32), (2.0, -102.0, -2.0),
+                      (0.5, 0.5, 0.5),
+                      [(W.source_at(0.0), W.square_block(10.0), W.SETTINGS)],
+                      good[0].cones)
+    reqs = [DoseRequest.from_case(good[0]), bad,
+            DoseRequest.from_case(good[1])]
+    be = BatchEngine(engine)
+    s = Stream()
+    alone = [be.run([reqs[0]], s)[0], be.run([reqs[2]], s)[0]]
+    out = be.collect(be.launch(reqs, s), errors="return")
+    assert isinstance(out[1], RuntimeError)
+    assert "within plane" in str(out[1])
+    assert np.array_equal(out[0], alone[0])
+    assert np.array_equal(out[2], alone[1])
+    with pytest.raises(RuntimeError, match="within plane"):
+        be.run(reqs, s)
+    # the flags start clear in the next batch
+    again = be.run([reqs[0], reqs[2]], s)
+    assert np.array_equal(again[0], alone[0])
+    with DoseServer(be, hbm_budget=1 << 30, max_inflight=1) as srv:
+        futs = [srv.submit(r) for r in reqs]
+        with pytest.raises(RuntimeError, match="within plane"):
+            futs[1].result(timeout=60)
+        assert np.array_equal(futs[0].result(timeout=60), alone[0])
+        assert np.array_equal(futs[2].result(timeout=60), alone[1])

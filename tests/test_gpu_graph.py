@@ -76,3 +76,58 @@ def test_capture_refuses_unwarmed_stream(engine):
     run(case, t, d, box, beam, plan, s)            # the stream still works
     s.sync()
     assert d.download(stream=s).max() > 0
+
+
+def test_graph_holds_its_plan_and_stream(engine):
+    """A live graph pins the plans it reads and the capture stream's
+    scratch: trim_spectra skips the plan, releasing its spectrum is refused,
+    destroying the plan is deferred to the graph's destruction, a call that
+    would grow the stream's work grid is refused and so is destroying the
+    stream; every replay still gives the eager bits.  Once the graph is
+    gone, all of that works again."""
+    s = Stream()
+    case, rho, t, d, box, beam, plan = setup(engine, s)
+    # a private plan (not the engine's cached one) to destroy under the graph
+    own = E.Plan(case.shape, case.spacing, case.cones)
+    run(case, t, d, box, beam, own, s, "fft")
+    ref = d.download(stream=s)
+    with E.Graph.capture(s) as cap:
+        run(case, t, d, box, beam, own, s, "fft")
+    g = cap.graph
+    assert own.graphs() == 1
+    with pytest.raises(_lib.CHError, match="live graph"):
+        own.release_spectrum()
+    # the engine's LRU trim leaves a pinned plan's spectrum alone
+    engine._plans["pinned-test"] = own
+    try:
+        engine.trim_spectra(0)
+        assert own.spectrum_bytes > 0
+    finally:
+        del engine._plans["pinned-test"]
+    # a bigger grid on the capture stream would regrow its work grid
+    big = W.hetero(n=64, spacing=0.5, n_theta=4, n_phi=4)
+    pb = engine.plan(big.shape, big.spacing, big.cones)
+    tb, db = DeviceArray(big.shape, zero=True), DeviceArray(big.shape)
+    with pytest.raises(_lib.CHError, match="live graph"):
+        E.convolve_fft(pb, tb, db, stream=s)
+    s2 = Stream()
+    E.convolve_fft(pb, tb, db, stream=s2)          # another stream is fine
+    s2.sync()
+    assert _lib.lib().ch_stream_destroy(s.handle) == _lib.CH_ERR_ARG
+    handle = own.handle
+    own.close()                                    # deferred: graph lives
+    for _ in range(3):
+        d.zero(s)
+        s.sync()
+        g.launch(s)
+        assert np.array_equal(d.download(stream=s), ref)
+    g.close()                                      # frees the plan as well
+    E.convolve_fft(pb, tb, db, stream=s)           # the stream may grow now
+    s.sync()
+    assert handle is not None
+    p2 = E.Plan(case.shape, case.spacing, case.cones)
+    p2.spectrum(s)
+    run(case, t, d, box, beam, p2, s, "fft")
+    assert p2.graphs() == 0
+    p2.release_spectrum()
+    assert np.array_equal(d.download(stream=s), ref)

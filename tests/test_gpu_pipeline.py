@@ -146,11 +146,18 @@ def test_calculate_cuda_dose_reference_settings(oracle, engine, method):
     rb = oracle.make_beam(src.position, src.v_y, src.transform, src.sad,
                           SETTINGS, e, w, mu)
     dd = dose_grid_densities(phantom)
-    ref_t = oracle.terma(rb, case.shape, case.origin, case.spacing, dd,
-                         blk.block_values, n_threads=16)["terma"]
+    # the dose grid's own geometry: Phantom holds float32 origin and
+    # spacing (conehead/phantom.py:13-14), so 0.4 cm is 0.4000000059604645
+    # cm here as in the reference, and kernel indices floor(100 t - 5) sit
+    # where that spacing puts them
+    org = np.asarray(phantom.origin, np.float64)
+    spc = np.asarray(phantom.spacing, np.float64)
+    assert spc[0] != 0.4
+    ref_t = oracle.terma(rb, case.shape, org, spc, dd, blk.block_values,
+                         n_threads=16)["terma"]
     ref = np.zeros(case.shape)
-    oracle.convolve_rad(ref_t, None, ref, case.spacing, cs.dirs, cs.rows,
-                        cs.cum, radiological=False, n_threads=16)
+    oracle.convolve_rad(ref_t, None, ref, spc, cs.dirs, cs.rows, cs.cum,
+                        radiological=False, n_threads=16)
     assert ref.max() > 0
     err = np.abs(dose - ref).max() / ref.max()
     assert err < 1e-4, err
