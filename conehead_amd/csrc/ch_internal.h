@@ -187,7 +187,8 @@ enum ScratchSlot {
     kScratchBoxPartials = 0,  // per-block partial support boxes
     kScratchStatus = 1,       // TERMA geometry flag (one int32)
     kScratchSpectrum = 2,     // complex work grids of the spectral passes
-    kScratchSlots = 3
+    kScratchTermaList = 3,    // TERMA brick lists by cost class + counters
+    kScratchSlots = 4
 };
 // zero_new: a newly allocated buffer is zero-filled (synchronously, under
 // the allocator's lock) before it is handed out.

@@ -8,6 +8,7 @@
 // traversal and the aperture lookups match the FP64 oracle bit for bit; the
 // depth integral uses FP32 with Kahan compensation and the fluence and TERMA
 // arithmetic is FP32.
+#include <algorithm>
 #include <climits>
 #include <cmath>
 #include <cstring>
@@ -148,87 +149,122 @@ __device__ __forceinline__ double sample_pos(const TermaParams& P, int a,
     return pos - P.spacing[a] / 2 + off / 2 + off * k;
 }
 
-// True when no raster cell of [a, b] (cells floor(c) + 1999, as doubles)
-// can read a value inside the aperture's bounding cells [lo, hi).  Cells
-// outside [-1, 3998] read 0 (conehead.py:53-67) and cell -1 reads the last
-// row or column (Python's negative indexing).
-__device__ __forceinline__ bool axis_misses(double a, double b, int lo,
-                                            int hi)
+// Isocentre-plane image of a corner of a sub-sample box: as plane_image,
+// with the plane-hit quotient from a reciprocal refined by one Newton step
+// (relative error ~1e-15, so the cell coordinates are off by ~1e-11 cell).
+// Only the enclosure of the corner images is used, with a margin of 1e-6
+// cell, so the samples' exact images stay inside it.
+__device__ __forceinline__ PlaneImage corner_image(const TermaParams& P,
+                                                   double ps0, double ps1,
+                                                   double ps2)
 {
-    if (b < -1.0 || a > 3998.0)
+    PlaneImage c;
+    const double u0 = P.src[0] - ps0, u1 = P.src[1] - ps1,
+                 u2 = P.src[2] - ps2;
+    c.ndotu = dot3(P.vy, u0, u1, u2);
+    double r = __builtin_amdgcn_rcp(c.ndotu);
+    r = r * (2.0 - c.ndotu * r);
+    const double si = -P.ndotn_src * r;
+    const double pp0 = P.src[0] + si * u0;
+    const double pp1 = P.src[1] + si * u1;
+    const double pp2 = P.src[2] + si * u2;
+    c.ci = dot3(P.tr0, pp0, pp1, pp2) * 100.0;
+    c.cj = dot3(P.tr2, pp0, pp1, pp2) * 100.0;
+    return c;
+}
+
+// Raster cells floor(c -/+ 1e-6) + 1999 (a cell coordinate widened by the
+// rounding margin), clamped to [-2, 3999]: every test below treats cells
+// below -1 or above 3998 alike, so the clamp changes no decision.
+__device__ __forceinline__ int cell_of(double c)
+{
+    const double f = fmin(fmax(floor(c) + 1999.0, -2.0), 3999.0);
+    return (int)f;
+}
+
+// True when no raster cell of [a, b] can read a value inside the
+// aperture's bounding cells [lo, hi).  Cells outside [-1, 3998] read 0
+// (conehead.py:53-67) and cell -1 reads the last row or column (Python's
+// negative indexing).
+__device__ __forceinline__ bool axis_misses(int a, int b, int lo, int hi)
+{
+    if (b < -1 || a > 3998)
         return true;
-    const double A = fmax(a, 0.0), B = fmin(b, 3998.0);
-    const bool in = A <= B && B >= (double)lo && A <= (double)(hi - 1);
-    const bool wrap = a <= -1.0 && 3999 >= lo && 3999 <= hi - 1;
+    const int A = max(a, 0), B = min(b, 3998);
+    const bool in = A <= B && B >= lo && A <= hi - 1;
+    const bool wrap = a <= -1 && 3999 >= lo && 3999 <= hi - 1;
     return !(in || wrap);
 }
 
-// The transmission shared by every sub-sample whose cell coordinates lie
-// in [lo_i, hi_i] x [lo_j, hi_j] (widened by 1e-6 cell for rounding), or -1
-// when it is not one known constant.
-__device__ __forceinline__ float box_value(const TermaParams& P, double lo_i,
-                                           double hi_i, double lo_j,
-                                           double hi_j)
-{
-    const double a_i = floor(lo_i - 1e-6) + 1999.0;
-    const double b_i = floor(hi_i + 1e-6) + 1999.0;
-    const double a_j = floor(lo_j - 1e-6) + 1999.0;
-    const double b_j = floor(hi_j + 1e-6) + 1999.0;
-    if (axis_misses(a_i, b_i, P.ap_bbox[0], P.ap_bbox[1]) ||
-        axis_misses(a_j, b_j, P.ap_bbox[2], P.ap_bbox[3]))
-        return 0.0f;
-    if (P.ap_kind == CH_APERTURE_RASTER ||
-        !(a_i >= 0.0 && b_i <= 3998.0 && a_j >= 0.0 && b_j <= 3998.0))
-        return -1.0f;
-    const int i0 = (int)a_i, i1 = (int)b_i, j0 = (int)a_j, j1 = (int)b_j;
-    return P.ap_kind == CH_APERTURE_MLC
-               ? mlc_block_value(P.ap_table, i0, i1, j0, j1)
-               : rects_block_value(P.ap_table, P.ap_n, i0, i1, j0, j1);
-}
-
-// Extent of a set of plane images: cell box, and whether every n.u has
-// one sign and |n.u| >= 1e-6 (then the central projection maps the convex
-// hull of the points into the convex hull of their images).
-struct ImageBox {
-    double lo_i, hi_i, lo_j, hi_j;
-    bool pos, neg, ok;
+// Cell box of a set of plane images and whether every n.u has one sign and
+// |n.u| >= 1e-6 (then the central projection maps the convex hull of the
+// points into the convex hull of their images).  Cells are monotone in the
+// coordinate, so the box of the cells is the cells of the box.
+struct CellBox {
+    int a_i, b_i, a_j, b_j;
+    int flags;  // 1: all |n.u| >= 1e-6, 2: all n.u > 0, 4: all n.u <= 0
     __device__ __forceinline__ void init(const PlaneImage& c)
     {
-        lo_i = hi_i = c.ci;
-        lo_j = hi_j = c.cj;
-        ok = fabs(c.ndotu) >= 1e-6;
-        pos = c.ndotu > 0;
-        neg = !pos;
+        a_i = cell_of(c.ci - 1e-6);
+        b_i = cell_of(c.ci + 1e-6);
+        a_j = cell_of(c.cj - 1e-6);
+        b_j = cell_of(c.cj + 1e-6);
+        flags = (fabs(c.ndotu) >= 1e-6 ? 1 : 0) | (c.ndotu > 0 ? 2 : 4);
     }
-    __device__ __forceinline__ void add(const ImageBox& o)
+    __device__ __forceinline__ void add(const CellBox& o)
     {
-        lo_i = fmin(lo_i, o.lo_i);
-        hi_i = fmax(hi_i, o.hi_i);
-        lo_j = fmin(lo_j, o.lo_j);
-        hi_j = fmax(hi_j, o.hi_j);
-        ok = ok && o.ok;
-        pos = pos && o.pos;
-        neg = neg && o.neg;
+        a_i = min(a_i, o.a_i);
+        b_i = max(b_i, o.b_i);
+        a_j = min(a_j, o.a_j);
+        b_j = max(b_j, o.b_j);
+        flags &= o.flags;
     }
+    __device__ __forceinline__ bool ok() const
+    {
+        return (flags & 1) && (flags & 6);
+    }
+    // The transmission shared by every sub-sample whose image lies in the
+    // box, or -1 when it is not one known constant: 0 outside the
+    // aperture's bounding cells or the raster, or the leaf table's or the
+    // rectangles' value over the block.
     __device__ __forceinline__ float value(const TermaParams& P) const
     {
-        return ok && (pos || neg) ? box_value(P, lo_i, hi_i, lo_j, hi_j)
-                                  : -1.0f;
+        if (!ok())
+            return -1.0f;
+        if (axis_misses(a_i, b_i, P.ap_bbox[0], P.ap_bbox[1]) ||
+            axis_misses(a_j, b_j, P.ap_bbox[2], P.ap_bbox[3]))
+            return 0.0f;
+        if (P.ap_kind == CH_APERTURE_RASTER ||
+            !(a_i >= 0 && b_i <= 3998 && a_j >= 0 && b_j <= 3998))
+            return -1.0f;
+        return P.ap_kind == CH_APERTURE_MLC
+                   ? mlc_block_value(P.ap_table, a_i, b_i, a_j, b_j)
+                   : rects_block_value(P.ap_table, P.ap_n, a_i, b_i, a_j,
+                                       b_j);
     }
 };
 
-__device__ __forceinline__ ImageBox shfl_xor_box(const ImageBox& b, int m)
+// Reduction of a CellBox over each group of 8 lanes with DPP moves: lanes
+// exchange within quads (xor 1, xor 2), then mirror within half rows of 8.
+template <int CTRL>
+__device__ __forceinline__ int dpp(int v)
 {
-    ImageBox o;
-    o.lo_i = __shfl_xor(b.lo_i, m, 64);
-    o.hi_i = __shfl_xor(b.hi_i, m, 64);
-    o.lo_j = __shfl_xor(b.lo_j, m, 64);
-    o.hi_j = __shfl_xor(b.hi_j, m, 64);
-    o.ok = __shfl_xor((int)b.ok, m, 64) != 0;
-    o.pos = __shfl_xor((int)b.pos, m, 64) != 0;
-    o.neg = __shfl_xor((int)b.neg, m, 64) != 0;
-    return o;
+    return __builtin_amdgcn_mov_dpp(v, CTRL, 0xF, 0xF, false);
 }
+template <int CTRL>
+__device__ __forceinline__ void add_dpp(CellBox& b)
+{
+    CellBox o;
+    o.a_i = dpp<CTRL>(b.a_i);
+    o.b_i = dpp<CTRL>(b.b_i);
+    o.a_j = dpp<CTRL>(b.a_j);
+    o.b_j = dpp<CTRL>(b.b_j);
+    o.flags = dpp<CTRL>(b.flags);
+    b.add(o);
+}
+constexpr int kDppQuadXor1 = 0xB1;      // quad_perm [1, 0, 3, 2]
+constexpr int kDppQuadXor2 = 0x4E;      // quad_perm [2, 3, 0, 1]
+constexpr int kDppHalfMirror = 0x141;   // row_half_mirror
 
 // conehead.py:50-68 at cell coordinates (times 100, not floored) of an
 // image: raster cell floor(c) + 2000 - 1, a -1 index wrapping to the last
@@ -270,8 +306,9 @@ __device__ __forceinline__ void wave_lds_sync()
 // become three masks that select the crossing taken and the axis that
 // advances.  kWalkBatch steps are traversed first and their density loads
 // issued together before the first is used; lanes past the grid exit keep
-// stepping harmlessly on voxel 0 with their sums frozen.  Byte offsets are
-// 32-bit (ch_terma requires 4 X Y Z < 2^32).
+// stepping with their sums frozen, their range-checked loads reading 0
+// wherever they land.  Byte offsets are 32-bit (ch_terma requires
+// 4 X Y Z < 2^32).
 constexpr int kWalkBatch = 8;
 
 __device__ __forceinline__ float depth_walk(const float* __restrict__ rho,
@@ -289,7 +326,9 @@ __device__ __forceinline__ float depth_walk(const float* __restrict__ rho,
     double tprev = 0.0;
     float de = 0.0f, dc = 0.0f;
     bool live = true;
-    const char* base = (const char*)rho;
+    // range-checked loads: past the grid a read returns 0 (and is not used)
+    const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)rho, (short)0, 4 * X * Y * Z, kBufferFlags);
     while (live) {
         uint32_t off[kWalkBatch];
         float seg[kWalkBatch];
@@ -302,7 +341,7 @@ __device__ __forceinline__ float depth_walk(const float* __restrict__ rho,
             const bool m1 = !c01 & c12;
             const bool m2 = !(m0 | m1);
             const double t = m0 ? t0 : (m1 ? t1 : t2);
-            off[k] = live ? 4u * (uint32_t)lin : 0u;
+            off[k] = 4u * (uint32_t)lin;
             seg[k] = (float)(t - tprev);
             tprev = t;
             const double n0 = t0 + dt0, n1 = t1 + dt1, n2 = t2 + dt2;
@@ -318,7 +357,8 @@ __device__ __forceinline__ float depth_walk(const float* __restrict__ rho,
         float r[kWalkBatch];
 #pragma unroll
         for (int k = 0; k < kWalkBatch; ++k)
-            r[k] = *(const float*)(base + off[k]);
+            r[k] = __int_as_float(__builtin_amdgcn_raw_buffer_load_b32(
+                rsrc, (int)off[k], 0, 0));
 #pragma unroll
         for (int k = 0; k < kWalkBatch; ++k) {
             // a step past the exit adds dc: yk = 0, de keeps its bits (dc
@@ -336,90 +376,173 @@ __device__ __forceinline__ float depth_walk(const float* __restrict__ rho,
 // Sub-sample values staged per wavefront by the compacted hit test.
 constexpr int kSampleBuf = 512;
 
-// Thread layout: a workgroup covers a 4 x 4 x 16 brick of voxels, each
-// wavefront a 4 x 4 x 4 cube, so a wavefront's voxels are either all well
-// inside, all outside or on the edge of the field and their rays toward
-// the source have similar lengths.
-//
-// Hit test (conehead.py:78-120), in three stages that each give the exact
-// blocked fraction of the loop over n^3 sub-samples:
-//  1. the brick: 8 lanes project the 8 corners of the box spanned by every
-//     sub-sample of the wavefront's 64 voxels.  The sub-samples are convex
-//     combinations of those corners, and the projection through the source
-//     onto the isocentre plane keeps convex combinations while n.u keeps
-//     one sign (n.u is affine in the point), so when the corner images
-//     have one sign of n.u and their cell box (widened by 1e-6 cell for
-//     rounding) reads one known transmission v -- 0 outside the aperture's
-//     bounding cells or the raster, or a constant of the leaf table or the
-//     rectangles -- every sub-sample of the brick reads v and each voxel's
-//     fraction is v / n^3 added n^3 times, as the loop adds it;
-//  2. otherwise each voxel applies the same test to its own 8 corners;
-//  3. the voxels still open have their sub-samples spread over all 64
-//     lanes (a few voxels at a time), each lane evaluating one sub-sample
-//     into LDS; each voxel then sums its own values in the loop's order,
-//     so the result is bitwise that of the loop.  When most of the lanes
-//     need samples the loop runs on each lane instead.
-__global__ __launch_bounds__(256) void terma_kernel(TermaParams P)
+// TERMA in two phases.  The hit test (conehead.py:78-120) runs in three
+// stages that each give the exact blocked fraction of the loop over n^3
+// sub-samples: (1) the brick of 64 voxels at once (brick_value), (2) each
+// voxel's own 8 corners by the same argument, (3) the sub-samples of the
+// voxels still open, spread over all 64 lanes (a few voxels at a time),
+// each lane evaluating one sub-sample into LDS; each voxel then sums its
+// own values in the loop's order, so the result is bitwise that of the
+// loop (when most of the lanes need samples the loop runs on each lane).
+// Phase 1 sorts the bricks by stage 1: those outside the aperture are done
+// there; the rest -- the voxels the beam reaches, a seventh of a 128^3 grid
+// for a 10 x 10 cm field -- go to phase 2 in order of their estimated cost,
+// the longest depth walks first, so they start together and the cheaper
+// ones fill in behind them.
+// Bricks of 4 x 4 x 4 voxels, one per wavefront, indexed z fastest: the
+// corner voxel of brick b, and lane l's voxel (x0 + l / 16, y0 + l / 4 % 4,
+// z0 + l % 4).
+struct Brick {
+    int x0, y0, z0;
+};
+__device__ __forceinline__ Brick brick_of(const TermaParams& P, int b)
 {
-    __shared__ double s_samp[4][kSampleBuf];
-    __shared__ int32_t s_lane[4][64];
-    const int64_t nv = (int64_t)P.X * P.Y * P.Z;
-    const int nbz = (P.Z + 15) / 16, nby = (P.Y + 3) / 4;
-    const int bz = blockIdx.x % nbz;
-    const int br = blockIdx.x / nbz;
+    const int nbz = (P.Z + 3) / 4, nby = (P.Y + 3) / 4;
+    Brick k;
+    k.z0 = (b % nbz) * 4;
+    k.y0 = ((b / nbz) % nby) * 4;
+    k.x0 = (b / nbz / nby) * 4;
+    return k;
+}
+
+// Hit-test stage 1 for the whole wavefront (every lane must call it): the
+// box spanned by every sub-sample of the brick's 64 voxels, corner (l & 7)
+// per lane.  The sub-samples are convex combinations of those corners, and
+// the projection through the source onto the isocentre plane keeps convex
+// combinations while n.u keeps one sign (n.u is affine in the point), so
+// when the corner images have one sign of n.u and their cell box (widened by
+// 1e-6 cell for rounding) reads one known transmission v, every sub-sample
+// of the brick reads v.  Returns v, or -1.
+__device__ __forceinline__ float brick_value(const TermaParams& P,
+                                             const Brick& k, int l)
+{
+    const int ns = P.samples;
+    const int c = l & 7;
+    const int xa = (c & 1) ? min(k.x0 + 3, P.X - 1) : k.x0;
+    const int ya = (c & 2) ? min(k.y0 + 3, P.Y - 1) : k.y0;
+    const int za = (c & 4) ? min(k.z0 + 3, P.Z - 1) : k.z0;
+    const int s0 = (c & 1) ? ns - 1 : 0;
+    const int s1 = (c & 2) ? ns - 1 : 0;
+    const int s2 = (c & 4) ? ns - 1 : 0;
+    CellBox b;
+    b.init(corner_image(P, sample_pos(P, 0, xa, P.off[0], s0),
+                        sample_pos(P, 1, ya, P.off[1], s1),
+                        sample_pos(P, 2, za, P.off[2], s2)));
+    add_dpp<kDppQuadXor1>(b);
+    add_dpp<kDppQuadXor2>(b);
+    add_dpp<kDppHalfMirror>(b);
+    return b.value(P);
+}
+
+// Cost classes of the bricks that need work: kCostClasses buckets of the
+// estimated depth-walk length of the brick's centre (voxels to the grid exit
+// toward the source), plus 2 n^3 steps for a brick on the aperture's edge
+// (its sub-samples cost about that much).
+constexpr int kCostClasses = 16;
+// Each class is split over kShards counters and sub-lists (brick b in shard
+// b % kShards), so the appending atomics of phase 1 spread over
+// kCostClasses x kShards addresses instead of queueing on a few.
+constexpr int kShards = 16;
+constexpr int kCounters = kCostClasses * kShards;   // 4 per lane
+
+__device__ __forceinline__ int brick_cost_class(const TermaParams& P,
+                                                const Brick& k, bool edge)
+{
+    const float c[3] = {k.x0 + 1.5f, k.y0 + 1.5f, k.z0 + 1.5f};
+    const int n[3] = {P.X, P.Y, P.Z};
+    float d[3], t = 3.0e38f;
+    for (int a = 0; a < 3; ++a) {
+        d[a] = (float)((P.src[a] - P.origin[a]) / P.spacing[a]) - c[a];
+        if (d[a] > 0.0f)
+            t = fminf(t, ((float)n[a] - 0.5f - c[a]) / d[a]);
+        else if (d[a] < 0.0f)
+            t = fminf(t, (c[a] + 0.5f) / -d[a]);
+    }
+    float steps = 0.0f;
+    for (int a = 0; a < 3; ++a)
+        steps += t * fabsf(d[a]);
+    const int n3 = P.samples * P.samples * P.samples;
+    const float full = (float)(P.X + P.Y + P.Z + 2 * n3);
+    const float cost = steps + (edge ? 2.0f * n3 : 0.0f);
+    const int q = (int)(cost * kCostClasses / full);
+    return q < 0 ? 0 : (q >= kCostClasses ? kCostClasses - 1 : q);
+}
+
+// Capacity of one shard's list: the bricks b with b % kShards == shard.
+__host__ __device__ __forceinline__ int shard_cap(int n_bricks)
+{
+    return (n_bricks + kShards - 1) / kShards;
+}
+
+// Phase 1: one wavefront per brick runs stage 1 of the hit test.  A brick
+// that reads 0 everywhere gets T = 0 here; every other brick (all of them
+// when the debug planes are asked for) is appended to its cost class for
+// phase 2, with its stage-1 result.
+__global__ __launch_bounds__(256) void terma_sort_kernel(
+    TermaParams P, int32_t* __restrict__ list, int32_t* __restrict__ counts,
+    int32_t n_bricks)
+{
+    const int b = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (b >= n_bricks)
+        return;  // wavefront-uniform
     const int l = threadIdx.x & 63;
-    const int w = threadIdx.x >> 6;
-    const int x0 = (br / nby) * 4, y0 = (br % nby) * 4, z0 = bz * 16 + w * 4;
+    const Brick k = brick_of(P, b);
+    const float v = brick_value(P, k, l);
+    if (v == 0.0f && P.debug == nullptr) {
+        const int x = k.x0 + (l >> 4), y = k.y0 + ((l >> 2) & 3),
+                  z = k.z0 + (l & 3);
+        if (x < P.X && y < P.Y && z < P.Z && !P.accumulate)
+            P.terma[((int64_t)x * P.Y + y) * P.Z + z] = 0.0f;
+        return;
+    }
+    if (l == 0) {
+        const int q = brick_cost_class(P, k, v < 0.0f);
+        const int c = q * kShards + b % kShards;
+        const int at = atomicAdd(&counts[c], 1);
+        // entry: brick index and stage-1 value (0 -> 0, 1 -> 1, -1 -> 2)
+        list[(int64_t)c * shard_cap(n_bricks) + at] =
+            b * 4 + (v > 0.0f ? 1 : (v < 0.0f ? 2 : 0));
+    }
+}
+
+// TERMA of one brick (one wavefront, every lane calls it) whose stage-1 hit
+// test gave v: hit-test stages 2 and 3, then per voxel the radiological
+// depth, off-axis distance, fluence, softening, horn and the polyenergetic
+// sum (conehead.py:71-377).  Returns the lane's T (0 outside the grid).
+__device__ __forceinline__ float terma_brick(const TermaParams& P,
+                                             const Brick& k, float v, int l,
+                                             double* __restrict__ s_samp,
+                                             int32_t* __restrict__ s_lane,
+                                             bool& bad)
+{
+    const int64_t nv = (int64_t)P.X * P.Y * P.Z;
+    const int x0 = k.x0, y0 = k.y0, z0 = k.z0;
     const int x = x0 + (l >> 4);
     const int y = y0 + ((l >> 2) & 3);
     const int z = z0 + (l & 3);
     const bool inside = x < P.X && y < P.Y && z < P.Z;
+    const int ns = P.samples;
+    const int n3i = ns * ns * ns;
+    const double n3 = P.n3;
+    const double* off = P.off;
+    const double ndotn_src = P.ndotn_src;
     float T = 0.0f;
-    if (z0 < P.Z) {  // wavefront-uniform: the brick holds grid voxels
-        const int ns = P.samples;
-        const int n3i = ns * ns * ns;
-        const double n3 = P.n3;
-        const double* off = P.off;
-        const double ndotn_src = P.ndotn_src;
-        bool bad = false;
-
-        // 1. the brick's sub-sample box, corner (l & 7) per lane
-        float v;
-        {
-            const int k = l & 7;
-            const int xa = (k & 1) ? min(x0 + 3, P.X - 1) : x0;
-            const int ya = (k & 2) ? min(y0 + 3, P.Y - 1) : y0;
-            const int za = (k & 4) ? min(z0 + 3, P.Z - 1) : z0;
-            const int s0 = (k & 1) ? ns - 1 : 0;
-            const int s1 = (k & 2) ? ns - 1 : 0;
-            const int s2 = (k & 4) ? ns - 1 : 0;
-            ImageBox b;
-            b.init(plane_image(P, sample_pos(P, 0, xa, off[0], s0),
-                               sample_pos(P, 1, ya, off[1], s1),
-                               sample_pos(P, 2, za, off[2], s2), ndotn_src));
-            b.add(shfl_xor_box(b, 1));
-            b.add(shfl_xor_box(b, 2));
-            b.add(shfl_xor_box(b, 4));
-            v = b.value(P);
-        }
-
         // 2. the voxel's own corners
         if (v < 0.0f && inside) {
-            ImageBox b;
+            CellBox b;
             for (int k = 0; k < 8; ++k) {
                 const int s0 = (k & 1) ? ns - 1 : 0;
                 const int s1 = (k & 2) ? ns - 1 : 0;
                 const int s2 = (k & 4) ? ns - 1 : 0;
-                ImageBox c;
-                c.init(plane_image(P, sample_pos(P, 0, x, off[0], s0),
-                                   sample_pos(P, 1, y, off[1], s1),
-                                   sample_pos(P, 2, z, off[2], s2),
-                                   ndotn_src));
+                CellBox c;
+                c.init(corner_image(P, sample_pos(P, 0, x, off[0], s0),
+                                    sample_pos(P, 1, y, off[1], s1),
+                                    sample_pos(P, 2, z, off[2], s2)));
                 if (k == 0)
                     b = c;
                 else
                     b.add(c);
-                if (!b.ok || !(b.pos || b.neg))
+                if (!b.ok())
                     break;
             }
             v = b.value(P);
@@ -436,8 +559,8 @@ __global__ __launch_bounds__(256) void terma_kernel(TermaParams P)
         const int n_need = __popcll(need_mask);
         const int G = n3i <= kSampleBuf ? kSampleBuf / n3i : 0;
         if (n_need > 0 && n_need <= 2 * G) {
-            double* buf = s_samp[w];
-            int32_t* lanes = s_lane[w];
+            double* buf = s_samp;
+            int32_t* lanes = s_lane;
             const int rank = __popcll(need_mask & ((1ull << l) - 1ull));
             if (need)
                 lanes[rank] = l;
@@ -563,18 +686,93 @@ __global__ __launch_bounds__(256) void terma_kernel(TermaParams P)
             const int64_t li = ((int64_t)x * P.Y + y) * P.Z + z;
             P.terma[li] = P.accumulate ? P.terma[li] + T : T;
         }
+    return T;
+}
+
+// Phase 2: wavefront g takes the g-th brick of phase 1's lists in order of
+// descending cost class, so the longest depth walks are dispatched first
+// and the rest fill in behind them (wavefronts past the last brick exit at
+// once).  Each wavefront writes its support-box partial.
+__global__ __launch_bounds__(256) void terma_kernel(
+    TermaParams P, const int32_t* __restrict__ list,
+    const int32_t* __restrict__ counts, int32_t n_bricks)
+{
+    __shared__ double s_samp[4][kSampleBuf];
+    __shared__ int32_t s_lane[4][64];
+    const int l = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int g = blockIdx.x * 4 + w;
+    // the counters in order of descending cost class, shards in order:
+    // position j = (kCostClasses - 1 - q) * kShards + shard, lane l holding
+    // j = 4 l .. 4 l + 3; incl = inclusive prefix of the lanes' sums
+    int c4[4], sum = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int j = 4 * l + i;
+        const int q = kCostClasses - 1 - j / kShards;
+        c4[i] = counts[q * kShards + j % kShards];
+        sum += c4[i];
+    }
+    int incl = sum;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int u = __shfl_up(incl, o, 64);
+        if (l >= o)
+            incl += u;
+    }
+    const int total = __shfl(incl, 63, 64);
+    float T = 0.0f;
+    int x = INT_MAX, y = 0, z = 0;
+    if (g < total) {  // wavefront-uniform
+        const uint64_t past = __ballot(incl > g);
+        const int owner = __ffsll((unsigned long long)past) - 1;
+        int before = incl - sum, pick = 0, rank = 0;
+        bool found = false;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            if (!found && g < before + c4[i]) {
+                pick = 4 * l + i;
+                rank = g - before;
+                found = true;
+            }
+            before += c4[i];
+        }
+        pick = __shfl(pick, owner, 64);
+        rank = __shfl(rank, owner, 64);
+        const int q = kCostClasses - 1 - pick / kShards;
+        const int c = q * kShards + pick % kShards;
+        const int e = list[(int64_t)c * shard_cap(n_bricks) + rank];
+        const Brick k = brick_of(P, e >> 2);
+        const float v = (e & 3) == 1 ? 1.0f : ((e & 3) == 2 ? -1.0f : 0.0f);
+        bool bad = false;
+        T = terma_brick(P, k, v, l, s_samp[w], s_lane[w], bad);
         if (__ballot(bad) != 0 && l == 0)
             atomicOr(P.status, 1);
+        x = k.x0 + (l >> 4);
+        y = k.y0 + ((l >> 2) & 3);
+        z = k.z0 + (l & 3);
     }
     if (P.box) {
-        // one partial box per block; box_finalize reduces them (no atomics)
-        int32_t bv[6] = {INT_MAX, INT_MAX, INT_MAX, INT_MIN, INT_MIN, INT_MIN};
-        if (inside && T != 0.0f) {
-            bv[0] = bv[3] = x;
-            bv[1] = bv[4] = y;
-            bv[2] = bv[5] = z;
+        // one partial box per wavefront; box_finalize reduces them
+        int32_t bx[6] = {INT_MAX, INT_MAX, INT_MAX, INT_MIN, INT_MIN,
+                         INT_MIN};
+        if (T != 0.0f) {
+            bx[0] = bx[3] = x;
+            bx[1] = bx[4] = y;
+            bx[2] = bx[5] = z;
         }
-        block_box_partial(bv, P.box);
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                bx[a] = min(bx[a], __shfl_xor(bx[a], o, 64));
+                bx[3 + a] = max(bx[3 + a], __shfl_xor(bx[3 + a], o, 64));
+            }
+        int32_t mine = bx[0];
+#pragma unroll
+        for (int a = 1; a < 6; ++a)
+            mine = l == a ? bx[a] : mine;
+        if (l < 6)
+            P.box[(int64_t)g * kBoxStride + l] = mine;
     }
 }
 
@@ -582,6 +780,15 @@ __global__ __launch_bounds__(64) void terma_prep_kernel(int32_t* status)
 {
     if (threadIdx.x == 0)
         status[0] = 0;
+}
+
+// Zeroes the cost-class counters before phase 1 (a kernel, so the call can
+// be captured in a graph).
+__global__ __launch_bounds__(64) void terma_clear_kernel(int32_t* counts,
+                                                         int n)
+{
+    for (int i = threadIdx.x; i < n; i += 64)
+        counts[i] = 0;
 }
 
 static TermaParams make_params(const ch_beam& B, const int32_t* h_shape,
@@ -690,24 +897,42 @@ ch_status ch_terma(int32_t n_beams, const ch_beam* h_beams,
         P.accumulate = B.accumulate;
         P.debug = d_debug ? d_debug + (size_t)b * 4 * nv : nullptr;
         P.status = B.d_status ? B.d_status : (int32_t*)status;
-        const unsigned blocks = (unsigned)(((h_shape[0] + 3) / 4) *
-                                           ((h_shape[1] + 3) / 4) *
-                                           ((h_shape[2] + 15) / 16));
+        // one wavefront per brick in each phase
+        const int32_t n_bricks = ((h_shape[0] + 3) / 4) *
+                                 ((h_shape[1] + 3) / 4) *
+                                 ((h_shape[2] + 3) / 4);
+        const unsigned blocks = (unsigned)((n_bricks + 3) / 4);
+        void* lists = nullptr;
+        ch_status st = stream_scratch(
+            s, kScratchTermaList,
+            ((size_t)kCounters * (shard_cap(n_bricks) + 1)) *
+                sizeof(int32_t),
+            &lists);
+        if (st != CH_OK)
+            return st;
+        int32_t* counts = (int32_t*)lists;
+        int32_t* list = counts + kCounters;
         P.box = nullptr;
         if (B.d_box) {
             void* parts = nullptr;
-            ch_status st = stream_scratch(s, kScratchBoxPartials,
-                                          (size_t)blocks * kBoxStride *
-                                              sizeof(int32_t),
-                                          &parts);
+            st = stream_scratch(s, kScratchBoxPartials,
+                                (size_t)blocks * 4 * kBoxStride *
+                                    sizeof(int32_t),
+                                &parts);
             if (st != CH_OK)
                 return st;
             P.box = (int32_t*)parts;
         }
-        hipLaunchKernelGGL(terma_kernel, dim3(blocks), dim3(256), 0, s, P);
+        hipLaunchKernelGGL(terma_clear_kernel, dim3(1), dim3(64), 0, s,
+                           counts, kCounters);
+        hipLaunchKernelGGL(terma_sort_kernel, dim3(blocks), dim3(256), 0, s,
+                           P, list, counts, n_bricks);
+        hipLaunchKernelGGL(terma_kernel, dim3(blocks), dim3(256), 0, s, P,
+                           (const int32_t*)list, (const int32_t*)counts,
+                           n_bricks);
         CH_HIP(hipGetLastError());
         if (B.d_box) {
-            launch_box_finalize(P.box, (int)blocks, B.d_box,
+            launch_box_finalize(P.box, (int)blocks * 4, B.d_box,
                                 B.accumulate ? 1 : 0, s);
             CH_HIP(hipGetLastError());
         }
