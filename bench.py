@@ -13,21 +13,35 @@ with events at every step gives latency, the per-call split and the kernel
 times.
 
 The same JSON line carries
-  roofline      the headline method's dominant kernel (the FFT form's
-                convolution pass: 8 B per complex64 bin read or written; the
-                gather: 12 B per voxel-cone update) over its launch time,
-                measured with HIP events on the stream it runs on;
+  roofline      the step's dominant kernel over its launch time, measured
+                with HIP events the library attaches to the kernels on the
+                stream they run on: TERMA (its sort pass and brick kernel;
+                8 B per voxel) or the convolution (the FFT form's
+                convolution pass: 8 B per complex64 bin read or written, or
+                the gather: 12 B per voxel-cone update), whichever takes
+                longer; `rooflines` has both and `step_roofline` the whole
+                step's algorithmic bytes over the replayed step time;
   cpu_baseline  the FP64 oracle (TERMA + convolve_c) timed on this host's
                 cores for the same beam;
   batched       64 MLC beams at random gantry angles on the same phantom,
-                96 cones, all beams' gathers in one launch;
+                96 cones, all beams' convolutions sharing launches;
   plain         the plain single-beam GPU path (per-stage synchronisation
                 and host round trips, scatter convolution with atomics) on
-                beams of the batched workload, and the batched/plain ratio.
---workload uneven   32 phantoms of differing grids, voxel sizes, beam counts
-                    and cone sets in one batch (autotuned tiles).
---workload serve    Poisson request streams drawn from the uneven workload,
-                    lambda swept 10-500 req/s, through the request loop.
+                beams of the batched workload, and the batched/plain ratio;
+  uneven        32 phantoms of differing grids, voxel sizes, beam counts and
+                cone sets as one batch, with the error of one beam against
+                the oracle;
+  serve         Poisson request streams drawn from the uneven workload,
+                lambda swept 10-500 req/s through the request loop (p50/p99
+                latency, beams/s), its capacity at 2000 req/s, and one
+                served dose against the oracle.
+voxel_cone_updates_per_s and dda_step_updates_per_s are the metric's work
+units at the delivered beam rate -- the dense count Nv x C per beam and the
+reference's count of dose += T * k executions; the FFT form reaches the same
+doses without performing them one by one, so for it they are dense
+equivalents (updates_are_dense_equivalent).
+--workload uneven   the uneven batch alone, with both convolutions.
+--workload serve    the request loop alone, 64 requests per lambda.
 """
 import argparse
 import json
@@ -217,10 +231,13 @@ def run_hetero(args, eng):
         g1 = [Event() for _ in range(K)]
         k0 = [Event() for _ in range(K)]
         k1 = [Event() for _ in range(K)]
+        q0 = [Event() for _ in range(K)]
+        q1 = [Event() for _ in range(K)]
         s.sync()
         t_start = time.perf_counter()
         ev[0].record(s)
         for i in range(K):
+            arm_launch_events(q0[i], q1[i])
             E.terma([bm], shape, case.origin, case.spacing, stream=s,
                     check=False)
             g0[i].record(s)
@@ -238,6 +255,8 @@ def run_hetero(args, eng):
                                   for i in range(K)]),
              "kernel_ms": np.array([k0[i].elapsed_ms(k1[i])
                                     for i in range(K)]),
+             "terma_kernels_ms": np.array([q0[i].elapsed_ms(q1[i])
+                                           for i in range(K)]),
              "gpu_total_ms": ev[0].elapsed_ms(ev[K]),
              "dose": d_d.download(stream=s)}
         with E.Graph.capture(s) as cap:
@@ -278,24 +297,61 @@ def run_hetero(args, eng):
     }
     traffic_file = {"gather": "gather_hetero_pmc.md",
                     "fft": "spectral_hetero_pmc.md"}
+    # the FFT form's five passes over the half spectra (complex64, 8 B):
+    # z forward (T read, X x Y x Kz written), y forward (read, X x Fy x Kz
+    # written), x convolution (read, K^ read, write), y inverse, z inverse
+    hs = 8 * shape[0] * shape[1] * kz
+    full = 8 * shape[0] * frame[1] * kz
+    step_conv_bytes = {
+        "gather": BYTES_PER_VOXEL_CONE * nv * n_cones,
+        "fft": (4 * nv + hs) + (hs + full) + kernel_bytes["fft"] +
+               (full + hs) + (hs + 4 * nv)}
     lines = 16                      # the convolution pass's line group
     while lines > 2 and frame[0] * lines > 4096:
         lines //= 2
     kernel_name = {"gather": "gather_single (direct stencil gather)",
                    "fft": "col_kernel<%d, 2> (kColConv: x transforms, "
                           "times K^, inverse x transforms)" % lines}
+    # TERMA: the sort pass and the brick kernel, bracketed by events the
+    # library attaches to them; 8 B per voxel (a 4 B density read and a 4 B
+    # TERMA write)
+    terma_bytes = 8 * nv
     for method, r in runs.items():
         k_ms = float(r["kernel_ms"].mean())
         achieved = kernel_bytes[method] / (k_ms * 1e-3) / 1e9
         # headline: the step replayed as a HIP graph, no events inside
         r["beams_per_s"] = K / r["graph_wall"]
         r["eager_beams_per_s"] = K / r["wall"]
-        r["roofline"] = {"bound": "hbm", "kernel": kernel_name[method],
-                         "achieved": achieved, "peak": HBM_PEAK_GBS,
-                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                         "algorithmic_bytes": kernel_bytes[method],
-                         "traffic": traffic_from_profile(
-                             traffic_file[method])}
+        conv = {"bound": "hbm", "kernel": kernel_name[method],
+                "achieved": achieved, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                "algorithmic_bytes": kernel_bytes[method],
+                "kernel_ms": k_ms,
+                "traffic": traffic_from_profile(traffic_file[method])}
+        t_ms = float(r["terma_kernels_ms"].mean())
+        ta = terma_bytes / (t_ms * 1e-3) / 1e9
+        terma = {"bound": "hbm",
+                 "kernel": "terma_sort_kernel + terma_kernel (TERMA: hit "
+                           "test, FP64 depth walk, fluence, spectrum sum)",
+                 "achieved": ta, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                 "frac": ta / HBM_PEAK_GBS, "algorithmic_bytes": terma_bytes,
+                 "kernel_ms": t_ms,
+                 "traffic": traffic_from_profile("terma_hetero_pmc.md")}
+        r["rooflines"] = {"terma": terma, "convolution": conv}
+        # the step's dominant kernel (the longer of the two)
+        r["roofline"] = terma if t_ms >= k_ms else conv
+        # the whole step: every kernel's algorithmic bytes over the step
+        step_bytes = terma_bytes + step_conv_bytes[method]
+        sa = step_bytes / (r["graph_ms"] * 1e-3) / 1e9
+        r["step_roofline"] = {
+            "bound": "hbm", "achieved": sa, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": sa / HBM_PEAK_GBS,
+            "algorithmic_bytes": step_bytes,
+            "what": "TERMA's 8 B per voxel plus the convolution's "
+                    "algorithmic bytes (%s) over the graph-replayed step" %
+                    ("the five FFT passes' complex64 reads and writes and "
+                     "K^" if method == "fft" else
+                     "12 B per voxel-cone update")}
     if args.method == "auto":
         head = max(runs, key=lambda m: runs[m]["beams_per_s"])
     else:
@@ -385,9 +441,12 @@ def run_hetero(args, eng):
             "kernel_ms": float(r["kernel_ms"].mean()),
             "convolve_call_ms": float(r["call_ms"].mean()),
             "graph_ms_per_step": r["graph_ms"],
+            "terma_kernels_ms": float(r["terma_kernels_ms"].mean()),
             "dose_max_abs_err_over_dmax":
                 r.get("dose_max_abs_err_over_dmax"),
-            "roofline": r["roofline"]}
+            "roofline": r["roofline"],
+            "rooflines": r["rooflines"],
+            "step_roofline": r["step_roofline"]}
     methods["fft"]["frame"] = list(frame)
     return {
         "value": beams_per_s,
@@ -403,9 +462,14 @@ def run_hetero(args, eng):
                       "HIP graph and replayed K times; its dose is checked "
                       "bitwise against the eager step's",
         },
+        # work units of the metric at the delivered beam rate: the dense
+        # count Nv x C per beam and the reference's own count of dose +=
+        # T * k executions; the FFT form reaches the same doses without
+        # performing either one by one, so both are dense-equivalent rates
         "voxel_cone_updates_per_s": nv * n_cones * beams_per_s,
         "dda_step_updates_per_s": (cpu["dda_step_updates"] * beams_per_s
                                    if cpu else None),
+        "updates_are_dense_equivalent": head == "fft",
         "latency_ms": {"p50": float(np.percentile(step_ms, 50)),
                        "p99": float(np.percentile(step_ms, 99))},
         "gpu_ms_per_step": gpu_total_ms / K,
@@ -415,11 +479,14 @@ def run_hetero(args, eng):
         "kernel_ms": methods[head]["kernel_ms"],
         "convolve_call_ms": float(call_ms.mean()),
         "terma_ms": float((step_ms - call_ms).mean()),
+        "terma_kernels_ms": methods[head]["terma_kernels_ms"],
         "methods": methods,
         "error_vs_fp64_oracle": result_err,
         "dropin_convolve_c_ms": dropin_ms,
         "physics_gather_ms": physics_ms,
         "roofline": runs[head]["roofline"],
+        "rooflines": runs[head]["rooflines"],
+        "step_roofline": runs[head]["step_roofline"],
         "cpu_baseline": cpu,
     }
 
@@ -543,17 +610,20 @@ def tune_requests(eng, be, reqs, s):
     return seen
 
 
-def run_uneven(args, eng):
+def run_uneven(args, eng, methods=("gather", "fft"), cases_reqs=None):
+    """The uneven batch (32 phantoms) as one batch with each method; returns
+    (line, the oracle dose of the smallest case's first beam and its index
+    or None)."""
     from conehead_amd.batch import BatchEngine
     from conehead_amd.device import Stream
     log("uneven: generating phantoms")
-    cases, reqs = uneven_requests()
+    cases, reqs = cases_reqs or uneven_requests()
     s = Stream()
     nb = sum(len(c.beams) for c in cases)
     upd = sum(c.voxel_cone_updates() * len(c.beams) for c in cases)
     steps = max(1, min(args.steps, 5))
     per, outs, tiles = {}, {}, {}
-    for method in ("gather", "fft"):
+    for method in methods:
         log("uneven: %s" % method)
         be = BatchEngine(eng, method=method)
         if method == "gather":
@@ -578,6 +648,7 @@ def run_uneven(args, eng):
                     "distinct_tiles": len(set(tiles.values()))},
          "voxel_cone_updates_per_s": upd / (ms * 1e-3),
          "methods": per}
+    check = None
     if not args.no_cpu_baseline:
         i = int(np.argmin([c.n_voxels * c.cones.n_cones for c in cases]))
         ref, _, _, _ = oracle_beam(cases[i], 0, cpu_threads(args))
@@ -587,16 +658,22 @@ def run_uneven(args, eng):
         r["error_vs_fp64_oracle"] = {"case": cases[i].name, "beam": 0,
                                      "dose_max_abs_err_over_dmax":
                                          per[head]["error_vs_fp64_oracle"]}
-    return r
+        check = (i, ref)
+    return r, check
 
 
 # --------------------------------------------------------------- c5 ------
-def run_serve(args, eng):
+def run_serve(args, eng, n_req=64, cases_reqs=None, check=None):
+    """Poisson request streams from the uneven workload through the
+    request loop, lambda swept 10-500 req/s, then its capacity at 2000
+    req/s.  check = (pool index, oracle dose of that request's first beam):
+    that request rides first in the lambda = 500 stream and its served
+    dose is compared with the oracle."""
     from conehead_amd.batch import BatchEngine
     from conehead_amd.device import Stream
     from conehead_amd.serve import DoseServer, poisson_arrivals, replay
     log("serve: generating phantoms")
-    cases, pool = uneven_requests()
+    cases, pool = cases_reqs or uneven_requests()
     method = "fft" if args.method == "auto" else args.method
     be = BatchEngine(eng, method=method)
     s = Stream()
@@ -605,14 +682,18 @@ def run_serve(args, eng):
     be.release(be.collect(be.launch(pool, s), download=False))  # warm-up
     rng = np.random.default_rng(99)
     sweep = []
-    n_req = 64
+    served_err = None
     for lam in (10, 20, 50, 100, 200, 500):
         reqs = [pool[int(i)] for i in rng.integers(0, len(pool), n_req)]
+        if check is not None and lam == 500:
+            reqs[0] = pool[check[0]]
         arr = poisson_arrivals(float(lam), n_req, seed=99 + lam)
         with DoseServer(be, max_inflight=2, max_batch_jobs=64) as srv:
             futs = replay(srv, reqs, arr)
-            for f in futs:
-                f.result(timeout=300)
+            res = [f.result(timeout=300) for f in futs]
+        if check is not None and lam == 500:
+            served_err = err_over_dmax(res[0][0], check[1])
+        del res
         st = srv.stats()
         st["lambda_req_per_s"] = lam
         sweep.append(st)
@@ -641,7 +722,12 @@ def run_serve(args, eng):
         log("serve lambda=%d, doses %s: %.0f beams/s" %
             (lam, "to the host" if download else "left in HBM",
              st["beams_per_s"]))
+    err = None
+    if served_err is not None:
+        err = {"case": cases[check[0]].name, "beam": 0, "lambda": 500,
+               "dose_max_abs_err_over_dmax": served_err}
     return {"metric": METRIC, "value": best["beams_per_s"],
+            "error_vs_fp64_oracle": err,
             "unit": "beams/s", "n_gpus": 1, "steps": n_req * len(sweep),
             "warmup": 1, "ms_per_step": None, "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None,
@@ -650,9 +736,10 @@ def run_serve(args, eng):
                          "geometry",
             "data": "synthetic (seeded phantoms; EDKnrc varian_6 kernel)",
             "config": {"workload": "serving: Poisson arrivals (seed 99), "
-                                   "64 requests per lambda drawn from the "
+                                   "%d requests per lambda drawn from the "
                                    "uneven batch, resident densities, HBM "
-                                   "budget admission, 2 stream slots",
+                                   "budget admission, 2 stream slots" %
+                                   n_req,
                        "method": method},
             "sweep": sweep, "saturation": saturation}
 
@@ -679,7 +766,7 @@ def main():
         del a, b
 
     if args.workload == "uneven":
-        line = run_uneven(args, eng)
+        line, _ = run_uneven(args, eng)
     elif args.workload == "serve":
         line = run_serve(args, eng)
     else:
@@ -703,6 +790,18 @@ def main():
             pl["batched_over_plain"] = bt["beams_per_s"] / pl["beams_per_s"]
             line["batched"] = bt
             line["plain"] = pl
+            # the uneven batch and the request loop on its requests, with
+            # the headline convolution
+            cr = uneven_requests()
+            m = line["config"]["method"]
+            un, check = run_uneven(args, eng, methods=(m,), cases_reqs=cr)
+            line["uneven"] = {k: un[k] for k in (
+                "config", "ms_per_step", "value", "unit",
+                "voxel_cone_updates_per_s", "error_vs_fp64_oracle")
+                if k in un}
+            sv = run_serve(args, eng, n_req=32, cases_reqs=cr, check=check)
+            line["serve"] = {k: sv[k] for k in (
+                "config", "sweep", "saturation", "error_vs_fp64_oracle")}
     line["device"] = _lib.device_info()
     line["host"] = host_info()
     print(json.dumps(line))

@@ -871,6 +871,7 @@ ch_status ch_terma(int32_t n_beams, const ch_beam* h_beams,
     ch_status sst = stream_scratch(s, kScratchStatus, 256, &status, true);
     if (sst != CH_OK)
         return sst;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
     for (int32_t b = 0; b < n_beams; ++b) {
         const ch_beam& B = h_beams[b];
         const ch_aperture& A = B.aperture;
@@ -925,11 +926,17 @@ ch_status ch_terma(int32_t n_beams, const ch_beam* h_beams,
         }
         hipLaunchKernelGGL(terma_clear_kernel, dim3(1), dim3(64), 0, s,
                            counts, kCounters);
-        hipLaunchKernelGGL(terma_sort_kernel, dim3(blocks), dim3(256), 0, s,
-                           P, list, counts, n_bricks);
-        hipLaunchKernelGGL(terma_kernel, dim3(blocks), dim3(256), 0, s, P,
-                           (const int32_t*)list, (const int32_t*)counts,
-                           n_bricks);
+        // the caller's armed event pair (ch_set_launch_events) brackets
+        // the first beam's phase 1 and the last beam's phase 2
+        if (b == 0 && ev1 == nullptr && !capturing(s))
+            (void)take_launch_events(&ev0, &ev1);
+        hipExtLaunchKernelGGL(terma_sort_kernel, dim3(blocks), dim3(256), 0,
+                              s, b == 0 ? ev0 : nullptr, nullptr, 0, P, list,
+                              counts, n_bricks);
+        hipExtLaunchKernelGGL(terma_kernel, dim3(blocks), dim3(256), 0, s,
+                              nullptr, b == n_beams - 1 ? ev1 : nullptr, 0,
+                              P, (const int32_t*)list, (const int32_t*)counts,
+                              n_bricks);
         CH_HIP(hipGetLastError());
         if (B.d_box) {
             launch_box_finalize(P.box, (int)blocks * 4, B.d_box,

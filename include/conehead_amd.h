@@ -440,7 +440,9 @@ ch_status ch_last_kernel_ms(float* ms);
  * disarms the pair (read it with ch_event_elapsed once `stop` completed).
  * Pass two nulls to disarm.  Launches that prepare the gather (padding,
  * support and slab boxes) and the FFT form's other passes fall outside the
- * pair.  No counterpart in the reference: it is the measurement hook of
+ * pair.  ch_terma takes an armed pair too: `start` at the start of the
+ * first beam's first TERMA kernel, `stop` at the end of the last beam's
+ * last one.  No counterpart in the reference: it is the measurement hook of
  * the benchmark. */
 ch_status ch_set_launch_events(void* start, void* stop);
 
