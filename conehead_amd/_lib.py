@@ -201,6 +201,9 @@ SIGNATURES = {
     "ch_terma": (ctypes.c_int32, [ctypes.c_int32, ctypes.POINTER(Beam),
                                   c_i32p, c_f64p, c_f64p, c_vp, c_vp]),
     "ch_terma_check": (ctypes.c_int32, [c_vp]),
+    "ch_terma_noalloc": (ctypes.c_int32, [ctypes.c_int32,
+                                          ctypes.POINTER(Beam), c_i32p,
+                                          c_f64p, c_f64p, c_vp, c_vp]),
     "ch_terma_rays": (ctypes.c_int32, [ctypes.POINTER(Beam), c_i32p, c_f64p,
                                        c_f64p, ctypes.c_int32, c_vp,
                                        ctypes.c_int32, c_vp, c_vp, c_vp, c_vp,

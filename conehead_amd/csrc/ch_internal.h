@@ -15,6 +15,9 @@ namespace ch {
 void set_error(const char* fmt, ...);
 ch_status hip_fail(hipError_t e, const char* what, const char* file, int line);
 int32_t num_cus();
+// Resolves the current device's address of the allocation-free TERMA
+// scratch (terma.hip); ch_init calls it with the device current.
+ch_status terma_init_device();
 
 #define CH_HIP(call)                                                        \
     do {                                                                    \

@@ -316,6 +316,9 @@ ch_status ch_init(int32_t device, int32_t* n_cu, int64_t* hbm_bytes,
     hipDeviceProp_t p;
     CH_HIP(hipGetDeviceProperties(&p, device));
     g_num_cus = p.multiProcessorCount;
+    const ch_status st = terma_init_device();
+    if (st != CH_OK)
+        return st;
     if (n_cu)
         *n_cu = p.multiProcessorCount;
     if (hbm_bytes)

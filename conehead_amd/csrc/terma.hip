@@ -12,6 +12,7 @@
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 #include "aperture.h"
 #include "dda_core.h"
@@ -843,6 +844,120 @@ static TermaParams make_params(const ch_beam& B, const int32_t* h_shape,
     return P;
 }
 
+// Device memory one beam's TERMA launches need beyond its inputs and
+// outputs: the cost-class counters and brick lists of phase 1 and, with a
+// support box, one partial box per phase-2 wavefront.
+struct TermaScratch {
+    int32_t* counts;
+    int32_t* list;
+    int32_t* box_parts;  // null without a support box
+};
+
+inline int32_t terma_bricks(const int32_t* shape)
+{
+    return ((shape[0] + 3) / 4) * ((shape[1] + 3) / 4) *
+           ((shape[2] + 3) / 4);
+}
+inline size_t terma_list_bytes(int32_t n_bricks)
+{
+    return (size_t)kCounters * (shard_cap(n_bricks) + 1) * sizeof(int32_t);
+}
+inline size_t terma_box_bytes(int32_t n_bricks)
+{
+    return (size_t)((n_bricks + 3) / 4) * 4 * kBoxStride * sizeof(int32_t);
+}
+
+// Checks beam b and fills its kernel parameters.
+static ch_status terma_params(const ch_beam& B, int b, const int32_t* shape,
+                              const double* origin, const double* spacing,
+                              float* d_debug, int32_t* stream_flag,
+                              TermaParams* out)
+{
+    const ch_aperture& A = B.aperture;
+    CH_REQUIRE(B.d_density && B.d_terma, "beam %d: null device buffer", b);
+    CH_REQUIRE((A.kind == CH_APERTURE_RASTER && A.d_raster) ||
+                   (A.kind == CH_APERTURE_RECTS && A.n >= 0 &&
+                    (A.n == 0 || A.d_table)) ||
+                   (A.kind == CH_APERTURE_MLC && A.d_table),
+               "beam %d: bad aperture (kind %d)", b, A.kind);
+    CH_REQUIRE(B.samples >= 1 && B.samples <= 16,
+               "beam %d: fluenceResampling %d", b, B.samples);
+    CH_REQUIRE(B.n_energy >= 0 && B.n_energy <= CH_MAX_ENERGIES,
+               "beam %d: %d energies", b, B.n_energy);
+    const int64_t nv = (int64_t)shape[0] * shape[1] * shape[2];
+    TermaParams P = make_params(B, shape, origin, spacing);
+    P.ap_kind = A.kind;
+    P.ap_n = A.n;
+    P.ap_table = A.d_table;
+    P.raster = A.d_raster;
+    for (int q = 0; q < 4; ++q)
+        P.ap_bbox[q] = A.bbox[q];
+    P.density = B.d_density;
+    P.terma = B.d_terma;
+    P.accumulate = B.accumulate;
+    P.debug = d_debug ? d_debug + (size_t)b * 4 * nv : nullptr;
+    P.status = B.d_status ? B.d_status : stream_flag;
+    *out = P;
+    return CH_OK;
+}
+
+// One beam's launches: clear the counters, phase 1, phase 2 and the
+// support box.  ev0 / ev1 (nullable) are attached to the start of phase 1
+// and the end of phase 2.
+static ch_status terma_launch(TermaParams P, const ch_beam& B,
+                              int32_t n_bricks, const TermaScratch& sc,
+                              hipEvent_t ev0, hipEvent_t ev1, hipStream_t s)
+{
+    const unsigned blocks = (unsigned)((n_bricks + 3) / 4);
+    P.box = B.d_box ? sc.box_parts : nullptr;
+    hipLaunchKernelGGL(terma_clear_kernel, dim3(1), dim3(64), 0, s,
+                       sc.counts, kCounters);
+    hipExtLaunchKernelGGL(terma_sort_kernel, dim3(blocks), dim3(256), 0, s,
+                          ev0, nullptr, 0, P, sc.list, sc.counts, n_bricks);
+    hipExtLaunchKernelGGL(terma_kernel, dim3(blocks), dim3(256), 0, s,
+                          nullptr, ev1, 0, P, (const int32_t*)sc.list,
+                          (const int32_t*)sc.counts, n_bricks);
+    CH_HIP(hipGetLastError());
+    if (B.d_box) {
+        launch_box_finalize(P.box, (int)blocks * 4, B.d_box,
+                            B.accumulate ? 1 : 0, s);
+        CH_HIP(hipGetLastError());
+    }
+    return CH_OK;
+}
+
+// The allocation-free variant's scratch: one array per device, resolved to
+// a device address per device once (hipGetSymbolAddress) and used only
+// through that address.
+constexpr size_t kNoAllocScratchBytes = (size_t)16 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+constexpr int kMaxDevices = 64;
+static void* g_scratch_ptr[kMaxDevices];
+static std::mutex g_scratch_ptr_mu;
+
+static ch_status noalloc_scratch(void** out)
+{
+    int dev = 0;
+    CH_HIP(hipGetDevice(&dev));
+    CH_REQUIRE(dev >= 0 && dev < kMaxDevices, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_ptr_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        CH_HIP(hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch)));
+        CH_REQUIRE(ptr, "hipGetSymbolAddress gave no address");
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return CH_OK;
+}
+
+// Called by ch_init with the device current.
+ch_status terma_init_device()
+{
+    void* ptr = nullptr;
+    return noalloc_scratch(&ptr);
+}
+
 }  // namespace ch
 
 using namespace ch;
@@ -868,81 +983,95 @@ ch_status ch_terma(int32_t n_beams, const ch_beam* h_beams,
     // one check covers every call queued since the last one; it is zeroed
     // when it is allocated.  A beam with its own d_status uses that instead.
     void* status = nullptr;
-    ch_status sst = stream_scratch(s, kScratchStatus, 256, &status, true);
-    if (sst != CH_OK)
-        return sst;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    for (int32_t b = 0; b < n_beams; ++b) {
-        const ch_beam& B = h_beams[b];
-        const ch_aperture& A = B.aperture;
-        CH_REQUIRE(B.d_density && B.d_terma, "beam %d: null device buffer",
-                   b);
-        CH_REQUIRE((A.kind == CH_APERTURE_RASTER && A.d_raster) ||
-                       (A.kind == CH_APERTURE_RECTS && A.n >= 0 &&
-                        (A.n == 0 || A.d_table)) ||
-                       (A.kind == CH_APERTURE_MLC && A.d_table),
-                   "beam %d: bad aperture (kind %d)", b, A.kind);
-        CH_REQUIRE(B.samples >= 1 && B.samples <= 16,
-                   "beam %d: fluenceResampling %d", b, B.samples);
-        CH_REQUIRE(B.n_energy >= 0 && B.n_energy <= CH_MAX_ENERGIES,
-                   "beam %d: %d energies", b, B.n_energy);
-        TermaParams P = make_params(B, h_shape, h_origin, h_spacing);
-        P.ap_kind = A.kind;
-        P.ap_n = A.n;
-        P.ap_table = A.d_table;
-        P.raster = A.d_raster;
-        for (int q = 0; q < 4; ++q)
-            P.ap_bbox[q] = A.bbox[q];
-        P.density = B.d_density;
-        P.terma = B.d_terma;
-        P.accumulate = B.accumulate;
-        P.debug = d_debug ? d_debug + (size_t)b * 4 * nv : nullptr;
-        P.status = B.d_status ? B.d_status : (int32_t*)status;
-        // one wavefront per brick in each phase
-        const int32_t n_bricks = ((h_shape[0] + 3) / 4) *
-                                 ((h_shape[1] + 3) / 4) *
-                                 ((h_shape[2] + 3) / 4);
-        const unsigned blocks = (unsigned)((n_bricks + 3) / 4);
-        void* lists = nullptr;
-        ch_status st = stream_scratch(
-            s, kScratchTermaList,
-            ((size_t)kCounters * (shard_cap(n_bricks) + 1)) *
-                sizeof(int32_t),
-            &lists);
-        if (st != CH_OK)
-            return st;
-        int32_t* counts = (int32_t*)lists;
-        int32_t* list = counts + kCounters;
-        P.box = nullptr;
-        if (B.d_box) {
+    ch_status st = stream_scratch(s, kScratchStatus, 256, &status, true);
+    if (st != CH_OK)
+        return st;
+    const int32_t n_bricks = terma_bricks(h_shape);
+    TermaScratch sc;
+    void* lists = nullptr;
+    st = stream_scratch(s, kScratchTermaList, terma_list_bytes(n_bricks),
+                        &lists);
+    if (st != CH_OK)
+        return st;
+    sc.counts = (int32_t*)lists;
+    sc.list = sc.counts + kCounters;
+    sc.box_parts = nullptr;
+    for (int32_t b = 0; b < n_beams; ++b)
+        if (h_beams[b].d_box) {
             void* parts = nullptr;
             st = stream_scratch(s, kScratchBoxPartials,
-                                (size_t)blocks * 4 * kBoxStride *
-                                    sizeof(int32_t),
-                                &parts);
+                                terma_box_bytes(n_bricks), &parts);
             if (st != CH_OK)
                 return st;
-            P.box = (int32_t*)parts;
+            sc.box_parts = (int32_t*)parts;
+            break;
         }
-        hipLaunchKernelGGL(terma_clear_kernel, dim3(1), dim3(64), 0, s,
-                           counts, kCounters);
-        // the caller's armed event pair (ch_set_launch_events) brackets
-        // the first beam's phase 1 and the last beam's phase 2
-        if (b == 0 && ev1 == nullptr && !capturing(s))
-            (void)take_launch_events(&ev0, &ev1);
-        hipExtLaunchKernelGGL(terma_sort_kernel, dim3(blocks), dim3(256), 0,
-                              s, b == 0 ? ev0 : nullptr, nullptr, 0, P, list,
-                              counts, n_bricks);
-        hipExtLaunchKernelGGL(terma_kernel, dim3(blocks), dim3(256), 0, s,
-                              nullptr, b == n_beams - 1 ? ev1 : nullptr, 0,
-                              P, (const int32_t*)list, (const int32_t*)counts,
-                              n_bricks);
-        CH_HIP(hipGetLastError());
-        if (B.d_box) {
-            launch_box_finalize(P.box, (int)blocks * 4, B.d_box,
-                                B.accumulate ? 1 : 0, s);
-            CH_HIP(hipGetLastError());
-        }
+    // the caller's armed event pair (ch_set_launch_events) brackets the
+    // first beam's phase 1 and the last beam's phase 2
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    if (!capturing(s))
+        (void)take_launch_events(&ev0, &ev1);
+    for (int32_t b = 0; b < n_beams; ++b) {
+        TermaParams P;
+        st = terma_params(h_beams[b], b, h_shape, h_origin, h_spacing,
+                          d_debug, (int32_t*)status, &P);
+        if (st != CH_OK)
+            return st;
+        st = terma_launch(P, h_beams[b], n_bricks, sc,
+                          b == 0 ? ev0 : nullptr,
+                          b == n_beams - 1 ? ev1 : nullptr, s);
+        if (st != CH_OK)
+            return st;
+    }
+    return CH_OK;
+}
+
+ch_status ch_terma_noalloc(int32_t n_beams, const ch_beam* h_beams,
+                           const int32_t h_shape[3],
+                           const double h_origin[3],
+                           const double h_spacing[3], float* d_debug,
+                           void* stream)
+{
+    CH_REQUIRE(n_beams >= 0, "n_beams < 0");
+    if (n_beams == 0)
+        return CH_OK;
+    CH_REQUIRE(h_beams && h_shape && h_origin && h_spacing,
+               "ch_terma_noalloc: null pointer");
+    CH_REQUIRE(h_shape[0] > 0 && h_shape[1] > 0 && h_shape[2] > 0,
+               "ch_terma_noalloc: empty grid");
+    const int64_t nv = (int64_t)h_shape[0] * h_shape[1] * h_shape[2];
+    CH_REQUIRE(nv < (1LL << 30), "ch_terma_noalloc: grid too large (4 X Y "
+                                 "Z must stay below 2^32)");
+    const int32_t n_bricks = terma_bricks(h_shape);
+    const size_t need = terma_list_bytes(n_bricks) +
+                        terma_box_bytes(n_bricks);
+    CH_REQUIRE(need <= kNoAllocScratchBytes,
+               "ch_terma_noalloc: a %d x %d x %d grid needs %zu bytes of "
+               "scratch, more than its %zu (use ch_terma)",
+               h_shape[0], h_shape[1], h_shape[2], need,
+               kNoAllocScratchBytes);
+    for (int32_t b = 0; b < n_beams; ++b)
+        CH_REQUIRE(h_beams[b].d_status,
+                   "ch_terma_noalloc: beam %d has no d_status (the geometry "
+                   "flag is the caller's output here)", b);
+    void* base = nullptr;
+    ch_status st = noalloc_scratch(&base);
+    if (st != CH_OK)
+        return st;
+    TermaScratch sc;
+    sc.counts = (int32_t*)base;
+    sc.list = sc.counts + kCounters;
+    sc.box_parts = (int32_t*)((char*)base + terma_list_bytes(n_bricks));
+    hipStream_t s = (hipStream_t)stream;
+    for (int32_t b = 0; b < n_beams; ++b) {
+        TermaParams P;
+        st = terma_params(h_beams[b], b, h_shape, h_origin, h_spacing,
+                          d_debug, nullptr, &P);
+        if (st != CH_OK)
+            return st;
+        st = terma_launch(P, h_beams[b], n_bricks, sc, nullptr, nullptr, s);
+        if (st != CH_OK)
+            return st;
     }
     return CH_OK;
 }

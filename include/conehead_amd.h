@@ -381,6 +381,23 @@ ch_status ch_terma(int32_t n_beams, const ch_beam* h_beams,
                    const int32_t h_shape[3], const double h_origin[3],
                    const double h_spacing[3], float* d_debug, void* stream);
 
+/* ch_terma without any allocation, for callers that cannot allocate device
+ * memory (code under HIP-graph capture on a fresh stream, tight serving
+ * loops, harnesses that forbid hipMalloc); no counterpart in the
+ * reference.  Same arguments, kernels and results as ch_terma, except:
+ * every beam must carry d_status (its geometry flag, an output the caller
+ * zeroes; ch_terma_check does not see these calls), and the launch scratch
+ * (cost-class lists and support-box partials) is a fixed 16 MiB array per
+ * GPU, held until the process ends, so grids needing more (about 11 M
+ * voxels and beyond) are refused with CH_ERR_ARG.  Calls that overlap on
+ * one GPU (on different streams or threads) share that array and race;
+ * calls queued on one stream do not. */
+ch_status ch_terma_noalloc(int32_t n_beams, const ch_beam* h_beams,
+                           const int32_t h_shape[3],
+                           const double h_origin[3],
+                           const double h_spacing[3], float* d_debug,
+                           void* stream);
+
 /* The FP64 traversals of the TERMA rays toward the source from n voxels
  * (d_vox_in: n x 3 indices), with the same direction arithmetic as
  * ch_terma: d_dirs (n x 3) receives each unit direction, d_vox / d_t /
