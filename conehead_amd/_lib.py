@@ -185,6 +185,21 @@ SIGNATURES = {
     "ch_resample_linear": (ctypes.c_int32, [
         c_vp, c_i32p, c_f64p, c_f64p, c_vp, c_i32p, c_f64p, c_f64p,
         ctypes.c_int32, c_vp]),
+    "ch_ct_to_density": (ctypes.c_int32, [
+        c_vp, ctypes.c_int32, ctypes.c_int64, c_f64p, c_f64p, ctypes.c_int32,
+        c_vp, ctypes.POINTER(ctypes.c_float), c_vp, c_vp]),
+    "ch_beamset_create": (ctypes.c_int32, [
+        ctypes.c_int32, ctypes.POINTER(Beam), c_i32p, c_f64p, c_f64p,
+        ctypes.POINTER(c_vp)]),
+    "ch_beamset_update": (ctypes.c_int32, [c_vp, ctypes.c_int32,
+                                           ctypes.POINTER(Beam), c_vp]),
+    "ch_terma_beamset": (ctypes.c_int32, [c_vp, c_vp]),
+    "ch_beamset_flags": (ctypes.c_int32, [c_vp, c_i32p, c_vp]),
+    "ch_beamset_destroy": (ctypes.c_int32, [c_vp]),
+    "ch_convset_create": (ctypes.c_int32, [
+        ctypes.c_int32, ctypes.POINTER(ConvJob), ctypes.POINTER(c_vp)]),
+    "ch_convolve_fft_set": (ctypes.c_int32, [c_vp, c_vp]),
+    "ch_convset_destroy": (ctypes.c_int32, [c_vp]),
     "ch_graph_begin": (ctypes.c_int32, [c_vp]),
     "ch_graph_end": (ctypes.c_int32, [c_vp, ctypes.POINTER(c_vp)]),
     "ch_graph_launch": (ctypes.c_int32, [c_vp, c_vp]),

@@ -319,6 +319,7 @@ class BatchEngine:
         s = stream or self._default_stream()
         eng = self.engine
         jobs, groups, outputs, temps, pending = [], [], [], [], []
+        aps = []
         s2 = self._companion(s) if self.pipeline > 0 else None
         events = []
         if self.method == "fft":
@@ -343,7 +344,11 @@ class BatchEngine:
                      for _ in range(n_jobs)]
             for b, (src, blk, st) in enumerate(req.beams):
                 j = b if req.per_beam else 0
-                bs = E.beam_struct(src, st, eng.aperture(blk, s), d_rho,
+                # the aperture's tables stay referenced until collect (the
+                # engine's cache may drop them meanwhile)
+                ap = eng.aperture(blk, s)
+                aps.append(ap)
+                bs = E.beam_struct(src, st, ap, d_rho,
                                    t_bufs[j], boxes[j])
                 bs.accumulate = 0 if (req.per_beam or b == 0) else 1
                 bs.d_status = flags.ptr + 4 * ri
@@ -408,6 +413,7 @@ class BatchEngine:
         ev = Event().record(s)
         h = BatchHandle(list(requests), outputs, temps, ev, s, staged, flags)
         h.events = events          # kept alive until the batch is collected
+        h.apertures = aps
         return h
 
     def collect(self, handle, download=True, errors="raise"):
@@ -464,6 +470,7 @@ class BatchEngine:
                 handle.flags = None
             handle.temps = []
             handle.events = []
+            handle.apertures = []
         if failed and errors == "raise":
             raise _lib_error("no intersection or line is within plane "
                              "(requests %s of the batch)" % failed)
@@ -478,3 +485,140 @@ class BatchEngine:
     def run(self, requests, stream=None):
         """Launch and collect one batch."""
         return self.collect(self.launch(requests, stream))
+
+
+class BatchGraph:
+    """A batch of requests captured once and replayed by one graph launch.
+
+    The requests' TERMA runs from device-resident beam sets (one
+    E.BeamSet per dose grid: the beams of the grid share each launch) and
+    their convolutions from one device-resident E.ConvSet (the FFT form,
+    one launch per pass for all beams of a plan), all recorded in one
+    graph on the graph's own stream; nothing is staged from the host when
+    it is replayed.  Outputs, TERMA grids and densities are fixed device
+    buffers: update() loads new requests of the same structure (grids,
+    beam counts, output modes and cone sets) for the next replay -- new
+    densities, beam geometry and apertures.  The result of a replay is
+    bitwise the result of BatchEngine(method="fft") on the same requests.
+    """
+
+    def __init__(self, requests, engine=None):
+        self.engine = engine or E.Engine()
+        eng = self.engine
+        self.stream = Stream()
+        self.requests = list(requests)
+        for r in self.requests:
+            if not spectral_ok(r.shape):
+                raise ValueError("grid %s exceeds the FFT form's frames" %
+                                 (r.shape,))
+        s = self.stream
+        self.rho = [DeviceArray(r.shape) for r in self.requests]
+        self.t_bufs = [[DeviceArray(r.shape) for _ in range(r.n_jobs)]
+                       for r in self.requests]
+        self.outs = [[DeviceArray(r.shape) for _ in range(r.n_jobs)]
+                     for r in self.requests]
+        jobs = []
+        for ri, r in enumerate(self.requests):
+            plan = eng.plan(r.shape, r.spacing, r.cones)
+            jobs.extend((plan, t, d, False)
+                        for t, d in zip(self.t_bufs[ri], self.outs[ri]))
+        self.sets = []
+        beams, keep = self._load(self.requests)
+        for (key, idx), (bl, kp) in zip(self._grids(), zip(beams, keep)):
+            self.sets.append(E.BeamSet(bl, *key, keep=kp))
+        self.conv = E.ConvSet(jobs, stream=s)
+        # one eager pass sizes the stream's FFT work grids, then capture
+        self._run_eager()
+        s.sync()
+        self._check()
+        with E.Graph.capture(s) as cap:
+            for bs in self.sets:
+                bs.run(s)
+            self.conv.run(s)
+        self.graph = cap.graph
+
+    def _grids(self):
+        """(shape, origin, spacing) -> request indices, in first-seen order."""
+        out = {}
+        for ri, r in enumerate(self.requests):
+            key = (r.shape, tuple(float(v) for v in r.origin),
+                   tuple(float(v) for v in r.spacing))
+            out.setdefault(key, []).append(ri)
+        return list(out.items())
+
+    def _load(self, requests):
+        """Uploads the requests' densities; ch_beam lists per grid."""
+        eng, s = self.engine, self.stream
+        beams, keep = [], []
+        for key, idx in self._grids():
+            bl, kp = [], []
+            for ri in idx:
+                r = requests[ri]
+                self.rho[ri].upload(r.densities, s)
+                for b, (src, blk, st) in enumerate(r.beams):
+                    j = b if r.per_beam else 0
+                    ap = eng.aperture(blk, s)
+                    bs = E.beam_struct(src, st, ap, self.rho[ri],
+                                       self.t_bufs[ri][j], None)
+                    bs.accumulate = 0 if (r.per_beam or b == 0) else 1
+                    bl.append(bs)
+                    kp.append(ap)
+            beams.append(bl)
+            keep.append(kp)
+        return beams, keep
+
+    def _run_eager(self):
+        for bs in self.sets:
+            bs.run(self.stream)
+        self.conv.run(self.stream)
+
+    def _check(self):
+        for bs in self.sets:
+            if bs.flags(self.stream).any():
+                raise _lib_error("no intersection or line is within plane")
+
+    def update(self, requests):
+        """Loads new requests of the same structure for the next launch()."""
+        requests = list(requests)
+        if len(requests) != len(self.requests):
+            raise ValueError("update needs %d requests" % len(self.requests))
+        for a, b in zip(self.requests, requests):
+            if (a.shape, a.n_jobs, len(a.beams), a.per_beam, a.cones) != \
+                    (b.shape, b.n_jobs, len(b.beams), b.per_beam, b.cones) \
+                    or not (np.array_equal(a.origin, b.origin) and
+                            np.array_equal(a.spacing, b.spacing)):
+                raise ValueError("update must keep grids, beam counts, "
+                                 "output modes and cone sets")
+        beams, keep = self._load(requests)
+        for bs, bl, kp in zip(self.sets, beams, keep):
+            bs.update(bl, self.stream, keep=kp)
+        self.requests = requests
+
+    def launch(self):
+        """Queues one replay; returns its event."""
+        self.graph.launch(self.stream)
+        return Event().record(self.stream)
+
+    def results(self):
+        """Waits for the stream; per request float32 [n_jobs][X][Y][Z].
+        Raises the reference's geometry error if a replay since the last
+        call had a ray in the source plane."""
+        self.stream.sync()
+        self._check()
+        out = []
+        for r, outs in zip(self.requests, self.outs):
+            arr = np.empty((len(outs),) + r.shape, np.float32)
+            for j, d in enumerate(outs):
+                d.download(arr[j], stream=self.stream)
+            out.append(arr)
+        return out
+
+    def close(self):
+        self.stream.sync()
+        if getattr(self, "graph", None) is not None:
+            self.graph.close()
+            self.graph = None
+        for bs in self.sets:
+            bs.close()
+        self.conv.close()
+        self.sets = []

@@ -89,21 +89,30 @@ class Conehead:
         The dose grid is the phantom's own (conehead/conehead.py:536) unless
         settings['dose_grid'] = (size, origin, spacing) names another one;
         the densities are then resampled on the GPU (ch_resample_linear)
-        with the reference's interpolation and layout."""
+        with the reference's interpolation and layout.  A CtPhantom's
+        densities are made on the GPU from its CT numbers
+        (ch_ct_to_density) and resampled there."""
         eng = self.engine
         swap = settings.get("density_xy_swap", True)
         grid = settings.get("dose_grid")
         s = self.stream
-        if grid is None:
+        on_device = hasattr(phantom, "device_densities")  # CtPhantom
+        if grid is None and not on_device:
             self.dose_grid = DoseGrid(phantom.size, phantom.origin,
                                       phantom.spacing)
             dens = dose_grid_densities(phantom, swap)
             d_rho = DeviceArray.from_host(dens, stream=s)
         else:
+            # resampling the phantom's own grid onto itself gives every
+            # node's value, i.e. dose_grid_densities bit for bit
             from .density import resample
-            self.dose_grid = DoseGrid(*grid)
-            d_src = DeviceArray.from_host(phantom.densities, stream=s)
-            d_rho = resample(d_src, phantom.densities.shape, phantom.origin,
+            self.dose_grid = DoseGrid(*grid) if grid is not None else \
+                DoseGrid(phantom.size, phantom.origin, phantom.spacing)
+            if on_device:
+                d_src = phantom.device_densities(stream=s)
+            else:
+                d_src = DeviceArray.from_host(phantom.densities, stream=s)
+            d_rho = resample(d_src, tuple(phantom.size), phantom.origin,
                              phantom.spacing, self.dose_grid.size,
                              self.dose_grid.origin, self.dose_grid.spacing,
                              swap_xy=swap, stream=s)

@@ -202,22 +202,36 @@ void* stream_scratch_peek(hipStream_t s, int slot);
 // Free a stream's scratch and gather workspace (before it is destroyed).
 void release_stream_scratch(hipStream_t s);
 // What a live captured graph reads stays valid while the graph lives:
-//  capture_pin(s, p) -- while `s` is being captured, records plan p in the
-//    capture (once) and pins it: ch_plan_destroy of a pinned plan is
-//    deferred to the destruction of its last graph, and its spectrum cannot
-//    be released;
+//  capture_hold(s, h) -- while `s` is being captured, records object h (a
+//    plan, beam set or convolution set) in the capture (once) and pins it:
+//    destroying a pinned object is deferred to the destruction of its last
+//    graph (h->release is called then), and a plan's spectrum cannot be
+//    released;
 //  stream_graph_holds(s) -- live graphs captured on `s`: while there are
 //    any, the stream's scratch and gather workspace must not be
 //    reallocated and the stream must not be destroyed.
+struct GraphHeld {
+    int32_t pins = 0;        // live graphs that read the object
+    bool doomed = false;     // destroyed while pinned: freed when unpinned
+    void (*release)(GraphHeld*) = nullptr;
+};
+void capture_hold(hipStream_t s, GraphHeld* h);
 void capture_pin(hipStream_t s, const ch_plan* p);
 int stream_graph_holds(hipStream_t s);
+int held_pins(const GraphHeld* h);
 int plan_pins(const ch_plan* p);
-// True when `p` is pinned: it is then marked to be freed with its last
-// graph (plan_release is called then).
+// True when `h` is pinned: it is then marked to be freed with its last
+// graph.
+bool defer_free(GraphHeld* h);
 bool plan_defer_free(ch_plan* p);
 void plan_release(ch_plan* p);
 void release_gather_workspace(hipStream_t s);
-void release_spectral_tables(hipStream_t s);
+// Copies `bytes` of host data into a device table read by the kernels
+// queued next on stream s (a per-stream ring of slots staged through pinned
+// memory) and returns its device address.  Refused while s is captured.
+ch_status stage_table(hipStream_t s, const void* h_src, size_t bytes,
+                      void** d_out);
+void release_stage_tables(hipStream_t s);
 // Pads two grids of plan p's shape into the stream's gather workspace
 // (the layout the gathers read) and returns the interior voxel (0, 0, 0)
 // of each copy.
@@ -302,7 +316,7 @@ __device__ __forceinline__ void vrange_t(int v, int lo, int hi, int step,
 
 }  // namespace ch
 
-struct ch_plan {
+struct ch_plan : ch::GraphHeld {
     int32_t shape[3];
     double vsize[3];
     int32_t n_cones;
@@ -313,10 +327,6 @@ struct ch_plan {
     int32_t max_taps;
     int32_t tile[3];
     int32_t accum_mode;
-    // live graphs that read this plan, and whether ch_plan_destroy was
-    // called while there were some (guarded by the graph registry's lock)
-    int32_t pins;
-    bool doomed;
     int32_t* d_lin;
     float* d_k;
     double* d_k64;

@@ -219,6 +219,7 @@ ch_status ch_plan_create(const int32_t h_shape[3], const double h_vsize[3],
                    h_cone_row[c], n_rows);
 
     ch_plan* p = new ch_plan();
+    p->release = [](GraphHeld* h) { plan_release(static_cast<ch_plan*>(h)); };
     memset(p->shape, 0, sizeof p->shape);
     for (int a = 0; a < 3; ++a) {
         p->shape[a] = h_shape[a];

@@ -14,6 +14,9 @@
 // itertools.product order.
 #include "ch_internal.h"
 
+#include <cmath>
+#include <cstring>
+
 namespace ch {
 
 struct Axis {
@@ -100,6 +103,51 @@ __global__ __launch_bounds__(256) void resample_kernel(
     dst[li] = out;
 }
 
+// CT numbers to densities: a piecewise-linear calibration curve evaluated
+// as numpy.interp does (clamped to the end values outside the curve, the
+// node's own density at a node, slope * (x - x_j) + y_j between nodes, FP64
+// without contraction), then a per-label override for the voxels of a
+// label map (structures forced to a density).  Both tables travel in the
+// kernel arguments.
+struct CtTable {
+    double hu[CH_CT_MAX_POINTS];
+    double rho[CH_CT_MAX_POINTS];
+    double slope[CH_CT_MAX_POINTS];
+    float override_rho[256];  // NaN: keep the curve's density
+    int n;
+};
+
+__device__ __forceinline__ float ct_density(const CtTable& c, double x)
+{
+    if (x < c.hu[0])
+        return (float)c.rho[0];
+    if (x >= c.hu[c.n - 1])
+        return (float)c.rho[c.n - 1];
+    int j = 0;  // the last node at or below x
+    for (int k = 1; k < c.n - 1; ++k)
+        j = x >= c.hu[k] ? k : j;
+    if (x == c.hu[j])
+        return (float)c.rho[j];
+    return (float)(c.slope[j] * (x - c.hu[j]) + c.rho[j]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void ct_kernel(
+    const T* __restrict__ hu, const uint8_t* __restrict__ labels, int64_t n,
+    CtTable c, float* __restrict__ rho)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    float v = ct_density(c, (double)hu[i]);
+    if (labels) {
+        const float o = c.override_rho[labels[i]];
+        if (o == o)
+            v = o;
+    }
+    rho[i] = v;
+}
+
 static Axis axis(double origin, double spacing, int n)
 {
     Axis a;
@@ -148,6 +196,53 @@ ch_status ch_resample_linear(const float* d_src, const int32_t src_shape[3],
         axis(dst_origin[0], dst_spacing[0], dst_shape[0]),
         axis(dst_origin[1], dst_spacing[1], dst_shape[1]),
         axis(dst_origin[2], dst_spacing[2], dst_shape[2]), xy_swap ? 1 : 0);
+    CH_HIP(hipGetLastError());
+    return CH_OK;
+}
+
+ch_status ch_ct_to_density(const void* d_ct, int32_t ct_type, int64_t n,
+                           const double* curve_hu, const double* curve_rho,
+                           int32_t n_points, const uint8_t* d_labels,
+                           const float* override_rho, float* d_rho,
+                           void* stream)
+{
+    CH_REQUIRE(d_ct && d_rho && curve_hu && curve_rho,
+               "ch_ct_to_density: null pointer");
+    CH_REQUIRE(ct_type == CH_CT_I16 || ct_type == CH_CT_F32,
+               "ch_ct_to_density: unknown CT number type %d", ct_type);
+    CH_REQUIRE(n >= 0 && n < (1LL << 40), "ch_ct_to_density: bad count");
+    CH_REQUIRE(n_points >= 1 && n_points <= CH_CT_MAX_POINTS,
+               "ch_ct_to_density: the curve needs 1 to %d points (got %d)",
+               CH_CT_MAX_POINTS, n_points);
+    CH_REQUIRE(!d_labels || override_rho,
+               "ch_ct_to_density: a label map needs its override table");
+    CtTable c;
+    memset(&c, 0, sizeof(c));
+    c.n = n_points;
+    for (int k = 0; k < n_points; ++k) {
+        CH_REQUIRE(std::isfinite(curve_hu[k]) && std::isfinite(curve_rho[k]),
+                   "ch_ct_to_density: curve point %d is not finite", k);
+        CH_REQUIRE(k == 0 || curve_hu[k] > curve_hu[k - 1],
+                   "ch_ct_to_density: curve CT numbers must increase "
+                   "(point %d)", k);
+        c.hu[k] = curve_hu[k];
+        c.rho[k] = curve_rho[k];
+    }
+    for (int k = 0; k + 1 < n_points; ++k)
+        c.slope[k] = (c.rho[k + 1] - c.rho[k]) / (c.hu[k + 1] - c.hu[k]);
+    for (int k = 0; k < 256; ++k)
+        c.override_rho[k] = override_rho ? override_rho[k] : NAN;
+    if (n == 0)
+        return CH_OK;
+    const dim3 grid((unsigned)((n + 255) / 256));
+    if (ct_type == CH_CT_I16)
+        hipLaunchKernelGGL(ct_kernel<int16_t>, grid, dim3(256), 0,
+                           (hipStream_t)stream, (const int16_t*)d_ct,
+                           d_labels, n, c, d_rho);
+    else
+        hipLaunchKernelGGL(ct_kernel<float>, grid, dim3(256), 0,
+                           (hipStream_t)stream, (const float*)d_ct, d_labels,
+                           n, c, d_rho);
     CH_HIP(hipGetLastError());
     return CH_OK;
 }

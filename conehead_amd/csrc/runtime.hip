@@ -1,4 +1,5 @@
 // runtime.hip -- device, memory, stream and event plumbing of the C-ABI.
+#include <algorithm>
 #include <climits>
 #include <cstdarg>
 #include <cstdio>
@@ -220,41 +221,45 @@ void* stream_scratch_peek(hipStream_t s, int slot)
     return nullptr;
 }
 
-// Graphs and what they read.  A capture records every plan an entry point
-// uses on the capturing stream (capture_pin); the graph made from it pins
-// those plans and holds the capture stream's scratch until it is
-// destroyed.
+// Graphs and what they read.  A capture records every object an entry
+// point uses on the capturing stream (capture_hold: plans, beam sets,
+// convolution sets); the graph made from it pins those objects and holds
+// the capture stream's scratch until it is destroyed.
 struct ActiveCapture {
     hipStream_t stream;
-    std::vector<ch_plan*> plans;
+    std::vector<GraphHeld*> held;
 };
 static std::vector<ActiveCapture> g_active;
 static std::vector<std::pair<hipStream_t, int>> g_holds;
 static std::mutex g_graph_mu;
 
-static void unpin_locked(const std::vector<ch_plan*>& plans)
+static void unpin_locked(const std::vector<GraphHeld*>& held)
 {
-    for (ch_plan* p : plans)
-        if (--p->pins == 0 && p->doomed)
-            plan_release(p);
+    for (GraphHeld* h : held)
+        if (--h->pins == 0 && h->doomed)
+            h->release(h);
 }
 
-void capture_pin(hipStream_t s, const ch_plan* cp)
+void capture_hold(hipStream_t s, GraphHeld* h)
 {
-    if (!cp || !capturing(s))
+    if (!h || !capturing(s))
         return;
-    ch_plan* p = const_cast<ch_plan*>(cp);
     std::lock_guard<std::mutex> lock(g_graph_mu);
     for (auto& a : g_active) {
         if (a.stream != s)
             continue;
-        for (ch_plan* q : a.plans)
-            if (q == p)
+        for (GraphHeld* q : a.held)
+            if (q == h)
                 return;
-        a.plans.push_back(p);
-        ++p->pins;
+        a.held.push_back(h);
+        ++h->pins;
         return;
     }
+}
+
+void capture_pin(hipStream_t s, const ch_plan* cp)
+{
+    capture_hold(s, const_cast<ch_plan*>(cp));
 }
 
 int stream_graph_holds(hipStream_t s)
@@ -266,33 +271,133 @@ int stream_graph_holds(hipStream_t s)
     return 0;
 }
 
-int plan_pins(const ch_plan* p)
+int held_pins(const GraphHeld* h)
 {
     std::lock_guard<std::mutex> lock(g_graph_mu);
-    return p->pins;
+    return h->pins;
+}
+
+int plan_pins(const ch_plan* p)
+{
+    return held_pins(p);
+}
+
+bool defer_free(GraphHeld* h)
+{
+    std::lock_guard<std::mutex> lock(g_graph_mu);
+    if (h->pins == 0)
+        return false;
+    h->doomed = true;
+    return true;
 }
 
 bool plan_defer_free(ch_plan* p)
 {
-    std::lock_guard<std::mutex> lock(g_graph_mu);
-    if (p->pins == 0)
-        return false;
-    p->doomed = true;
-    return true;
+    return defer_free(p);
 }
 
-// Takes the capture record of `s` (its plans stay pinned).
-static std::vector<ch_plan*> take_capture(hipStream_t s)
+// Takes the capture record of `s` (its objects stay pinned).
+static std::vector<GraphHeld*> take_capture(hipStream_t s)
 {
     std::lock_guard<std::mutex> lock(g_graph_mu);
-    std::vector<ch_plan*> plans;
+    std::vector<GraphHeld*> plans;
     for (size_t i = 0; i < g_active.size(); ++i)
         if (g_active[i].stream == s) {
-            plans.swap(g_active[i].plans);
+            plans.swap(g_active[i].held);
             g_active.erase(g_active.begin() + i);
             break;
         }
     return plans;
+}
+
+// Per-stream device tables (batch pointer lists, TERMA beam parameters),
+// staged through pinned memory.  Each stream owns a ring of kTableSlots
+// slots; a slot's pinned half is rewritten only after the event of its
+// previous copy, so the host waits only when it runs kTableSlots calls ahead
+// of the stream.  The device half needs no wait: the next copy into it is
+// queued on the same stream behind the kernels that read it.
+constexpr int kTableSlots = 64;
+
+struct StageRing {
+    hipStream_t stream;
+    char* h = nullptr;
+    char* d = nullptr;
+    size_t slot_bytes = 0;
+    hipEvent_t copied[kTableSlots] = {};
+    int next = 0;
+};
+static std::vector<StageRing*> g_rings;
+static std::mutex g_ring_mu;
+
+static void ring_free(StageRing* t)
+{
+    (void)hipFree(t->d);
+    (void)hipHostFree(t->h);
+    for (hipEvent_t e : t->copied)
+        if (e)
+            (void)hipEventDestroy(e);
+    delete t;
+}
+
+ch_status stage_table(hipStream_t s, const void* h_src, size_t bytes,
+                      void** d_out)
+{
+    if (capturing(s)) {
+        set_error("a staged table would be recorded by the capture: build "
+                  "a device-resident set (ch_beamset, ch_convset) instead");
+        return CH_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lock(g_ring_mu);
+    StageRing* t = nullptr;
+    for (auto* x : g_rings)
+        if (x->stream == s)
+            t = x;
+    if (!t) {
+        t = new StageRing();
+        t->stream = s;
+        g_rings.push_back(t);
+        for (hipEvent_t& e : t->copied)
+            CH_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    if (t->slot_bytes < bytes) {
+        // queued copies and kernels of this stream may use the old blocks
+        CH_HIP(hipStreamSynchronize(s));
+        CH_HIP(hipFree(t->d));
+        CH_HIP(hipHostFree(t->h));
+        t->d = nullptr;
+        t->h = nullptr;
+        t->slot_bytes = 0;
+        const size_t cap =
+            (std::max(bytes, (size_t)4096) + 255) & ~(size_t)255;
+        CH_HIP(hipMalloc((void**)&t->d, cap * kTableSlots));
+        CH_HIP(hipHostMalloc((void**)&t->h, cap * kTableSlots,
+                             hipHostMallocDefault));
+        t->slot_bytes = cap;
+    }
+    const int k = t->next;
+    t->next = (k + 1) % kTableSlots;
+    CH_HIP(hipEventSynchronize(t->copied[k]));
+    char* h = t->h + (size_t)k * t->slot_bytes;
+    char* d = t->d + (size_t)k * t->slot_bytes;
+    memcpy(h, h_src, bytes);
+    CH_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s));
+    CH_HIP(hipEventRecord(t->copied[k], s));
+    *d_out = d;
+    return CH_OK;
+}
+
+void release_stage_tables(hipStream_t s)
+{
+    std::lock_guard<std::mutex> lock(g_ring_mu);
+    for (size_t i = 0; i < g_rings.size(); ++i) {
+        StageRing* t = g_rings[i];
+        if (t->stream != s)
+            continue;
+        (void)hipStreamSynchronize(s);
+        ring_free(t);
+        g_rings.erase(g_rings.begin() + i);
+        return;
+    }
 }
 
 }  // namespace ch
@@ -460,7 +565,7 @@ ch_status ch_stream_destroy(void* stream)
                    "ch_stream_destroy: a live graph captured on this stream "
                    "reads its scratch; destroy the graph first");
         release_gather_workspace(s);
-        release_spectral_tables(s);
+        release_stage_tables(s);
         release_stream_scratch(s);
         CH_HIP(hipStreamDestroy(s));
     }
@@ -472,7 +577,7 @@ ch_status ch_graph_begin(void* stream)
     CH_REQUIRE(stream, "ch_graph_begin: capture needs a created stream");
     hipStream_t s = (hipStream_t)stream;
     {
-        std::vector<ch_plan*> stale = take_capture(s);
+        std::vector<GraphHeld*> stale = take_capture(s);
         std::lock_guard<std::mutex> lock(g_graph_mu);
         unpin_locked(stale);
         ActiveCapture a;
@@ -482,7 +587,7 @@ ch_status ch_graph_begin(void* stream)
     const hipError_t e =
         hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) {
-        std::vector<ch_plan*> plans = take_capture(s);
+        std::vector<GraphHeld*> plans = take_capture(s);
         std::lock_guard<std::mutex> lock(g_graph_mu);
         unpin_locked(plans);
         return hip_fail(e, "hipStreamBeginCapture", __FILE__, __LINE__);
@@ -494,7 +599,7 @@ struct ch_graph {
     hipGraph_t graph;
     hipGraphExec_t exec;
     hipStream_t stream;            // capture stream (its scratch is held)
-    std::vector<ch_plan*> plans;   // pinned plans
+    std::vector<GraphHeld*> plans;   // pinned plans and sets
 };
 
 ch_status ch_graph_end(void* stream, ch_graph** graph_out)
@@ -502,7 +607,7 @@ ch_status ch_graph_end(void* stream, ch_graph** graph_out)
     CH_REQUIRE(stream && graph_out, "ch_graph_end: null argument");
     *graph_out = nullptr;
     hipStream_t s = (hipStream_t)stream;
-    std::vector<ch_plan*> plans = take_capture(s);
+    std::vector<GraphHeld*> plans = take_capture(s);
     auto unpin = [&plans]() {
         std::lock_guard<std::mutex> lock(g_graph_mu);
         unpin_locked(plans);

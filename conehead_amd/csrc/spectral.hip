@@ -1254,78 +1254,86 @@ ch_status spectral_passes(const ch_plan* p, BeamIO io, float2* A,
     return st;
 }
 
-// Per-stream device tables of a batch (TERMA pointers, dose pointers,
-// accumulate flags), staged through pinned memory. Each stream owns a ring of
-// kTableSlots slots; a slot's pinned half is rewritten only after the event of
-// its previous copy, so the host waits only when it runs kTableSlots calls
-// ahead of the stream. The device half needs no wait: the next copy into it
-// is queued on the same stream behind the kernels that read it.
-constexpr int kTableSlots = 64;
-
-struct TableSlot {
-    void* h;
-    void* d;
-    hipEvent_t copied;
+// The launches of a batch of FFT-form jobs: jobs on one plan share each
+// launch, in order of first appearance, up to kSpectralWorkBytes of work
+// grids per launch.  Each launch's device table (TERMA pointers, dose
+// pointers, accumulate flags) is laid out in `tables` at L.table.
+struct FftLaunch {
+    const ch_plan* plan;
+    int32_t nb;
+    size_t work;          // work-grid bytes
+    size_t table;         // offset of the table in the set's tables
+    size_t table_bytes;
 };
 
-// The slots of one stream share one pinned and one device block of
-// kTableSlots x slot_bytes, allocated on first use and regrown (after the
-// stream drains) only when a batch needs a larger table.
-struct BeamTable {
-    hipStream_t stream;
-    char* h = nullptr;
-    char* d = nullptr;
-    size_t slot_bytes = 0;
-    hipEvent_t copied[kTableSlots] = {};
-    int next = 0;
-};
-std::vector<BeamTable*> g_tabs;
-std::mutex g_tab_mu;
-
-void table_free(BeamTable* t)
+static ch_status fft_launches(int32_t n_jobs, const ch_conv_job* h_jobs,
+                              const char* fn, std::vector<FftLaunch>& out,
+                              std::vector<char>& tables)
 {
-    (void)hipFree(t->d);
-    (void)hipHostFree(t->h);
-    for (hipEvent_t e : t->copied)
-        if (e)
-            (void)hipEventDestroy(e);
-    delete t;
+    CH_REQUIRE(n_jobs >= 0 && (n_jobs == 0 || h_jobs), "%s: bad job list",
+               fn);
+    for (int32_t i = 0; i < n_jobs; ++i) {
+        const ch_conv_job& j = h_jobs[i];
+        CH_REQUIRE(j.plan && j.d_terma && j.d_dose,
+                   "%s: job %d has a null pointer", fn, i);
+        CH_REQUIRE(j.plan->d_khat,
+                   "%s: job %d's plan has no spectrum (call "
+                   "ch_plan_spectrum first)", fn, i);
+    }
+    std::vector<const ch_plan*> plans;
+    std::vector<std::vector<int32_t>> members;
+    for (int32_t i = 0; i < n_jobs; ++i) {
+        size_t g = 0;
+        while (g < plans.size() && plans[g] != h_jobs[i].plan)
+            ++g;
+        if (g == plans.size()) {
+            plans.push_back(h_jobs[i].plan);
+            members.emplace_back();
+        }
+        members[g].push_back(i);
+    }
+    out.clear();
+    tables.clear();
+    for (size_t g = 0; g < plans.size(); ++g) {
+        const ch_plan* p = plans[g];
+        const size_t per = beam_points(p) * sizeof(float2);
+        const size_t cap = std::max<size_t>(1, kSpectralWorkBytes / per);
+        const std::vector<int32_t>& m = members[g];
+        for (size_t b0 = 0; b0 < m.size(); b0 += cap) {
+            FftLaunch L;
+            L.plan = p;
+            L.nb = (int32_t)std::min(cap, m.size() - b0);
+            L.work = per * L.nb;
+            L.table = tables.size();
+            const size_t ptrs = (size_t)L.nb * sizeof(void*);
+            L.table_bytes = 2 * ptrs + (size_t)L.nb * sizeof(int32_t);
+            tables.resize(L.table + ((L.table_bytes + 255) & ~(size_t)255));
+            char* h = tables.data() + L.table;
+            const float** ht = (const float**)h;
+            float** hd = (float**)(h + ptrs);
+            int32_t* ha = (int32_t*)(h + 2 * ptrs);
+            for (int32_t b = 0; b < L.nb; ++b) {
+                const ch_conv_job& j = h_jobs[m[b0 + b]];
+                ht[b] = j.d_terma;
+                hd[b] = j.d_dose;
+                ha[b] = j.accumulate ? 1 : 0;
+            }
+            out.push_back(L);
+        }
+    }
+    return CH_OK;
 }
 
-ch_status beam_table(hipStream_t s, size_t bytes, TableSlot* out)
+// The beams of a launch read through its device table at d.
+static BeamIO table_io(void* d, int32_t nb)
 {
-    BeamTable* t = nullptr;
-    for (auto* x : g_tabs)
-        if (x->stream == s)
-            t = x;
-    if (!t) {
-        t = new BeamTable();
-        t->stream = s;
-        g_tabs.push_back(t);
-        for (hipEvent_t& e : t->copied)
-            CH_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    }
-    if (t->slot_bytes < bytes) {
-        // queued copies and kernels of this stream may use the old blocks
-        CH_HIP(hipStreamSynchronize(s));
-        CH_HIP(hipFree(t->d));
-        CH_HIP(hipHostFree(t->h));
-        t->d = nullptr;
-        t->h = nullptr;
-        t->slot_bytes = 0;
-        const size_t cap = (std::max(bytes, (size_t)4096) + 255) & ~(size_t)255;
-        CH_HIP(hipMalloc((void**)&t->d, cap * kTableSlots));
-        CH_HIP(hipHostMalloc((void**)&t->h, cap * kTableSlots,
-                             hipHostMallocDefault));
-        t->slot_bytes = cap;
-    }
-    const int k = t->next;
-    t->next = (k + 1) % kTableSlots;
-    CH_HIP(hipEventSynchronize(t->copied[k]));
-    out->h = t->h + (size_t)k * t->slot_bytes;
-    out->d = t->d + (size_t)k * t->slot_bytes;
-    out->copied = t->copied[k];
-    return CH_OK;
+    const size_t ptrs = (size_t)nb * sizeof(void*);
+    BeamIO io = single_beam(nullptr, nullptr, 0);
+    io.t_tab = (const float* const*)d;
+    io.d_tab = (float* const*)((char*)d + ptrs);
+    io.acc_tab = (const int32_t*)((char*)d + 2 * ptrs);
+    io.n_beams = nb;
+    return io;
 }
 
 }  // namespace
@@ -1333,18 +1341,19 @@ ch_status beam_table(hipStream_t s, size_t bytes, TableSlot* out)
 
 using namespace ch;
 
-void ch::release_spectral_tables(hipStream_t s)
+// A device-resident set of FFT-form jobs: the launches of
+// ch_convolve_fft_batch with their tables in device memory, written once,
+// so ch_convolve_fft_set queues kernels only (it can be captured).
+struct ch_convset : ch::GraphHeld {
+    std::vector<ch::FftLaunch> launches;
+    void* d_tables = nullptr;
+};
+
+static void convset_release(ch_convset* set)
 {
-    std::lock_guard<std::mutex> lock(g_tab_mu);
-    for (size_t i = 0; i < g_tabs.size(); ++i) {
-        BeamTable* t = g_tabs[i];
-        if (t->stream != s)
-            continue;
-        (void)hipStreamSynchronize(s);
-        table_free(t);
-        g_tabs.erase(g_tabs.begin() + i);
-        return;
-    }
+    (void)hipDeviceSynchronize();  // a queued launch may still read it
+    (void)hipFree(set->d_tables);
+    delete set;
 }
 
 void spectrum_free(ch_plan* p)
@@ -1557,75 +1566,94 @@ ch_status ch_convolve_fft(const ch_plan* plan, const float* d_terma,
 ch_status ch_convolve_fft_batch(int32_t n_jobs, const ch_conv_job* h_jobs,
                                 void* stream)
 {
-    CH_REQUIRE(n_jobs >= 0 && (n_jobs == 0 || h_jobs),
-               "ch_convolve_fft_batch: bad job list");
     hipStream_t s = (hipStream_t)stream;
     CH_REQUIRE(!capturing(s), "ch_convolve_fft_batch stages a host table and "
-                              "cannot be captured; capture ch_convolve_fft");
-    for (int32_t i = 0; i < n_jobs; ++i) {
-        const ch_conv_job& j = h_jobs[i];
-        CH_REQUIRE(j.plan && j.d_terma && j.d_dose,
-                   "ch_convolve_fft_batch: job %d has a null pointer", i);
-        CH_REQUIRE(j.plan->d_khat,
-                   "ch_convolve_fft_batch: job %d's plan has no spectrum "
-                   "(call ch_plan_spectrum first)", i);
-    }
-    // jobs on one plan share launches, in order of first appearance
-    std::vector<const ch_plan*> plans;
-    std::vector<std::vector<int32_t>> members;
-    for (int32_t i = 0; i < n_jobs; ++i) {
-        size_t g = 0;
-        while (g < plans.size() && plans[g] != h_jobs[i].plan)
-            ++g;
-        if (g == plans.size()) {
-            plans.push_back(h_jobs[i].plan);
-            members.emplace_back();
-        }
-        members[g].push_back(i);
-    }
-    std::lock_guard<std::mutex> lock(g_tab_mu);
+                              "cannot be captured; capture ch_convolve_fft "
+                              "or a ch_convset");
+    std::vector<FftLaunch> launches;
+    std::vector<char> tables;
+    ch_status st = fft_launches(n_jobs, h_jobs, "ch_convolve_fft_batch",
+                                launches, tables);
+    if (st != CH_OK)
+        return st;
     LaunchTimer timer(s, true);
-    for (size_t g = 0; g < plans.size(); ++g) {
-        const ch_plan* p = plans[g];
-        const size_t per = beam_points(p) * sizeof(float2);
-        const size_t cap = std::max<size_t>(1, kSpectralWorkBytes / per);
-        const std::vector<int32_t>& m = members[g];
-        for (size_t b0 = 0; b0 < m.size(); b0 += cap) {
-            const int32_t nb = (int32_t)std::min(cap, m.size() - b0);
-            void* work = nullptr;
-            ch_status st =
-                stream_scratch(s, kScratchSpectrum, per * nb, &work);
-            if (st != CH_OK)
-                return st;
-            TableSlot slot;
-            TableSlot* t = &slot;
-            const size_t ptrs = (size_t)nb * sizeof(void*);
-            st = beam_table(s, 2 * ptrs + (size_t)nb * sizeof(int32_t), t);
-            if (st != CH_OK)
-                return st;
-            const float** ht = (const float**)t->h;
-            float** hd = (float**)((char*)t->h + ptrs);
-            int32_t* ha = (int32_t*)((char*)t->h + 2 * ptrs);
-            for (int32_t b = 0; b < nb; ++b) {
-                const ch_conv_job& j = h_jobs[m[b0 + b]];
-                ht[b] = j.d_terma;
-                hd[b] = j.d_dose;
-                ha[b] = j.accumulate ? 1 : 0;
-            }
-            CH_HIP(hipMemcpyAsync(t->d, t->h, 2 * ptrs + nb * sizeof(int32_t),
-                                  hipMemcpyHostToDevice, s));
-            CH_HIP(hipEventRecord(t->copied, s));
-            BeamIO io = single_beam(nullptr, nullptr, 0);
-            io.t_tab = (const float* const*)t->d;
-            io.d_tab = (float* const*)((char*)t->d + ptrs);
-            io.acc_tab = (const int32_t*)((char*)t->d + 2 * ptrs);
-            io.n_beams = nb;
-            st = spectral_passes(p, io, (float2*)work, s, false);
-            if (st != CH_OK)
-                return st;
-        }
+    for (const FftLaunch& L : launches) {
+        void* work = nullptr;
+        st = stream_scratch(s, kScratchSpectrum, L.work, &work);
+        if (st != CH_OK)
+            return st;
+        void* d = nullptr;
+        st = stage_table(s, tables.data() + L.table, L.table_bytes, &d);
+        if (st != CH_OK)
+            return st;
+        st = spectral_passes(L.plan, table_io(d, L.nb), (float2*)work, s,
+                             false);
+        if (st != CH_OK)
+            return st;
     }
     timer.stop();
+    return CH_OK;
+}
+
+ch_status ch_convset_create(int32_t n_jobs, const ch_conv_job* h_jobs,
+                            ch_convset** out)
+{
+    CH_REQUIRE(out, "ch_convset_create: null output");
+    *out = nullptr;
+    CH_REQUIRE(n_jobs > 0, "ch_convset_create: no jobs");
+    ch_convset* set = new ch_convset();
+    set->release = [](GraphHeld* h) {
+        convset_release(static_cast<ch_convset*>(h));
+    };
+    std::vector<char> tables;
+    ch_status st = fft_launches(n_jobs, h_jobs, "ch_convset_create",
+                                set->launches, tables);
+    if (st != CH_OK) {
+        convset_release(set);
+        return st;
+    }
+    hipError_t e = hipMalloc(&set->d_tables, tables.size());
+    if (e == hipSuccess)
+        e = hipMemcpy(set->d_tables, tables.data(), tables.size(),
+                      hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        convset_release(set);
+        return hip_fail(e, "ch_convset_create", __FILE__, __LINE__);
+    }
+    *out = set;
+    return CH_OK;
+}
+
+ch_status ch_convolve_fft_set(const ch_convset* cset, void* stream)
+{
+    CH_REQUIRE(cset, "ch_convolve_fft_set: null set");
+    ch_convset* set = const_cast<ch_convset*>(cset);
+    hipStream_t s = (hipStream_t)stream;
+    capture_hold(s, set);
+    LaunchTimer timer(s, true);
+    for (const FftLaunch& L : set->launches) {
+        void* work = nullptr;
+        ch_status st = stream_scratch(s, kScratchSpectrum, L.work, &work);
+        if (st != CH_OK)
+            return st;
+        capture_pin(s, L.plan);
+        st = spectral_passes(L.plan,
+                             table_io((char*)set->d_tables + L.table, L.nb),
+                             (float2*)work, s, false);
+        if (st != CH_OK)
+            return st;
+    }
+    timer.stop();
+    return CH_OK;
+}
+
+ch_status ch_convset_destroy(ch_convset* set)
+{
+    if (!set)
+        return CH_OK;
+    if (defer_free(set))  // read by a live graph: freed with the last one
+        return CH_OK;
+    convset_release(set);
     return CH_OK;
 }
 

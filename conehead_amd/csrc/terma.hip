@@ -42,7 +42,9 @@ struct TermaParams {
     const float* raster;
     const float* density;
     float* terma;
-    int32_t* box;
+    int32_t* box;              // support-box partials of this beam or null
+    int32_t* counts;           // this beam's cost-class counters (phase 1)
+    int32_t* list;             // and brick lists
     float* debug;              // 4 planes of X*Y*Z or null
     int32_t* status;           // geometry error flag
     int32_t accumulate;        // terma += this beam
@@ -479,14 +481,9 @@ __host__ __device__ __forceinline__ int shard_cap(int n_bricks)
 // that reads 0 everywhere gets T = 0 here; every other brick (all of them
 // when the debug planes are asked for) is appended to its cost class for
 // phase 2, with its stage-1 result.
-__global__ __launch_bounds__(256) void terma_sort_kernel(
-    TermaParams P, int32_t* __restrict__ list, int32_t* __restrict__ counts,
-    int32_t n_bricks)
+__device__ __forceinline__ void sort_brick(const TermaParams& P, int b,
+                                           int l, int32_t n_bricks)
 {
-    const int b = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (b >= n_bricks)
-        return;  // wavefront-uniform
-    const int l = threadIdx.x & 63;
     const Brick k = brick_of(P, b);
     const float v = brick_value(P, k, l);
     if (v == 0.0f && P.debug == nullptr) {
@@ -499,11 +496,35 @@ __global__ __launch_bounds__(256) void terma_sort_kernel(
     if (l == 0) {
         const int q = brick_cost_class(P, k, v < 0.0f);
         const int c = q * kShards + b % kShards;
-        const int at = atomicAdd(&counts[c], 1);
+        const int at = atomicAdd(&P.counts[c], 1);
         // entry: brick index and stage-1 value (0 -> 0, 1 -> 1, -1 -> 2)
-        list[(int64_t)c * shard_cap(n_bricks) + at] =
+        P.list[(int64_t)c * shard_cap(n_bricks) + at] =
             b * 4 + (v > 0.0f ? 1 : (v < 0.0f ? 2 : 0));
     }
+}
+
+__global__ __launch_bounds__(256) void terma_sort_kernel(TermaParams P,
+                                                         int32_t n_bricks)
+{
+    const int b = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (b >= n_bricks)
+        return;  // wavefront-uniform
+    sort_brick(P, b, threadIdx.x & 63, n_bricks);
+}
+
+// Phase 1 of several beams on one grid shape in one launch: the blocks of
+// beam i are i * blocks_per_beam ... , its parameters tab[i] (a device
+// table written before the launch and not written by it).
+__global__ __launch_bounds__(256) void terma_sort_set_kernel(
+    const TermaParams* __restrict__ tab, int32_t n_bricks,
+    int32_t blocks_per_beam)
+{
+    const int beam = blockIdx.x / blocks_per_beam;
+    const int b = (blockIdx.x - beam * blocks_per_beam) * 4 +
+                  (threadIdx.x >> 6);
+    if (b >= n_bricks)
+        return;  // wavefront-uniform
+    sort_brick(tab[beam], b, threadIdx.x & 63, n_bricks);
 }
 
 // TERMA of one brick (one wavefront, every lane calls it) whose stage-1 hit
@@ -694,15 +715,13 @@ __device__ __forceinline__ float terma_brick(const TermaParams& P,
 // descending cost class, so the longest depth walks are dispatched first
 // and the rest fill in behind them (wavefronts past the last brick exit at
 // once).  Each wavefront writes its support-box partial.
-__global__ __launch_bounds__(256) void terma_kernel(
-    TermaParams P, const int32_t* __restrict__ list,
-    const int32_t* __restrict__ counts, int32_t n_bricks)
+__device__ __forceinline__ void terma_wave(const TermaParams& P, int g,
+                                           int l, double* __restrict__ s_samp,
+                                           int32_t* __restrict__ s_lane,
+                                           int32_t n_bricks)
 {
-    __shared__ double s_samp[4][kSampleBuf];
-    __shared__ int32_t s_lane[4][64];
-    const int l = threadIdx.x & 63;
-    const int w = threadIdx.x >> 6;
-    const int g = blockIdx.x * 4 + w;
+    const int32_t* __restrict__ counts = P.counts;
+    const int32_t* __restrict__ list = P.list;
     // the counters in order of descending cost class, shards in order:
     // position j = (kCostClasses - 1 - q) * kShards + shard, lane l holding
     // j = 4 l .. 4 l + 3; incl = inclusive prefix of the lanes' sums
@@ -745,7 +764,7 @@ __global__ __launch_bounds__(256) void terma_kernel(
         const Brick k = brick_of(P, e >> 2);
         const float v = (e & 3) == 1 ? 1.0f : ((e & 3) == 2 ? -1.0f : 0.0f);
         bool bad = false;
-        T = terma_brick(P, k, v, l, s_samp[w], s_lane[w], bad);
+        T = terma_brick(P, k, v, l, s_samp, s_lane, bad);
         if (__ballot(bad) != 0 && l == 0)
             atomicOr(P.status, 1);
         x = k.x0 + (l >> 4);
@@ -777,6 +796,34 @@ __global__ __launch_bounds__(256) void terma_kernel(
     }
 }
 
+__global__ __launch_bounds__(256) void terma_kernel(TermaParams P,
+                                                    int32_t n_bricks)
+{
+    __shared__ double s_samp[4][kSampleBuf];
+    __shared__ int32_t s_lane[4][64];
+    const int w = threadIdx.x >> 6;
+    terma_wave(P, blockIdx.x * 4 + w, threadIdx.x & 63, s_samp[w], s_lane[w],
+               n_bricks);
+}
+
+// Phase 2 of several beams in one launch, beam by beam (the blocks of beam
+// i are i * blocks_per_beam ...): the wavefronts in flight then serve one
+// or two beams, whose rays toward one source cross the same density lines
+// and read one aperture table, so they share the caches; the next beam's
+// longest walks start while the last of the previous beam finish.
+__global__ __launch_bounds__(256) void terma_set_kernel(
+    const TermaParams* __restrict__ tab, int32_t n_bricks,
+    int32_t blocks_per_beam)
+{
+    __shared__ double s_samp[4][kSampleBuf];
+    __shared__ int32_t s_lane[4][64];
+    const int w = threadIdx.x >> 6;
+    const int beam = blockIdx.x / blocks_per_beam;
+    const int g = (blockIdx.x - beam * blocks_per_beam) * 4 + w;
+    terma_wave(tab[beam], g, threadIdx.x & 63, s_samp[w], s_lane[w],
+               n_bricks);
+}
+
 __global__ __launch_bounds__(64) void terma_prep_kernel(int32_t* status)
 {
     if (threadIdx.x == 0)
@@ -785,10 +832,11 @@ __global__ __launch_bounds__(64) void terma_prep_kernel(int32_t* status)
 
 // Zeroes the cost-class counters before phase 1 (a kernel, so the call can
 // be captured in a graph).
-__global__ __launch_bounds__(64) void terma_clear_kernel(int32_t* counts,
-                                                         int n)
+__global__ __launch_bounds__(256) void terma_clear_kernel(int32_t* counts,
+                                                          int n)
 {
-    for (int i = threadIdx.x; i < n; i += 64)
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n)
         counts[i] = 0;
 }
 
@@ -844,27 +892,78 @@ static TermaParams make_params(const ch_beam& B, const int32_t* h_shape,
     return P;
 }
 
-// Device memory one beam's TERMA launches need beyond its inputs and
-// outputs: the cost-class counters and brick lists of phase 1 and, with a
-// support box, one partial box per phase-2 wavefront.
-struct TermaScratch {
-    int32_t* counts;
-    int32_t* list;
-    int32_t* box_parts;  // null without a support box
-};
-
+// Device memory TERMA launches need beyond their inputs and outputs, per
+// beam of one launch: kCounters cost-class counters and the brick lists of
+// phase 1 and, with a support box, one partial box per phase-2 wavefront.
+// The counters of all the launch's beams come first, so one clear kernel
+// zeroes them.
 inline int32_t terma_bricks(const int32_t* shape)
 {
     return ((shape[0] + 3) / 4) * ((shape[1] + 3) / 4) *
            ((shape[2] + 3) / 4);
 }
-inline size_t terma_list_bytes(int32_t n_bricks)
+inline size_t terma_list_ints(int32_t n_bricks)
 {
-    return (size_t)kCounters * (shard_cap(n_bricks) + 1) * sizeof(int32_t);
+    return (size_t)kCounters * shard_cap(n_bricks);
 }
-inline size_t terma_box_bytes(int32_t n_bricks)
+inline size_t terma_list_bytes(int32_t n_beams, int32_t n_bricks)
 {
-    return (size_t)((n_bricks + 3) / 4) * 4 * kBoxStride * sizeof(int32_t);
+    return (size_t)n_beams * (kCounters + terma_list_ints(n_bricks)) *
+           sizeof(int32_t);
+}
+inline size_t terma_box_ints(int32_t n_bricks)
+{
+    return (size_t)((n_bricks + 3) / 4) * 4 * kBoxStride;
+}
+inline size_t terma_box_bytes(int32_t n_beams, int32_t n_bricks)
+{
+    return (size_t)n_beams * terma_box_ints(n_bricks) * sizeof(int32_t);
+}
+
+// Points beam i of a launch of n beams at its part of the scratch (lists:
+// counters then lists; boxes: partial boxes, null without any support box).
+static void terma_assign(TermaParams& P, const ch_beam& B, int i, int n,
+                         int32_t n_bricks, int32_t* lists, int32_t* boxes)
+{
+    P.counts = lists + (size_t)i * kCounters;
+    P.list = lists + (size_t)n * kCounters +
+             (size_t)i * terma_list_ints(n_bricks);
+    P.box = B.d_box && boxes ? boxes + (size_t)i * terma_box_ints(n_bricks)
+                             : nullptr;
+}
+
+// Beams that share one launch: runs of consecutive beams with distinct
+// TERMA grids (a beam that adds into the grid of an earlier beam of the
+// run starts the next launch, so the two stay ordered), at most kSetBeams
+// beams and kSetScratchBytes of scratch per launch.  Returns the start of
+// every launch followed by n.
+constexpr int kSetBeams = 64;
+constexpr size_t kSetScratchBytes = (size_t)512 << 20;
+static std::vector<int32_t> terma_groups(int32_t n, const ch_beam* beams,
+                                         int32_t n_bricks)
+{
+    const size_t per = terma_list_bytes(1, n_bricks) +
+                       terma_box_bytes(1, n_bricks);
+    const int32_t cap = (int32_t)std::max<size_t>(
+        1, std::min<size_t>(kSetBeams, kSetScratchBytes / per));
+    std::vector<int32_t> starts{0};
+    for (int32_t b = 1; b < n; ++b) {
+        bool split = b - starts.back() >= cap;
+        for (int32_t a = starts.back(); a < b && !split; ++a)
+            split = beams[a].d_terma == beams[b].d_terma;
+        if (split)
+            starts.push_back(b);
+    }
+    starts.push_back(n);
+    return starts;
+}
+
+static int32_t largest_group(const std::vector<int32_t>& starts)
+{
+    int32_t m = 0;
+    for (size_t g = 0; g + 1 < starts.size(); ++g)
+        m = std::max(m, starts[g + 1] - starts[g]);
+    return m;
 }
 
 // Checks beam b and fills its kernel parameters.
@@ -901,28 +1000,53 @@ static ch_status terma_params(const ch_beam& B, int b, const int32_t* shape,
     return CH_OK;
 }
 
-// One beam's launches: clear the counters, phase 1, phase 2 and the
-// support box.  ev0 / ev1 (nullable) are attached to the start of phase 1
-// and the end of phase 2.
-static ch_status terma_launch(TermaParams P, const ch_beam& B,
-                              int32_t n_bricks, const TermaScratch& sc,
-                              hipEvent_t ev0, hipEvent_t ev1, hipStream_t s)
+// The launches of n beams on one grid: clear their counters, phase 1,
+// phase 2 and the support boxes.  h holds the beams' parameters with their
+// scratch assigned; with d_tab (a device copy of h) the beams share each
+// launch, without it (n == 1) the parameters go by value.  ev0 / ev1
+// (nullable) are attached to the start of phase 1 and the end of phase 2.
+static ch_status terma_launch(const TermaParams* h, const TermaParams* d_tab,
+                              int32_t n, const ch_beam* beams,
+                              int32_t n_bricks, hipEvent_t ev0,
+                              hipEvent_t ev1, hipStream_t s)
 {
-    const unsigned blocks = (unsigned)((n_bricks + 3) / 4);
-    P.box = B.d_box ? sc.box_parts : nullptr;
-    hipLaunchKernelGGL(terma_clear_kernel, dim3(1), dim3(64), 0, s,
-                       sc.counts, kCounters);
-    hipExtLaunchKernelGGL(terma_sort_kernel, dim3(blocks), dim3(256), 0, s,
-                          ev0, nullptr, 0, P, sc.list, sc.counts, n_bricks);
-    hipExtLaunchKernelGGL(terma_kernel, dim3(blocks), dim3(256), 0, s,
-                          nullptr, ev1, 0, P, (const int32_t*)sc.list,
-                          (const int32_t*)sc.counts, n_bricks);
-    CH_HIP(hipGetLastError());
-    if (B.d_box) {
-        launch_box_finalize(P.box, (int)blocks * 4, B.d_box,
-                            B.accumulate ? 1 : 0, s);
-        CH_HIP(hipGetLastError());
+    const unsigned bpb = (unsigned)((n_bricks + 3) / 4);
+    const int nc = n * kCounters;
+    hipLaunchKernelGGL(terma_clear_kernel, dim3((unsigned)(nc + 255) / 256),
+                       dim3(256), 0, s, h[0].counts, nc);
+    if (!d_tab) {
+        hipExtLaunchKernelGGL(terma_sort_kernel, dim3(bpb), dim3(256), 0, s,
+                              ev0, nullptr, 0, h[0], n_bricks);
+        hipExtLaunchKernelGGL(terma_kernel, dim3(bpb), dim3(256), 0, s,
+                              nullptr, ev1, 0, h[0], n_bricks);
+    } else {
+        hipExtLaunchKernelGGL(terma_sort_set_kernel, dim3(bpb * n),
+                              dim3(256), 0, s, ev0, nullptr, 0, d_tab,
+                              n_bricks, (int32_t)bpb);
+        hipExtLaunchKernelGGL(terma_set_kernel, dim3(bpb * n), dim3(256), 0,
+                              s, nullptr, ev1, 0, d_tab, n_bricks,
+                              (int32_t)bpb);
     }
+    CH_HIP(hipGetLastError());
+    for (int32_t i = 0; i < n; ++i)
+        if (beams[i].d_box) {
+            launch_box_finalize(h[i].box, (int)bpb * 4, beams[i].d_box,
+                                beams[i].accumulate ? 1 : 0, s);
+            CH_HIP(hipGetLastError());
+        }
+    return CH_OK;
+}
+
+// Checks the shared grid arguments of a TERMA call.
+static ch_status terma_grid_ok(const char* fn, const int32_t* shape,
+                               const double* origin, const double* spacing)
+{
+    CH_REQUIRE(shape && origin && spacing, "%s: null pointer", fn);
+    CH_REQUIRE(shape[0] > 0 && shape[1] > 0 && shape[2] > 0,
+               "%s: empty grid", fn);
+    const int64_t nv = (int64_t)shape[0] * shape[1] * shape[2];
+    CH_REQUIRE(nv < (1LL << 30),
+               "%s: grid too large (4 X Y Z must stay below 2^32)", fn);
     return CH_OK;
 }
 
@@ -960,6 +1084,79 @@ ch_status terma_init_device()
 
 }  // namespace ch
 
+// A device-resident set of beams on one grid: their kernel parameters in a
+// device table, written when the set is created or updated, and the scratch
+// of their launches, so ch_terma_beamset queues kernels only (it can be
+// captured in a graph and replayed).
+struct ch_beamset : ch::GraphHeld {
+    int32_t n = 0, n_bricks = 0;
+    int32_t shape[3] = {0, 0, 0};
+    double origin[3] = {0, 0, 0}, spacing[3] = {0, 0, 0};
+    std::vector<ch::TermaParams> h;   // host copy of the table
+    std::vector<ch_beam> beams;       // d_terma, d_box, accumulate per beam
+    std::vector<int32_t> starts;      // launch groups (terma_groups)
+    ch::TermaParams* d_tab = nullptr;
+    int32_t* lists = nullptr;         // counters and lists, largest group
+    int32_t* boxes = nullptr;         // partial boxes (null without boxes)
+    int32_t* flags = nullptr;         // geometry flag per beam (own d_status
+                                      // overrides)
+    ch::TermaParams* h_stage = nullptr;  // pinned staging of updates
+    hipEvent_t staged = nullptr;         // the last update's copy
+};
+
+namespace ch {
+
+static void beamset_release(ch_beamset* set)
+{
+    (void)hipDeviceSynchronize();  // a queued launch may still read it
+    (void)hipFree(set->d_tab);
+    (void)hipFree(set->lists);
+    (void)hipFree(set->boxes);
+    (void)hipFree(set->flags);
+    (void)hipHostFree(set->h_stage);
+    if (set->staged)
+        (void)hipEventDestroy(set->staged);
+    delete set;
+}
+
+// Parameters of every beam of a set with their scratch assigned; the
+// launch groups must be `want` when it is given (an update keeps the
+// launches a captured graph has recorded).
+static ch_status beamset_params(ch_beamset* set, int32_t n,
+                                const ch_beam* beams,
+                                const std::vector<int32_t>* want,
+                                std::vector<TermaParams>& hp,
+                                std::vector<int32_t>& starts)
+{
+    hp.resize(n);
+    for (int32_t b = 0; b < n; ++b) {
+        ch_status st = terma_params(beams[b], b, set->shape, set->origin,
+                                    set->spacing, nullptr, set->flags + b,
+                                    &hp[b]);
+        if (st != CH_OK)
+            return st;
+    }
+    starts = terma_groups(n, beams, set->n_bricks);
+    if (want) {
+        CH_REQUIRE(starts == *want,
+                   "ch_beamset_update: the beams share their TERMA grids "
+                   "differently from the set's (create a new set)");
+        for (int32_t b = 0; b < n; ++b)
+            CH_REQUIRE((beams[b].d_box != nullptr) ==
+                           (set->beams[b].d_box != nullptr),
+                       "ch_beamset_update: beam %d adds or drops its "
+                       "support box (create a new set)", b);
+    }
+    for (size_t g = 0; g + 1 < starts.size(); ++g)
+        for (int32_t b = starts[g]; b < starts[g + 1]; ++b)
+            terma_assign(hp[b], beams[b], b - starts[g],
+                         starts[g + 1] - starts[g], set->n_bricks,
+                         set->lists, set->boxes);
+    return CH_OK;
+}
+
+}  // namespace ch
+
 using namespace ch;
 
 extern "C" {
@@ -971,55 +1168,79 @@ ch_status ch_terma(int32_t n_beams, const ch_beam* h_beams,
     CH_REQUIRE(n_beams >= 0, "n_beams < 0");
     if (n_beams == 0)
         return CH_OK;
-    CH_REQUIRE(h_beams && h_shape && h_origin && h_spacing,
-               "ch_terma: null pointer");
-    CH_REQUIRE(h_shape[0] > 0 && h_shape[1] > 0 && h_shape[2] > 0,
-               "ch_terma: empty grid");
-    const int64_t nv = (int64_t)h_shape[0] * h_shape[1] * h_shape[2];
-    CH_REQUIRE(nv < (1LL << 30), "ch_terma: grid too large (4 X Y Z must "
-                                 "stay below 2^32)");
+    CH_REQUIRE(h_beams, "ch_terma: null pointer");
+    ch_status st = terma_grid_ok("ch_terma", h_shape, h_origin, h_spacing);
+    if (st != CH_OK)
+        return st;
     hipStream_t s = (hipStream_t)stream;
     // The stream's geometry flag stays set until ch_terma_check reads it, so
     // one check covers every call queued since the last one; it is zeroed
     // when it is allocated.  A beam with its own d_status uses that instead.
     void* status = nullptr;
-    ch_status st = stream_scratch(s, kScratchStatus, 256, &status, true);
+    st = stream_scratch(s, kScratchStatus, 256, &status, true);
     if (st != CH_OK)
         return st;
-    const int32_t n_bricks = terma_bricks(h_shape);
-    TermaScratch sc;
-    void* lists = nullptr;
-    st = stream_scratch(s, kScratchTermaList, terma_list_bytes(n_bricks),
-                        &lists);
-    if (st != CH_OK)
-        return st;
-    sc.counts = (int32_t*)lists;
-    sc.list = sc.counts + kCounters;
-    sc.box_parts = nullptr;
-    for (int32_t b = 0; b < n_beams; ++b)
-        if (h_beams[b].d_box) {
-            void* parts = nullptr;
-            st = stream_scratch(s, kScratchBoxPartials,
-                                terma_box_bytes(n_bricks), &parts);
-            if (st != CH_OK)
-                return st;
-            sc.box_parts = (int32_t*)parts;
-            break;
-        }
-    // the caller's armed event pair (ch_set_launch_events) brackets the
-    // first beam's phase 1 and the last beam's phase 2
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    if (!capturing(s))
-        (void)take_launch_events(&ev0, &ev1);
+    std::vector<TermaParams> hp(n_beams);
+    bool any_box = false;
     for (int32_t b = 0; b < n_beams; ++b) {
-        TermaParams P;
         st = terma_params(h_beams[b], b, h_shape, h_origin, h_spacing,
-                          d_debug, (int32_t*)status, &P);
+                          d_debug, (int32_t*)status, &hp[b]);
         if (st != CH_OK)
             return st;
-        st = terma_launch(P, h_beams[b], n_bricks, sc,
-                          b == 0 ? ev0 : nullptr,
-                          b == n_beams - 1 ? ev1 : nullptr, s);
+        any_box = any_box || h_beams[b].d_box;
+    }
+    // Runs of beams share each launch (their parameters staged into a
+    // device table); under capture, where a staged table would be
+    // recorded, every beam takes its own launches.
+    const bool cap = capturing(s);
+    const int32_t n_bricks = terma_bricks(h_shape);
+    const std::vector<int32_t> starts =
+        terma_groups(n_beams, h_beams, n_bricks);
+    const int32_t width = cap ? 1 : largest_group(starts);
+    void* lists = nullptr;
+    st = stream_scratch(s, kScratchTermaList,
+                        terma_list_bytes(width, n_bricks), &lists);
+    if (st != CH_OK)
+        return st;
+    void* boxes = nullptr;
+    if (any_box) {
+        st = stream_scratch(s, kScratchBoxPartials,
+                            terma_box_bytes(width, n_bricks), &boxes);
+        if (st != CH_OK)
+            return st;
+    }
+    // the caller's armed event pair (ch_set_launch_events) brackets the
+    // first phase 1 and the last phase 2
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    if (!cap)
+        (void)take_launch_events(&ev0, &ev1);
+    for (size_t g = 0; g + 1 < starts.size(); ++g) {
+        const int32_t g0 = starts[g], ng = starts[g + 1] - g0;
+        const bool last_group = g + 2 == starts.size();
+        if (ng == 1 || cap) {
+            for (int32_t i = 0; i < ng; ++i) {
+                const int32_t b = g0 + i;
+                terma_assign(hp[b], h_beams[b], 0, 1, n_bricks,
+                             (int32_t*)lists, (int32_t*)boxes);
+                st = terma_launch(&hp[b], nullptr, 1, &h_beams[b], n_bricks,
+                                  b == 0 ? ev0 : nullptr,
+                                  b == n_beams - 1 ? ev1 : nullptr, s);
+                if (st != CH_OK)
+                    return st;
+            }
+            continue;
+        }
+        for (int32_t i = 0; i < ng; ++i)
+            terma_assign(hp[g0 + i], h_beams[g0 + i], i, ng, n_bricks,
+                         (int32_t*)lists, (int32_t*)boxes);
+        void* d_tab = nullptr;
+        st = stage_table(s, &hp[g0], (size_t)ng * sizeof(TermaParams),
+                         &d_tab);
+        if (st != CH_OK)
+            return st;
+        st = terma_launch(&hp[g0], (const TermaParams*)d_tab, ng,
+                          &h_beams[g0], n_bricks, g == 0 ? ev0 : nullptr,
+                          last_group ? ev1 : nullptr, s);
         if (st != CH_OK)
             return st;
     }
@@ -1035,16 +1256,14 @@ ch_status ch_terma_noalloc(int32_t n_beams, const ch_beam* h_beams,
     CH_REQUIRE(n_beams >= 0, "n_beams < 0");
     if (n_beams == 0)
         return CH_OK;
-    CH_REQUIRE(h_beams && h_shape && h_origin && h_spacing,
-               "ch_terma_noalloc: null pointer");
-    CH_REQUIRE(h_shape[0] > 0 && h_shape[1] > 0 && h_shape[2] > 0,
-               "ch_terma_noalloc: empty grid");
-    const int64_t nv = (int64_t)h_shape[0] * h_shape[1] * h_shape[2];
-    CH_REQUIRE(nv < (1LL << 30), "ch_terma_noalloc: grid too large (4 X Y "
-                                 "Z must stay below 2^32)");
+    CH_REQUIRE(h_beams, "ch_terma_noalloc: null pointer");
+    ch_status st =
+        terma_grid_ok("ch_terma_noalloc", h_shape, h_origin, h_spacing);
+    if (st != CH_OK)
+        return st;
     const int32_t n_bricks = terma_bricks(h_shape);
-    const size_t need = terma_list_bytes(n_bricks) +
-                        terma_box_bytes(n_bricks);
+    const size_t need =
+        terma_list_bytes(1, n_bricks) + terma_box_bytes(1, n_bricks);
     CH_REQUIRE(need <= kNoAllocScratchBytes,
                "ch_terma_noalloc: a %d x %d x %d grid needs %zu bytes of "
                "scratch, more than its %zu (use ch_terma)",
@@ -1055,24 +1274,173 @@ ch_status ch_terma_noalloc(int32_t n_beams, const ch_beam* h_beams,
                    "ch_terma_noalloc: beam %d has no d_status (the geometry "
                    "flag is the caller's output here)", b);
     void* base = nullptr;
-    ch_status st = noalloc_scratch(&base);
+    st = noalloc_scratch(&base);
     if (st != CH_OK)
         return st;
-    TermaScratch sc;
-    sc.counts = (int32_t*)base;
-    sc.list = sc.counts + kCounters;
-    sc.box_parts = (int32_t*)((char*)base + terma_list_bytes(n_bricks));
+    int32_t* lists = (int32_t*)base;
+    int32_t* boxes =
+        (int32_t*)((char*)base + terma_list_bytes(1, n_bricks));
     hipStream_t s = (hipStream_t)stream;
+    // one beam at a time through the one scratch array (no device table)
     for (int32_t b = 0; b < n_beams; ++b) {
         TermaParams P;
         st = terma_params(h_beams[b], b, h_shape, h_origin, h_spacing,
                           d_debug, nullptr, &P);
         if (st != CH_OK)
             return st;
-        st = terma_launch(P, h_beams[b], n_bricks, sc, nullptr, nullptr, s);
+        terma_assign(P, h_beams[b], 0, 1, n_bricks, lists, boxes);
+        st = terma_launch(&P, nullptr, 1, &h_beams[b], n_bricks, nullptr,
+                          nullptr, s);
         if (st != CH_OK)
             return st;
     }
+    return CH_OK;
+}
+
+ch_status ch_beamset_create(int32_t n_beams, const ch_beam* h_beams,
+                            const int32_t h_shape[3],
+                            const double h_origin[3],
+                            const double h_spacing[3], ch_beamset** out)
+{
+    CH_REQUIRE(out, "ch_beamset_create: null output");
+    *out = nullptr;
+    CH_REQUIRE(n_beams > 0 && h_beams, "ch_beamset_create: no beams");
+    ch_status st =
+        terma_grid_ok("ch_beamset_create", h_shape, h_origin, h_spacing);
+    if (st != CH_OK)
+        return st;
+    ch_beamset* set = new ch_beamset();
+    set->release = [](GraphHeld* h) {
+        beamset_release(static_cast<ch_beamset*>(h));
+    };
+    set->n = n_beams;
+    set->n_bricks = terma_bricks(h_shape);
+    for (int a = 0; a < 3; ++a) {
+        set->shape[a] = h_shape[a];
+        set->origin[a] = h_origin[a];
+        set->spacing[a] = h_spacing[a];
+    }
+    set->beams.assign(h_beams, h_beams + n_beams);
+    bool any_box = false;
+    for (int32_t b = 0; b < n_beams; ++b)
+        any_box = any_box || h_beams[b].d_box;
+    const std::vector<int32_t> starts =
+        terma_groups(n_beams, h_beams, set->n_bricks);
+    const int32_t width = largest_group(starts);
+    hipError_t e = hipMalloc((void**)&set->d_tab,
+                             (size_t)n_beams * sizeof(TermaParams));
+    if (e == hipSuccess)
+        e = hipMalloc((void**)&set->lists,
+                      terma_list_bytes(width, set->n_bricks));
+    if (e == hipSuccess && any_box)
+        e = hipMalloc((void**)&set->boxes,
+                      terma_box_bytes(width, set->n_bricks));
+    if (e == hipSuccess)
+        e = hipMalloc((void**)&set->flags, (size_t)n_beams * sizeof(int32_t));
+    if (e == hipSuccess)
+        e = hipMemset(set->flags, 0, (size_t)n_beams * sizeof(int32_t));
+    if (e == hipSuccess)
+        e = hipHostMalloc((void**)&set->h_stage,
+                          (size_t)n_beams * sizeof(TermaParams),
+                          hipHostMallocDefault);
+    if (e == hipSuccess)
+        e = hipEventCreateWithFlags(&set->staged, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        beamset_release(set);
+        return hip_fail(e, "ch_beamset_create", __FILE__, __LINE__);
+    }
+    std::vector<int32_t> got;
+    st = beamset_params(set, n_beams, h_beams, nullptr, set->h, got);
+    if (st != CH_OK) {
+        beamset_release(set);
+        return st;
+    }
+    set->starts = got;
+    e = hipMemcpy(set->d_tab, set->h.data(),
+                  (size_t)n_beams * sizeof(TermaParams),
+                  hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        beamset_release(set);
+        return hip_fail(e, "ch_beamset_create", __FILE__, __LINE__);
+    }
+    *out = set;
+    return CH_OK;
+}
+
+ch_status ch_beamset_update(ch_beamset* set, int32_t n_beams,
+                            const ch_beam* h_beams, void* stream)
+{
+    CH_REQUIRE(set && h_beams, "ch_beamset_update: null pointer");
+    CH_REQUIRE(n_beams == set->n,
+               "ch_beamset_update: %d beams for a set of %d", n_beams,
+               set->n);
+    hipStream_t s = (hipStream_t)stream;
+    CH_REQUIRE(!capturing(s), "ch_beamset_update copies from the host: "
+                              "call it outside the capture");
+    std::vector<TermaParams> hp;
+    std::vector<int32_t> starts;
+    ch_status st = beamset_params(set, n_beams, h_beams, &set->starts, hp,
+                                  starts);
+    if (st != CH_OK)
+        return st;
+    // the staging buffer is rewritten once the previous copy has read it;
+    // launches queued on `stream` before this call read the old table
+    CH_HIP(hipEventSynchronize(set->staged));
+    memcpy(set->h_stage, hp.data(), (size_t)n_beams * sizeof(TermaParams));
+    CH_HIP(hipMemcpyAsync(set->d_tab, set->h_stage,
+                          (size_t)n_beams * sizeof(TermaParams),
+                          hipMemcpyHostToDevice, s));
+    CH_HIP(hipEventRecord(set->staged, s));
+    set->h.swap(hp);
+    set->beams.assign(h_beams, h_beams + n_beams);
+    return CH_OK;
+}
+
+ch_status ch_terma_beamset(const ch_beamset* cset, void* stream)
+{
+    CH_REQUIRE(cset, "ch_terma_beamset: null set");
+    ch_beamset* set = const_cast<ch_beamset*>(cset);
+    hipStream_t s = (hipStream_t)stream;
+    capture_hold(s, set);
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    if (!capturing(s))
+        (void)take_launch_events(&ev0, &ev1);
+    const std::vector<int32_t>& starts = set->starts;
+    for (size_t g = 0; g + 1 < starts.size(); ++g) {
+        const int32_t g0 = starts[g], ng = starts[g + 1] - g0;
+        ch_status st = terma_launch(&set->h[g0], set->d_tab + g0, ng,
+                                    &set->beams[g0], set->n_bricks,
+                                    g == 0 ? ev0 : nullptr,
+                                    g + 2 == starts.size() ? ev1 : nullptr,
+                                    s);
+        if (st != CH_OK)
+            return st;
+    }
+    return CH_OK;
+}
+
+ch_status ch_beamset_flags(ch_beamset* set, int32_t* h_flags, void* stream)
+{
+    CH_REQUIRE(set && h_flags, "ch_beamset_flags: null pointer");
+    hipStream_t s = (hipStream_t)stream;
+    CH_REQUIRE(!capturing(s), "ch_beamset_flags waits for the stream");
+    CH_HIP(hipMemcpyAsync(h_flags, set->flags, (size_t)set->n * 4,
+                          hipMemcpyDeviceToHost, s));
+    hipLaunchKernelGGL(terma_clear_kernel,
+                       dim3((unsigned)(set->n + 255) / 256), dim3(256), 0, s,
+                       set->flags, set->n);
+    CH_HIP(hipGetLastError());
+    CH_HIP(hipStreamSynchronize(s));
+    return CH_OK;
+}
+
+ch_status ch_beamset_destroy(ch_beamset* set)
+{
+    if (!set)
+        return CH_OK;
+    if (defer_free(set))  // read by a live graph: freed with the last one
+        return CH_OK;
+    beamset_release(set);
     return CH_OK;
 }
 

@@ -317,6 +317,10 @@ def beam_struct(source, settings, aperture, d_density, d_terma, d_box):
     b.d_density = d_density.ptr
     b.d_terma = d_terma.ptr
     b.d_box = d_box.ptr if d_box is not None else None
+    # the struct holds raw device pointers: it also holds their owners, so
+    # an aperture dropped from the engine's cache stays allocated while a
+    # beam still points into it
+    b.owners = (aperture, d_density, d_terma, d_box)
     return b
 
 
@@ -348,6 +352,101 @@ def terma(beams, shape, origin, spacing, debug=None, stream=None,
                           _h(stream)))
     if check:
         terma_check(stream)
+
+
+class BeamSet:
+    """Device-resident TERMA beams on one grid (ch_beamset): the beams'
+    kernel parameters live in a device table, so run() queues kernels only
+    and can be captured in a Graph; update() rewrites the table for new
+    beams with the same outputs.  `keep` holds the Python objects the beams
+    point into (apertures, buffers) for as long as the set uses them."""
+
+    def __init__(self, beams, shape, origin, spacing, keep=()):
+        n = len(beams)
+        if n == 0:
+            raise ValueError("a beam set needs beams")
+        self.n = n
+        arr = (_lib.Beam * n)(*beams)
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().ch_beamset_create(
+            n, arr, (ctypes.c_int32 * 3)(*[int(v) for v in shape]),
+            (ctypes.c_double * 3)(*[float(v) for v in origin]),
+            (ctypes.c_double * 3)(*[float(v) for v in spacing]),
+            ctypes.byref(h)))
+        self.handle = h.value
+        self.keep = list(keep)
+
+    def update(self, beams, stream=None, keep=()):
+        if len(beams) != self.n:
+            raise ValueError("%d beams for a set of %d" % (len(beams), self.n))
+        arr = (_lib.Beam * self.n)(*beams)
+        _lib.check(_lib.lib().ch_beamset_update(self.handle, self.n, arr,
+                                                _h(stream)))
+        # the previous beams' objects outlive launches queued before this
+        self._prev, self.keep = self.keep, list(keep)
+
+    def run(self, stream=None):
+        _lib.check(_lib.lib().ch_terma_beamset(self.handle, _h(stream)))
+
+    def flags(self, stream=None):
+        """Per-beam geometry flags (1: a ray in the source plane), cleared."""
+        out = np.zeros(self.n, np.int32)
+        _lib.check(_lib.lib().ch_beamset_flags(
+            self.handle, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+            _h(stream)))
+        return out
+
+    def close(self):
+        if self.handle:
+            _lib.lib().ch_beamset_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class ConvSet:
+    """Device-resident FFT-form convolution jobs (ch_convset): the pointer
+    tables of convolve_fft_batch written to the device once, so run()
+    queues kernels only and can be captured in a Graph.  Jobs are (plan,
+    terma, dose, accumulate); missing plan spectra are built here."""
+
+    def __init__(self, jobs, stream=None):
+        n = len(jobs)
+        if n == 0:
+            raise ValueError("a convolution set needs jobs")
+        arr = (_lib.ConvJob * n)()
+        seen = set()
+        for i, (plan, t, d, acc) in enumerate(jobs):
+            if plan.handle not in seen:
+                plan.spectrum(stream)
+                seen.add(plan.handle)
+            arr[i].plan = plan.handle
+            arr[i].d_terma = t.ptr
+            arr[i].d_dose = d.ptr
+            arr[i].d_box = None
+            arr[i].accumulate = 1 if acc else 0
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().ch_convset_create(n, arr, ctypes.byref(h)))
+        self.handle = h.value
+        self.keep = [(p, t, d) for p, t, d, _ in jobs]
+
+    def run(self, stream=None):
+        _lib.check(_lib.lib().ch_convolve_fft_set(self.handle, _h(stream)))
+
+    def close(self):
+        if self.handle:
+            _lib.lib().ch_convset_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def terma_check(stream=None):

@@ -212,6 +212,22 @@ ch_status ch_convolve_fft(const ch_plan* plan, const float* d_terma,
 ch_status ch_convolve_fft_batch(int32_t n_jobs, const ch_conv_job* h_jobs,
                                 void* stream);
 
+/* A device-resident convolution set: the jobs of ch_convolve_fft_batch
+ * (same grouping: one launch per FFT pass for all jobs of one plan) with
+ * their pointer tables written to device memory once, at creation.
+ * ch_convolve_fft_set then queues kernels only, so it can be captured in a
+ * graph (ch_graph_begin) and replayed; the graph pins the set and its
+ * plans (ch_convset_destroy is deferred to the last such graph).  The
+ * stream's FFT work grids must already be large enough when it is
+ * captured: run the set once on that stream first.  The jobs' buffers and
+ * plans must outlive the set.  No counterpart in the reference, whose
+ * calculate handles one beam per call (conehead/conehead.py:536-738). */
+typedef struct ch_convset ch_convset;
+ch_status ch_convset_create(int32_t n_jobs, const ch_conv_job* h_jobs,
+                            ch_convset** out);
+ch_status ch_convolve_fft_set(const ch_convset* set, void* stream);
+ch_status ch_convset_destroy(ch_convset* set);
+
 /* Up to CH_MAX_GROUP beams on the same plan convolved together: their
  * TERMA grids are interleaved so each tap is one 16-byte load for all of
  * them, and the tap walk is shared.  Tap ranges use the union of the
@@ -293,6 +309,26 @@ ch_status ch_resample_linear(const float* d_src, const int32_t src_shape[3],
                              const double dst_spacing[3], int32_t xy_swap,
                              void* stream);
 
+/* CT numbers to densities on the phantom grid, before resampling.  The
+ * reference lists CT import and density overrides as milestones it has not
+ * built (README.md:26-27); its calculate interpolates whatever density grid
+ * the phantom holds (conehead/conehead.py:525-533), so this produces that
+ * grid.  d_ct holds n CT numbers (CH_CT_I16: int16, CH_CT_F32: float32);
+ * each is mapped through the calibration curve (curve_hu strictly
+ * increasing, n_points <= CH_CT_MAX_POINTS, host arrays) exactly as
+ * numpy.interp(ct, curve_hu, curve_rho) in FP64, rounded to FP32.  With a
+ * label map d_labels (uint8 per voxel, nullable) the voxels whose label L
+ * has a non-NaN override_rho[L] (256 host floats) take that density
+ * instead: a density override per structure. */
+#define CH_CT_I16 0
+#define CH_CT_F32 1
+#define CH_CT_MAX_POINTS 32
+ch_status ch_ct_to_density(const void* d_ct, int32_t ct_type, int64_t n,
+                           const double* curve_hu, const double* curve_rho,
+                           int32_t n_points, const uint8_t* d_labels,
+                           const float* override_rho, float* d_rho,
+                           void* stream);
+
 /* ------------------------------------------------------------ graphs --- */
 
 /* Capture of a pipeline of calls on `stream` into a HIP graph, replayed by
@@ -368,10 +404,14 @@ typedef struct ch_beam {
                                    ch_terma_check reads */
 } ch_beam;
 
-/* TERMA for n beams on one dose grid, queued in order on `stream` (one
- * fused launch per beam, so accumulating beams never race): hit test, FP64-DDA
- * radiological depth toward the source, off-axis distance, three-source
- * fluence, softening, horn and the polyenergetic TERMA sum.
+/* TERMA for n beams on one dose grid, queued in order on `stream`: hit
+ * test, FP64-DDA radiological depth toward the source, off-axis distance,
+ * three-source fluence, softening, horn and the polyenergetic TERMA sum.
+ * Runs of beams with distinct d_terma share one grid of workgroups (their
+ * parameters staged into a per-stream device table); a beam that adds into
+ * the grid of an earlier beam of the run starts a new launch, so
+ * accumulating beams never race.  Under graph capture every beam takes its
+ * own launches (see ch_beamset for a captured multi-beam launch).
  * Replaces the kernel chain cuda_hit_test, cuda_d_eff, cuda_oad,
  * cuda_fluence, cuda_terma and the host glue between them
  * (conehead/conehead.py:71-377, :536-682).
@@ -397,6 +437,32 @@ ch_status ch_terma_noalloc(int32_t n_beams, const ch_beam* h_beams,
                            const double h_origin[3],
                            const double h_spacing[3], float* d_debug,
                            void* stream);
+
+/* A device-resident beam set: n beams on one grid (the ch_terma arguments
+ * without debug planes) whose kernel parameters live in a device table and
+ * whose launch scratch belongs to the set.  ch_terma_beamset queues the
+ * same launches as ch_terma (beams with distinct TERMA grids share one
+ * grid of workgroups) and nothing else, so it can be captured in a graph
+ * and replayed; the graph pins the set (ch_beamset_destroy is deferred to
+ * the last such graph).  ch_beamset_update rewrites the table for new beams
+ * (new geometry, apertures or outputs with the same sharing of TERMA grids
+ * and support boxes) by a copy queued on `stream`, so launches and replays
+ * queued later on that stream see the new beams; it cannot be captured.
+ * Beams without their own d_status flag into the set's per-beam flags,
+ * which ch_beamset_flags copies to h_flags (n int32) and clears, waiting
+ * for `stream`.  The beams' buffers must outlive the set.  The beams of
+ * one call of the reference's calculate are its single beam
+ * (conehead/conehead.py:536-682); the set is several such calls. */
+typedef struct ch_beamset ch_beamset;
+ch_status ch_beamset_create(int32_t n_beams, const ch_beam* h_beams,
+                            const int32_t h_shape[3],
+                            const double h_origin[3],
+                            const double h_spacing[3], ch_beamset** out);
+ch_status ch_beamset_update(ch_beamset* set, int32_t n_beams,
+                            const ch_beam* h_beams, void* stream);
+ch_status ch_terma_beamset(const ch_beamset* set, void* stream);
+ch_status ch_beamset_flags(ch_beamset* set, int32_t* h_flags, void* stream);
+ch_status ch_beamset_destroy(ch_beamset* set);
 
 /* The FP64 traversals of the TERMA rays toward the source from n voxels
  * (d_vox_in: n x 3 indices), with the same direction arithmetic as
