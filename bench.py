@@ -24,7 +24,11 @@ The same JSON line carries
   cpu_baseline  the FP64 oracle (TERMA + convolve_c) timed on this host's
                 cores for the same beam;
   batched       64 MLC beams at random gantry angles on the same phantom,
-                96 cones, all beams' convolutions sharing launches;
+                96 cones: the beams' TERMA sharing launches, then their
+                convolutions sharing launches -- eager, and captured once
+                as one graph from device-resident tables and replayed --
+                against the engine's single-beam graph step replayed over
+                the same 64 beams (speedup_vs_single_beam_graph);
   plain         the plain single-beam GPU path (per-stage synchronisation
                 and host round trips, scatter convolution with atomics) on
                 beams of the batched workload, and the batched/plain ratio;
@@ -529,11 +533,17 @@ def run_batched(args, eng, steps=5):
     s = Stream()
     nb = len(case.beams)
     per, outs, tile = {}, {}, None
+    from conehead_amd import autotune
+    plan = eng.plan(case.shape, case.spacing, case.cones)
+    fft_lines, _ = autotune.tune_fft(plan, n_beams=nb, stream=s,
+                                     use_cache=False)
     for method in ("gather", "fft"):
         log("batched: %s" % method)
         be = BatchEngine(eng, method=method)
         if method == "gather":
             tile = tune_requests(eng, be, [req], s)
+        else:
+            chunk, _ = autotune.tune_pipeline(be, [req], stream=s)
         ms, gms, out = time_batches(be, [req], s, steps)
         outs[method] = out[0]
         per[method] = {"ms_per_batch": ms, "gpu_ms_per_batch": gms,
@@ -542,18 +552,37 @@ def run_batched(args, eng, steps=5):
     be1 = BatchEngine(eng, group_beams=True)
     ms1, _, _ = time_batches(be1, [req], s, 2)
     per["gather"]["ms_per_batch_with_beam_groups"] = ms1
-    head = pick(args, per)
+    # the FFT form captured once and replayed (device-resident tables)
+    log("batched: fft captured")
+    gms, gout = time_batch_graph(eng, req, steps)
+    if not np.array_equal(gout, outs["fft"]):
+        raise RuntimeError("captured batch differs from the eager batch")
+    per["fft_captured"] = {"ms_per_batch": gms, "gpu_ms_per_batch": gms,
+                           "beams_per_s": nb / (gms * 1e-3)}
+    outs["fft_captured"] = gout
+    # the engine's single-beam graph step on the same 64 beams
+    log("batched: single-beam graph steps")
+    sms = time_single_beam_graphs(eng, case, outs["fft"], steps)
+    if args.method in ("fft", "gather"):
+        per = {m: v for m, v in per.items() if m.startswith(args.method)}
+    head = pick(argparse.Namespace(method="auto"), per)
     ms = per[head]["ms_per_batch"]
     r = {"workload": "64 MLC beams (gantry U[0,360), seed 2024, 60 leaf "
                      "pairs x 5 mm, apertures 2-20 cm) on the 128^3 "
-                     "heterogeneous phantom, 96 cones: TERMA per beam, "
-                     "then one launch per convolution kernel for all 64",
+                     "heterogeneous phantom, 96 cones: TERMA of the beams "
+                     "in shared launches, then one launch per convolution "
+                     "kernel for all 64",
          "method": head, "gather_tile": list(list(tile.values())[0]),
+         "fft_lines_per_workgroup": list(fft_lines),
+         "fft_terma_chunk": chunk,
          "ms_per_batch": ms,
          "gpu_ms_per_batch": per[head]["gpu_ms_per_batch"],
          "beams_per_s": nb / (ms * 1e-3),
-         "voxel_cone_updates_per_s":
+         "dense_equivalent_voxel_cone_updates_per_s":
              case.n_voxels * case.cones.n_cones * nb / (ms * 1e-3),
+         "single_beam_graph_ms_per_beam": sms,
+         "ms_per_beam": ms / nb,
+         "speedup_vs_single_beam_graph": sms / (ms / nb),
          "methods": per}
     if not args.no_cpu_baseline:
         ref, _, _, cpu_s = oracle_beam(case, 0, cpu_threads(args))
@@ -563,6 +592,71 @@ def run_batched(args, eng, steps=5):
         r["error_vs_fp64_oracle_beam0"] = \
             per[head]["error_vs_fp64_oracle_beam0"]
     return r, case
+
+
+def time_batch_graph(eng, req, steps):
+    """Median GPU ms of `steps` replays of the batch captured as one graph
+    (BatchGraph: beam set TERMA + convolution set FFT passes), timed with
+    events on the graph's stream; the doses of the last replay."""
+    from conehead_amd.batch import BatchGraph
+    from conehead_amd.device import Event
+    bg = BatchGraph([req], eng)
+    try:
+        bg.launch()
+        bg.stream.sync()
+        times = []
+        for _ in range(steps):
+            e0 = Event().record(bg.stream)
+            e1 = bg.launch()
+            e1.sync()
+            times.append(e0.elapsed_ms(e1))
+        return float(np.median(times)), bg.results()[0]
+    finally:
+        bg.close()
+
+
+def time_single_beam_graphs(eng, case, want, steps):
+    """The engine's single-beam step (TERMA, zero, FFT-form convolution)
+    captured once per beam and replayed beam after beam: median GPU ms per
+    beam over `steps` passes of all beams; each beam's dose must equal the
+    batch's (`want`) bit for bit."""
+    from conehead_amd import engine as E
+    from conehead_amd.device import DeviceArray, Event, Stream
+    s = Stream()
+    shape = case.shape
+    rho = DeviceArray.from_host(case.densities, stream=s)
+    plan = eng.plan(shape, case.spacing, case.cones)
+    ts = [DeviceArray(shape) for _ in case.beams]
+    ds = [DeviceArray(shape) for _ in case.beams]
+    beams = [E.beam_struct(src, st, eng.aperture(blk, s), rho, t, None)
+             for (src, blk, st), t in zip(case.beams, ts)]
+    E.convolve_fft(plan, ts[0], ds[0], stream=s)   # sizes the work grids
+    graphs = []
+    try:
+        for b, t, d in zip(beams, ts, ds):
+            with E.Graph.capture(s) as cap:
+                E.terma([b], shape, case.origin, case.spacing, stream=s,
+                        check=False)
+                d.zero(s)
+                E.convolve_fft(plan, t, d, accumulate=True, stream=s)
+            graphs.append(cap.graph)
+        times = []
+        for _ in range(steps + 1):
+            e0 = Event().record(s)
+            for g in graphs:
+                g.launch(s)
+            e1 = Event().record(s)
+            e1.sync()
+            times.append(e0.elapsed_ms(e1) / len(graphs))
+        for i, d in enumerate(ds):
+            if not np.array_equal(d.download(stream=s), want[i]):
+                raise RuntimeError("single-beam graph step differs from the "
+                                   "batch for beam %d" % i)
+        return float(np.median(times[1:]))
+    finally:
+        for g in graphs:
+            g.close()
+        s.sync()
 
 
 def run_plain(args, case, n_beams):

@@ -212,6 +212,8 @@ SIGNATURES = {
     "ch_plan_set_tile": (ctypes.c_int32, [c_vp, ctypes.c_int32,
                                           ctypes.c_int32, ctypes.c_int32]),
     "ch_plan_get_tile": (ctypes.c_int32, [c_vp, c_i32p, c_i32p, c_i32p]),
+    "ch_plan_set_fft_lines": (ctypes.c_int32, [c_vp, c_i32p]),
+    "ch_plan_get_fft_lines": (ctypes.c_int32, [c_vp, c_i32p]),
     "ch_plan_set_accum": (ctypes.c_int32, [c_vp, ctypes.c_int32]),
     "ch_terma": (ctypes.c_int32, [ctypes.c_int32, ctypes.POINTER(Beam),
                                   c_i32p, c_f64p, c_f64p, c_vp, c_vp]),

@@ -352,6 +352,9 @@ struct ch_plan : ch::GraphHeld {
     // exp(-2 pi i q / F).  Null until built.
     int32_t fft_n[3];
     int32_t fft_kp;
+    // lines per workgroup of each FFT-form pass (z forward, y forward,
+    // x convolution, y inverse, z inverse); 0 = the default for the length
+    int32_t fft_lines[5];
     float2* d_khat;
     float2* d_tw[3];
 };

@@ -994,12 +994,15 @@ __global__ void scatter_frame_kernel(float* __restrict__ frame,
 // ---------------------------------------------------------------- host --
 
 // Lines per workgroup: the most (up to 16) with F * W <= limit, which
-// kPerThread elements per thread always cover.
-int lines_per_group(int F, int limit = kFftLdsElems)
+// kPerThread elements per thread always cover; `want` (2, 4, 8 or 16)
+// takes fewer when they fit.
+int lines_per_group(int F, int limit = kFftLdsElems, int want = 0)
 {
     int w = 16;
     while (w > 2 && F * w > limit)
         w /= 2;
+    if ((want == 2 || want == 4 || want == 8 || want == 16) && want <= w)
+        return want;
     return w;
 }
 
@@ -1127,9 +1130,10 @@ ch_status launch_col(const BeamIO& io, float2* A, int64_t row_stride,
 
 // W is chosen from the transform length at run time.
 ch_status z_fwd(const BeamIO& io, float2* A, int X, int Y, int Z, int a_rows,
-                int kp, const FftLen& L, const float2* tw, hipStream_t s)
+                int kp, const FftLen& L, const float2* tw, hipStream_t s,
+                int want)
 {
-    switch (lines_per_group(L.n)) {
+    switch (lines_per_group(L.n, kFftLdsElems, want)) {
     case 16:
         return launch_z_fwd<16>(io, A, X, Y, Z, a_rows, kp, L, tw, s);
     case 8:
@@ -1143,9 +1147,9 @@ ch_status z_fwd(const BeamIO& io, float2* A, int X, int Y, int Z, int a_rows,
 
 ch_status z_inv(const BeamIO& io, const float2* A, int X, int Y, int Z,
                 int a_rows, int kp, const FftLen& L, const float2* tw,
-                hipStream_t s)
+                hipStream_t s, int want)
 {
-    switch (lines_per_group(L.n)) {
+    switch (lines_per_group(L.n, kFftLdsElems, want)) {
     case 16:
         return launch_z_inv<16>(io, A, X, Y, Z, a_rows, kp, L, tw, s);
     case 8:
@@ -1162,11 +1166,11 @@ ch_status col(const BeamIO& io, float2* A, int64_t row_stride,
               int64_t outer_stride, int n_outer, int n_cols, int n_in,
               int n_out, const FftLen& L, const float2* tw,
               const float2* khat, int64_t k_row_stride,
-              int64_t k_outer_stride, hipStream_t s,
+              int64_t k_outer_stride, hipStream_t s, int want,
               LaunchTimer* timer = nullptr)
 {
     switch (lines_per_group(
-        L.n, MODE == kColConv ? kFftLdsElemsConv : kFftLdsElems)) {
+        L.n, MODE == kColConv ? kFftLdsElemsConv : kFftLdsElems, want)) {
     case 16:
         return launch_col<16, MODE>(io, A, row_stride, outer_stride, n_outer,
                                     n_cols, n_in, n_out, L, tw, khat,
@@ -1231,26 +1235,27 @@ ch_status spectral_passes(const ch_plan* p, BeamIO io, float2* A,
     const FftLen Lx = plan_len(p, 0), Ly = plan_len(p, 1), Lz = plan_len(p, 2);
     io.a_beam = (int64_t)beam_points(p);
     const int64_t slab = (int64_t)Fy * kp;
-    ch_status st = z_fwd(io, A, X, Y, Z, Fy, kp, Lz, p->d_tw[2], s);
+    const int32_t* W = p->fft_lines;
+    ch_status st = z_fwd(io, A, X, Y, Z, Fy, kp, Lz, p->d_tw[2], s, W[0]);
     if (st == CH_OK)
         st = col<kColFwd>(io, A, kp, slab, X, kz, Y, Fy, Ly, p->d_tw[1],
-                          nullptr, 0, 0, s);
+                          nullptr, 0, 0, s, W[1]);
     if (st == CH_OK) {
         if (time_conv) {
             LaunchTimer timer(s);
             st = col<kColConv>(io, A, slab, kp, Fy, kz, X, X, Lx, p->d_tw[0],
-                               p->d_khat, slab, kp, s, &timer);
+                               p->d_khat, slab, kp, s, W[2], &timer);
             timer.stop();
         } else {
             st = col<kColConv>(io, A, slab, kp, Fy, kz, X, X, Lx, p->d_tw[0],
-                               p->d_khat, slab, kp, s);
+                               p->d_khat, slab, kp, s, W[2]);
         }
     }
     if (st == CH_OK)
         st = col<kColInv>(io, A, kp, slab, X, kz, Fy, Y, Ly, p->d_tw[1],
-                          nullptr, 0, 0, s);
+                          nullptr, 0, 0, s, W[3]);
     if (st == CH_OK)
-        st = z_inv(io, A, X, Y, Z, Fy, kp, Lz, p->d_tw[2], s);
+        st = z_inv(io, A, X, Y, Z, Fy, kp, Lz, p->d_tw[2], s, W[4]);
     return st;
 }
 
@@ -1486,13 +1491,13 @@ ch_status ch_plan_spectrum(ch_plan* plan, void* stream)
         const int64_t slab = (int64_t)F[1] * kp;
         // the whole frame is the grid here: no zero rows, all rows kept
         const BeamIO io = single_beam(frame, nullptr, 0);
-        st = z_fwd(io, khat, F[0], F[1], F[2], F[1], kp, Lz, tw[2], s);
+        st = z_fwd(io, khat, F[0], F[1], F[2], F[1], kp, Lz, tw[2], s, 0);
         if (st == CH_OK)
             st = col<kColFwd>(io, khat, kp, slab, F[0], kz, F[1], F[1], Ly,
-                              tw[1], nullptr, 0, 0, s);
+                              tw[1], nullptr, 0, 0, s, 0);
         if (st == CH_OK)
             st = col<kColFwd>(io, khat, slab, kp, F[1], kz, F[0], F[0], Lx,
-                              tw[0], nullptr, 0, 0, s);
+                              tw[0], nullptr, 0, 0, s, 0);
         if (st == CH_OK && hipStreamSynchronize(s) != hipSuccess) {
             set_error("ch_plan_spectrum: %s",
                       hipGetErrorString(hipGetLastError()));
@@ -1654,6 +1659,35 @@ ch_status ch_convset_destroy(ch_convset* set)
     if (defer_free(set))  // read by a live graph: freed with the last one
         return CH_OK;
     convset_release(set);
+    return CH_OK;
+}
+
+ch_status ch_plan_set_fft_lines(ch_plan* plan, const int32_t lines[5])
+{
+    CH_REQUIRE(plan && lines, "ch_plan_set_fft_lines: null pointer");
+    for (int k = 0; k < 5; ++k)
+        CH_REQUIRE(lines[k] == 0 || lines[k] == 2 || lines[k] == 4 ||
+                       lines[k] == 8 || lines[k] == 16,
+                   "ch_plan_set_fft_lines: pass %d: %d lines (0, 2, 4, 8 "
+                   "or 16)", k, lines[k]);
+    for (int k = 0; k < 5; ++k)
+        plan->fft_lines[k] = lines[k];
+    return CH_OK;
+}
+
+ch_status ch_plan_get_fft_lines(const ch_plan* plan, int32_t lines[5])
+{
+    CH_REQUIRE(plan && lines, "ch_plan_get_fft_lines: null pointer");
+    int32_t F[3];
+    for (int a = 0; a < 3; ++a)
+        F[a] = plan->fft_n[a] ? plan->fft_n[a]
+                              : smooth_at_least(2 * plan->shape[a] - 1);
+    const int* W = plan->fft_lines;
+    lines[0] = lines_per_group(F[2], kFftLdsElems, W[0]);
+    lines[1] = lines_per_group(F[1], kFftLdsElems, W[1]);
+    lines[2] = lines_per_group(F[0], kFftLdsElemsConv, W[2]);
+    lines[3] = lines_per_group(F[1], kFftLdsElems, W[3]);
+    lines[4] = lines_per_group(F[2], kFftLdsElems, W[4]);
     return CH_OK;
 }
 

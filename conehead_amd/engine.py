@@ -65,6 +65,18 @@ class Plan:
                                                ctypes.byref(c)))
         return a.value, b.value, c.value
 
+    def set_fft_lines(self, lines):
+        """Lines per workgroup of the FFT form's five passes (z forward,
+        y forward, x convolution, y inverse, z inverse); 0 = default."""
+        arr = (ctypes.c_int32 * 5)(*[int(v) for v in lines])
+        _lib.check(_lib.lib().ch_plan_set_fft_lines(self.handle, arr))
+
+    def fft_lines(self):
+        """The lines per workgroup the five passes use."""
+        arr = (ctypes.c_int32 * 5)()
+        _lib.check(_lib.lib().ch_plan_get_fft_lines(self.handle, arr))
+        return tuple(arr)
+
     def set_accum(self, mode):
         _lib.check(_lib.lib().ch_plan_set_accum(self.handle, int(mode)))
 
@@ -491,8 +503,9 @@ class Engine:
             p = self._plans.get(key)
             if p is None:
                 p = Plan(shape, voxel_size, cones)
-                from .autotune import default_tile
+                from .autotune import apply_fft, default_tile
                 p.set_tile(*(tile or default_tile(p)))
+                apply_fft(p)
                 self._plans[key] = p
             else:
                 self._plans.move_to_end(key)

@@ -255,6 +255,15 @@ ch_status ch_convolve_groups(int32_t n_groups, const ch_conv_group* h_groups,
 ch_status ch_plan_set_tile(ch_plan* plan, int32_t tx, int32_t ty, int32_t tz);
 ch_status ch_plan_get_tile(const ch_plan* plan, int32_t* tx, int32_t* ty,
                            int32_t* tz);
+/* Lines per workgroup of the FFT form's five passes (z forward, y
+ * forward, x convolution, y inverse, z inverse); set by the autotuner.
+ * Each value is 0 (the default: the most lines, up to 16, whose frames fit
+ * the pass's LDS buffer) or 2, 4, 8 or 16; a value whose frames do not fit
+ * falls back to the default.  Every choice computes the same butterflies
+ * on every line, so the dose does not depend on it.  get returns the
+ * values the passes use. */
+ch_status ch_plan_set_fft_lines(ch_plan* plan, const int32_t lines[5]);
+ch_status ch_plan_get_fft_lines(const ch_plan* plan, int32_t lines[5]);
 /* Accumulation mode: 0 = Kahan per tap, 1 = plain per cone line with Kahan
  * across lines, 2 = Kahan over blocks of 8 taps (default). */
 ch_status ch_plan_set_accum(ch_plan* plan, int32_t mode);
