@@ -49,6 +49,26 @@ struct RadJob {
 constexpr int kRadTY = 4, kRadTZ = 16, kRadWaves = 8;
 constexpr int kRadRow = kMaxIndex + 1;
 
+// trunc(x) as int32, saturating (v_cvt_i32_f64 clamps to the int32 range
+// and reads NaN as 0; a C++ cast of an out-of-range value is undefined).
+__device__ __forceinline__ int trunc_sat(double x)
+{
+    int r;
+    asm("v_cvt_i32_f64 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+
+// |floor(x - 5)| clamped to kMaxIndex, x = 100 R as the FP64 product.
+__device__ __forceinline__ int rad_index_of(double x)
+{
+    const double f = floor(x - 5.0);
+    if (!(f < 2147483647.0 && f > -2147483648.0))
+        return kMaxIndex;
+    int i = (int)f;
+    i = i < 0 ? -i : i;
+    return i > kMaxIndex ? kMaxIndex : i;
+}
+
 __device__ __forceinline__ int rad_index(double r)
 {
     const double f = floor(r * 100.0 - 5.0);
@@ -65,6 +85,34 @@ __device__ __forceinline__ void rad_kahan(float& s, float& c, float x)
     const float t = s + y;
     c = (t - s) - y;
     s = t;
+}
+
+// Densities (and TERMA) of kUnroll taps from tap t of the tap tables ml,
+// relative to the lane's voxel at byte offset base4 of the padded grids.
+__device__ __forceinline__ void load_taps(__amdgpu_buffer_rsrc_t rr,
+                                          uint32_t base4, const c_i32* ml,
+                                          int t, float (&rho)[kUnroll])
+{
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u)
+        rho[u] = __int_as_float(__builtin_amdgcn_raw_buffer_load_b32(
+            rr, (int)(base4 - (uint32_t)ml[t + u]), 0, 0));
+}
+
+__device__ __forceinline__ void load_taps2(__amdgpu_buffer_rsrc_t rr,
+                                           __amdgpu_buffer_rsrc_t rt,
+                                           uint32_t base4, const c_i32* ml,
+                                           int t, float (&rho)[kUnroll],
+                                           float (&tv)[kUnroll])
+{
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+        const int o = (int)(base4 - (uint32_t)ml[t + u]);
+        rho[u] = __int_as_float(
+            __builtin_amdgcn_raw_buffer_load_b32(rr, o, 0, 0));
+        tv[u] = __int_as_float(
+            __builtin_amdgcn_raw_buffer_load_b32(rt, o, 0, 0));
+    }
 }
 
 __global__ __launch_bounds__(64 * kRadWaves) void gather_rad(RadJob j,
@@ -162,12 +210,7 @@ __global__ __launch_bounds__(64 * kRadWaves) void gather_rad(RadJob j,
                 int t = 0;
                 for (; t + kUnroll <= ulo; t += kUnroll) {
                     float rho[kUnroll];
-#pragma unroll
-                    for (int u = 0; u < kUnroll; ++u)
-                        rho[u] = __int_as_float(
-                            __builtin_amdgcn_raw_buffer_load_b32(
-                                rr, (int)(base4 - (uint32_t)ml[t + u]), 0,
-                                0));
+                    load_taps(rr, base4, ml, t, rho);
 #pragma unroll
                     for (int u = 0; u < kUnroll; ++u)
                         R = R + (double)rho[u] * ms[t + u];
@@ -185,21 +228,39 @@ __global__ __launch_bounds__(64 * kRadWaves) void gather_rad(RadJob j,
                 float acc = 0.0f, ac = 0.0f;
                 for (t = ulo; t < uhi; t += kUnroll) {
                     float rho[kUnroll], tv[kUnroll];
+                    load_taps2(rr, rt, base4, ml, t, rho, tv);
+                    // The index |floor(100 R - 5)| clamped to kMaxIndex.
+                    // For x = 100 R >= 5 the subtraction of 5 is exact
+                    // (the difference is a multiple of ulp(x) below x), so
+                    // the index is min(trunc(x), 6000) - 5; a chunk with a
+                    // tap below 5 (or NaN, which truncates to 0) takes
+                    // rad_index_of for all its taps.
+                    double x[kUnroll];
+                    int idx[kUnroll];
+                    bool slow = false;
 #pragma unroll
                     for (int u = 0; u < kUnroll; ++u) {
-                        const int o = (int)(base4 - (uint32_t)ml[t + u]);
-                        rho[u] = __int_as_float(
-                            __builtin_amdgcn_raw_buffer_load_b32(rr, o, 0,
-                                                                 0));
-                        tv[u] = __int_as_float(
-                            __builtin_amdgcn_raw_buffer_load_b32(rt, o, 0,
-                                                                 0));
+                        R = R + (double)rho[u] * ms[t + u];
+                        x[u] = R * 100.0;
+                        const int ti = trunc_sat(x[u]);
+                        slow = slow || ti < 5;
+                        idx[u] = min(ti, kMaxIndex + 5) - 5;
+                    }
+                    if (__builtin_amdgcn_ballot_w64(slow) != 0) {
+                        // a real branch: x passes through an empty asm
+                        // here, so the compiler cannot hoist this work
+                        // onto every chunk
+#pragma unroll
+                        for (int u = 0; u < kUnroll; ++u) {
+                            double xu = x[u];
+                            asm volatile("" : "+v"(xu));
+                            idx[u] = rad_index_of(xu);
+                        }
                     }
                     float part = 0.0f;
 #pragma unroll
                     for (int u = 0; u < kUnroll; ++u) {
-                        R = R + (double)rho[u] * ms[t + u];
-                        const double cv = s_cum[rad_index(R)];
+                        const double cv = s_cum[idx[u]];
                         const float k = (float)(cv - cprev);
                         cprev = cv;
                         part = fmaf(k, tv[u], part);

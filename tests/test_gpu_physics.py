@@ -181,3 +181,68 @@ def test_water_pdd_against_tank(engine):
     for d, c, m in zip(depth[::4], calc[::4], meas[::4]):
         print("%5.2f %6.2f %6.2f" % (d, c, m))
     assert dev < 2.5
+
+
+def test_water_profiles_against_tank(engine):
+    """Validation, not parity: physics-mode cross-profiles of a 10 x 10 cm
+    field in water (SSD 100, 12 x 12 cones, EDKnrc kernel) at 5, 10, 20
+    and 30 cm depth against the measured tank profiles the reference plots
+    beside its own (conehead/conehead.py:955-1014,
+    tests/golden/tank_10x10.json), each normalised to its central value as
+    the reference does.  Bounds, in % of the central value: inside the
+    field (|x| <= 0.8 of the field half-width projected to that depth)
+    within IN_FIELD, outside it (|x| >= 1.25 of the half-width) within
+    OUT_FIELD, and the 50 % crossing of each edge within EDGE_MM."""
+    import json
+    import os
+
+    from conehead_amd.block import Block
+    from conehead_amd.conehead import Conehead
+    from conehead_amd.phantom import Phantom
+    IN_FIELD, OUT_FIELD, EDGE_MM = 4.0, 4.0, 3.0
+    prof = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                       "tank_10x10.json")))["profiles"]
+    nx, ny, s = 56, 66, 0.5
+    origin = np.array([-(nx - 1) / 2 * s, s / 2, -(nx - 1) / 2 * s])
+    phantom = Phantom(np.ones((nx, ny, nx), np.float32), origin, [s] * 3)
+    blk = Block()
+    blk.set_square(10)
+    settings = dict(W.SETTINGS, semantics="physics", density_xy_swap=False)
+    dose = Conehead(engine).calculate(Source("varian_clinac_6MV"), blk,
+                                      phantom, settings)
+    xs = origin[0] + s * np.arange(nx)
+    ys = origin[1] + s * np.arange(ny)
+    zc = nx // 2 - 1                  # the voxel pair around z = 0
+    worst = {}
+    for depth_key, data in prof.items():
+        depth = float(depth_key)
+        x_m = np.asarray(data["x_cm"], float)
+        d_m = np.asarray(data["dose_pct"], float)
+        # dose along x at this depth: linear in y, mean of the z pair
+        line = dose[:, :, zc:zc + 2].mean(axis=2)
+        calc = np.array([np.interp(depth, ys, line[i]) for i in range(nx)])
+        calc = 100.0 * calc / np.interp(0.0, xs, calc)
+        c_at = np.interp(x_m, xs, calc)
+        half = 5.0 * (100.0 + depth) / 100.0
+        inside = np.abs(x_m) <= 0.8 * half
+        outside = (np.abs(x_m) >= 1.25 * half) & (np.abs(x_m) <= xs[-1])
+        dev_in = np.abs(c_at - d_m)[inside].max()
+        dev_out = np.abs(c_at - d_m)[outside].max()
+
+        def crossing(x, d, sign):
+            sel = (x * sign > 0)
+            xx, dd = x[sel] * sign, d[sel]
+            o = np.argsort(xx)
+            xx, dd = xx[o], dd[o]
+            k = np.nonzero(dd < 50.0)[0][0]
+            return xx[k - 1] + (50.0 - dd[k - 1]) * (xx[k] - xx[k - 1]) / \
+                (dd[k] - dd[k - 1])
+        edge = max(abs(crossing(xs, calc, sg) - crossing(x_m, d_m, sg))
+                   for sg in (-1, 1)) * 10.0
+        worst[depth] = (dev_in, dev_out, edge)
+        print("depth %4.1f cm: in-field %.2f %%, out-of-field %.2f %%, "
+              "50%% edge %.2f mm" % (depth, dev_in, dev_out, edge))
+    for depth, (dev_in, dev_out, edge) in worst.items():
+        assert dev_in < IN_FIELD, (depth, dev_in)
+        assert dev_out < OUT_FIELD, (depth, dev_out)
+        assert edge < EDGE_MM, (depth, edge)
