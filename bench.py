@@ -630,7 +630,9 @@ def time_single_beam_graphs(eng, case, want, steps):
     ds = [DeviceArray(shape) for _ in case.beams]
     beams = [E.beam_struct(src, st, eng.aperture(blk, s), rho, t, None)
              for (src, blk, st), t in zip(case.beams, ts)]
-    E.convolve_fft(plan, ts[0], ds[0], stream=s)   # sizes the work grids
+    # one eager step sizes the stream's TERMA scratch and FFT work grids
+    E.terma([beams[0]], shape, case.origin, case.spacing, stream=s)
+    E.convolve_fft(plan, ts[0], ds[0], stream=s)
     graphs = []
     try:
         for b, t, d in zip(beams, ts, ds):
