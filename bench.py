@@ -44,6 +44,7 @@ units at the delivered beam rate -- the dense count Nv x C per beam and the
 reference's count of dose += T * k executions; the FFT form reaches the same
 doses without performing them one by one, so for it they are dense
 equivalents (updates_are_dense_equivalent).
+--workload batched  the 64-beam MLC batch alone (one step = one batch).
 --workload uneven   the uneven batch alone, with both convolutions.
 --workload serve    the request loop alone, 64 requests per lambda.
 """
@@ -71,7 +72,7 @@ def parse():
     p.add_argument("--steps", type=int, default=50)
     p.add_argument("--warmup", type=int, default=5)
     p.add_argument("--workload", default="hetero",
-                   choices=["hetero", "uneven", "serve"])
+                   choices=["hetero", "batched", "uneven", "serve"])
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-extras", action="store_true",
                    help="skip the batched and plain sections")
@@ -863,6 +864,18 @@ def main():
 
     if args.workload == "uneven":
         line, _ = run_uneven(args, eng)
+    elif args.workload == "batched":
+        bt, _ = run_batched(args, eng, steps=max(args.steps, 1))
+        line = {"metric": METRIC, "value": bt["beams_per_s"],
+                "unit": "beams/s", "n_gpus": 1, "steps": max(args.steps, 1),
+                "warmup": args.warmup, "ms_per_step": bt["ms_per_batch"],
+                "higher_is_better": True, "scaling": "weak",
+                "vs_baseline": None, "dtype": "f32",
+                "data": "synthetic (seeded phantom and MLC apertures; "
+                        "EDKnrc varian_6 kernel)",
+                "config": {"workload": bt.pop("workload"),
+                           "method": bt["method"]}}
+        line.update(bt)
     elif args.workload == "serve":
         line = run_serve(args, eng)
     else:
