@@ -477,23 +477,70 @@ __host__ __device__ __forceinline__ int shard_cap(int n_bricks)
     return (n_bricks + kShards - 1) / kShards;
 }
 
-// Phase 1: one wavefront per brick runs stage 1 of the hit test.  A brick
-// that reads 0 everywhere gets T = 0 here; every other brick (all of them
-// when the debug planes are asked for) is appended to its cost class for
-// phase 2, with its stage-1 result.
-__device__ __forceinline__ void sort_brick(const TermaParams& P, int b,
+// Phase 1: stage 1 of the hit test for eight bricks per wavefront, the
+// 2 x 2 x 2 bricks of one super-brick, one group of 8 lanes each (the cell
+// box reduction stays within each group).  A brick that reads 0
+// everywhere gets T = 0 here; every other brick (all of them when the
+// debug planes are asked for) is appended to its cost class for phase 2,
+// with its stage-1 result.
+struct SuperGrid {
+    int nbx, nby, nbz;   // bricks per axis
+    int sby, sbz;        // super-bricks per axis (y, z)
+};
+__device__ __forceinline__ SuperGrid super_grid(const TermaParams& P)
+{
+    SuperGrid g;
+    g.nbx = (P.X + 3) / 4;
+    g.nby = (P.Y + 3) / 4;
+    g.nbz = (P.Z + 3) / 4;
+    g.sby = (g.nby + 1) / 2;
+    g.sbz = (g.nbz + 1) / 2;
+    return g;
+}
+__host__ __device__ __forceinline__ int32_t super_bricks(int X, int Y, int Z)
+{
+    return (((X + 3) / 4 + 1) / 2) * (((Y + 3) / 4 + 1) / 2) *
+           (((Z + 3) / 4 + 1) / 2);
+}
+
+// Brick index of child c (0..7) of super-brick sb, or -1 outside the grid.
+__device__ __forceinline__ int child_brick(const SuperGrid& g, int sb, int c)
+{
+    const int sz = sb % g.sbz, sy = (sb / g.sbz) % g.sby,
+              sx = sb / g.sbz / g.sby;
+    const int bx = 2 * sx + (c >> 2), by = 2 * sy + ((c >> 1) & 1),
+              bz = 2 * sz + (c & 1);
+    if (bx >= g.nbx || by >= g.nby || bz >= g.nbz)
+        return -1;
+    return (bx * g.nby + by) * g.nbz + bz;
+}
+
+__device__ __forceinline__ void sort_super(const TermaParams& P, int sb,
                                            int l, int32_t n_bricks)
 {
-    const Brick k = brick_of(P, b);
+    const SuperGrid g = super_grid(P);
+    const int child = l >> 3;
+    const int b = child_brick(g, sb, child);
+    // every lane takes part in the group reductions (a missing child tests
+    // brick 0 and its result is dropped)
+    const Brick k = brick_of(P, b < 0 ? 0 : b);
     const float v = brick_value(P, k, l);
-    if (v == 0.0f && P.debug == nullptr) {
-        const int x = k.x0 + (l >> 4), y = k.y0 + ((l >> 2) & 3),
-                  z = k.z0 + (l & 3);
-        if (x < P.X && y < P.Y && z < P.Z && !P.accumulate)
-            P.terma[((int64_t)x * P.Y + y) * P.Z + z] = 0.0f;
-        return;
+    const bool zero = b >= 0 && v == 0.0f && P.debug == nullptr;
+    // bit 8 c set when child c reads 0: lane l writes voxel l of each
+    const uint64_t zmask = __ballot(zero) & 0x0101010101010101ull;
+    if (zmask != 0 && !P.accumulate) {
+        const int dx = l >> 4, dy = (l >> 2) & 3, dz = l & 3;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            if (!((zmask >> (8 * c)) & 1))
+                continue;  // wavefront-uniform
+            const Brick kc = brick_of(P, child_brick(g, sb, c));
+            const int x = kc.x0 + dx, y = kc.y0 + dy, z = kc.z0 + dz;
+            if (x < P.X && y < P.Y && z < P.Z)
+                P.terma[((int64_t)x * P.Y + y) * P.Z + z] = 0.0f;
+        }
     }
-    if (l == 0) {
+    if (b >= 0 && !zero && (l & 7) == 0) {
         const int q = brick_cost_class(P, k, v < 0.0f);
         const int c = q * kShards + b % kShards;
         const int at = atomicAdd(&P.counts[c], 1);
@@ -506,10 +553,10 @@ __device__ __forceinline__ void sort_brick(const TermaParams& P, int b,
 __global__ __launch_bounds__(256) void terma_sort_kernel(TermaParams P,
                                                          int32_t n_bricks)
 {
-    const int b = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (b >= n_bricks)
+    const int sb = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (sb >= super_bricks(P.X, P.Y, P.Z))
         return;  // wavefront-uniform
-    sort_brick(P, b, threadIdx.x & 63, n_bricks);
+    sort_super(P, sb, threadIdx.x & 63, n_bricks);
 }
 
 // Phase 1 of several beams on one grid shape in one launch: the blocks of
@@ -520,11 +567,12 @@ __global__ __launch_bounds__(256) void terma_sort_set_kernel(
     int32_t blocks_per_beam)
 {
     const int beam = blockIdx.x / blocks_per_beam;
-    const int b = (blockIdx.x - beam * blocks_per_beam) * 4 +
-                  (threadIdx.x >> 6);
-    if (b >= n_bricks)
+    const TermaParams& P = tab[beam];
+    const int sb = (blockIdx.x - beam * blocks_per_beam) * 4 +
+                   (threadIdx.x >> 6);
+    if (sb >= super_bricks(P.X, P.Y, P.Z))
         return;  // wavefront-uniform
-    sort_brick(tab[beam], b, threadIdx.x & 63, n_bricks);
+    sort_super(P, sb, threadIdx.x & 63, n_bricks);
 }
 
 // TERMA of one brick (one wavefront, every lane calls it) whose stage-1 hit
@@ -1011,18 +1059,20 @@ static ch_status terma_launch(const TermaParams* h, const TermaParams* d_tab,
                               hipEvent_t ev1, hipStream_t s)
 {
     const unsigned bpb = (unsigned)((n_bricks + 3) / 4);
+    const unsigned spb =
+        (unsigned)((super_bricks(h[0].X, h[0].Y, h[0].Z) + 3) / 4);
     const int nc = n * kCounters;
     hipLaunchKernelGGL(terma_clear_kernel, dim3((unsigned)(nc + 255) / 256),
                        dim3(256), 0, s, h[0].counts, nc);
     if (!d_tab) {
-        hipExtLaunchKernelGGL(terma_sort_kernel, dim3(bpb), dim3(256), 0, s,
+        hipExtLaunchKernelGGL(terma_sort_kernel, dim3(spb), dim3(256), 0, s,
                               ev0, nullptr, 0, h[0], n_bricks);
         hipExtLaunchKernelGGL(terma_kernel, dim3(bpb), dim3(256), 0, s,
                               nullptr, ev1, 0, h[0], n_bricks);
     } else {
-        hipExtLaunchKernelGGL(terma_sort_set_kernel, dim3(bpb * n),
+        hipExtLaunchKernelGGL(terma_sort_set_kernel, dim3(spb * n),
                               dim3(256), 0, s, ev0, nullptr, 0, d_tab,
-                              n_bricks, (int32_t)bpb);
+                              n_bricks, (int32_t)spb);
         hipExtLaunchKernelGGL(terma_set_kernel, dim3(bpb * n), dim3(256), 0,
                               s, nullptr, ev1, 0, d_tab, n_bricks,
                               (int32_t)bpb);
