@@ -45,6 +45,8 @@ class CHError(RuntimeError):
 CH_APERTURE_RASTER = 0
 CH_APERTURE_RECTS = 1
 CH_APERTURE_MLC = 2
+CH_F32 = 0          # ch_convert element types
+CH_F64 = 1
 
 
 class Aperture(ctypes.Structure):
@@ -200,6 +202,9 @@ SIGNATURES = {
         ctypes.c_int32, ctypes.POINTER(ConvJob), ctypes.POINTER(c_vp)]),
     "ch_convolve_fft_set": (ctypes.c_int32, [c_vp, c_vp]),
     "ch_convset_destroy": (ctypes.c_int32, [c_vp]),
+    "ch_convert": (ctypes.c_int32, [c_vp, ctypes.c_int32, c_vp,
+                                    ctypes.c_int32, ctypes.c_size_t,
+                                    ctypes.c_int32, c_vp]),
     "ch_graph_begin": (ctypes.c_int32, [c_vp]),
     "ch_graph_end": (ctypes.c_int32, [c_vp, ctypes.POINTER(c_vp)]),
     "ch_graph_launch": (ctypes.c_int32, [c_vp, c_vp]),

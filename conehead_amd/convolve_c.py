@@ -3,17 +3,26 @@
 Same signature and semantics: `dose_grid_dose` (float64, any strides) is
 accumulated in place with the collapsed-cone convolution of
 `dose_grid_terma`, for the cones thetas x phis (degrees) and the cumulative
-kernel tables kernel.cumulative["%.2f" % phi].  The work runs on the GPU as
-the deterministic stencil gather (FP32 with compensated sums) and the result
-is added to the FP64 dose on the device.
+kernel tables kernel.cumulative["%.2f" % phi].  The work runs on the GPU:
+the FP64 TERMA is converted to FP32 on the device, the convolution runs in
+the FFT form (grids whose axes fit its frames) or as the deterministic
+stencil gather, and the FP32 result is added into the FP64 dose on the
+device.  Cone sets, plans and device buffers are kept between calls.
 """
+import threading
+
 import numpy as np
 
-from .cones import ConeSet
-from .device import DeviceArray
+from . import _lib
 from . import engine as _engine
+from .cones import ConeSet, kernel_rows
+from .device import DeviceArray
 
 _ENGINE = None
+_LOCK = threading.Lock()
+_CALL_LOCK = threading.Lock()   # one call at a time owns the buffers
+_CONES = {}     # (thetas, phis, id(kernel)) -> (kernel rows, ConeSet)
+_BUFS = {}      # grid shape -> device buffers of one call
 
 
 def default_engine():
@@ -37,29 +46,89 @@ def _check_grid(a, name, writable=False):
     return a
 
 
+def _cones(thetas, phis, kernel):
+    """The ConeSet of these angles and kernel, reused while the kernel's
+    rows are unchanged (they are compared by value on every call)."""
+    th = np.asarray(thetas, dtype=np.float64).ravel()
+    ph = np.asarray(phis, dtype=np.float64).ravel()
+    rows = kernel_rows(kernel, ph) if len(ph) else None
+    key = (th.tobytes(), ph.tobytes(), id(kernel))
+    with _LOCK:
+        hit = _CONES.get(key)
+    if hit is not None and (rows is None or np.array_equal(hit[0], rows)):
+        return hit[1]
+    cones = ConeSet(th, ph, kernel, semantics="convolve_c")
+    with _LOCK:
+        if len(_CONES) > 64:
+            _CONES.clear()
+        _CONES[key] = (None if rows is None else rows.copy(), cones)
+    return cones
+
+
+def _buffers(shape):
+    with _LOCK:
+        b = _BUFS.get(shape)
+        if b is None:
+            if len(_BUFS) > 4:
+                _BUFS.clear()
+            b = _BUFS[shape] = {
+                "t64": DeviceArray(shape, np.float64),
+                "t": DeviceArray(shape, np.float32),
+                "part": DeviceArray(shape, np.float32),
+                "dose": DeviceArray(shape, np.float64),
+                "box": DeviceArray((6,), np.int32)}
+        return b
+
+
+def _convert(dst, src, accumulate=False):
+    types = {np.dtype(np.float32): _lib.CH_F32,
+             np.dtype(np.float64): _lib.CH_F64}
+    _lib.check(_lib.lib().ch_convert(dst.ptr, types[dst.dtype], src.ptr,
+                                     types[src.dtype], dst.size,
+                                     1 if accumulate else 0, None))
+
+
 def convolve_c(dose_grid_terma, dose_grid_dose, dose_grid_dim, thetas, phis,
-               kernel):
-    """Accumulate the cone convolution of TERMA into `dose_grid_dose`."""
+               kernel, method="auto"):
+    """Accumulate the cone convolution of TERMA into `dose_grid_dose`.
+
+    method: "auto" (the FFT form where the grid fits its frames, else the
+    gather), "fft" or "gather"."""
     terma = _check_grid(dose_grid_terma, "dose_grid_terma")
     dose = _check_grid(dose_grid_dose, "dose_grid_dose", writable=True)
     if dose.shape != terma.shape:
         # the reference indexes dose with the TERMA grid's voxel indices
         raise ValueError("dose grid %s does not match TERMA grid %s" %
                          (dose.shape, terma.shape))
+    if method not in ("auto", "fft", "gather"):
+        raise ValueError("method must be auto, fft or gather")
     dims = np.asarray(dose_grid_dim, dtype=np.float64).reshape(3)
-    cones = ConeSet(thetas, phis, kernel, semantics="convolve_c")
+    cones = _cones(thetas, phis, kernel)
     if cones.n_cones == 0:
         return None
+    from .batch import spectral_ok
+    shape = tuple(int(v) for v in terma.shape)
+    fft = method == "fft" or (method == "auto" and spectral_ok(shape))
     eng = default_engine()
-    plan = eng.plan(terma.shape, dims, cones)
-    d_t = DeviceArray.from_host(np.ascontiguousarray(terma, np.float32))
-    d_box = DeviceArray((6,), np.int32)
-    _engine.support_box(d_t, terma.shape, d_box)
-    d_dose = DeviceArray.from_host(np.ascontiguousarray(dose))
-    _engine.convolve_f64(plan, d_t, d_dose, d_box)
-    out = d_dose.download()
-    if dose.flags.c_contiguous:
-        dose[...] = out
-    else:
-        np.copyto(dose, out)
+    plan = eng.plan(shape, dims, cones)
+    with _CALL_LOCK:
+        _run(plan, shape, terma, dose, fft)
     return None
+
+
+def _run(plan, shape, terma, dose, fft):
+    b = _buffers(shape)
+    b["t64"].upload(np.ascontiguousarray(terma))
+    _convert(b["t"], b["t64"])
+    b["dose"].upload(np.ascontiguousarray(dose))
+    if fft:
+        plan.spectrum()
+        _engine.convolve_fft(plan, b["t"], b["part"])
+        _convert(b["dose"], b["part"], accumulate=True)
+    else:
+        _engine.support_box(b["t"], shape, b["box"])
+        _engine.convolve_f64(plan, b["t"], b["dose"], b["box"])
+    if dose.flags.c_contiguous:
+        b["dose"].download(out=dose)
+    else:
+        np.copyto(dose, b["dose"].download())

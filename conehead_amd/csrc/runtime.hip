@@ -84,6 +84,20 @@ __global__ void zero_kernel(uint8_t* p, size_t n)
         p[j] = 0;
 }
 
+// Elementwise conversion between FP32 and FP64 grids, dst = src or
+// dst += src (the sum rounded once in the destination type).
+template <typename D, typename S>
+__global__ void convert_kernel(D* __restrict__ dst, const S* __restrict__ src,
+                               size_t n, int accumulate)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += stride) {
+        const D v = (D)src[i];
+        dst[i] = accumulate ? dst[i] + v : v;
+    }
+}
+
 // Grid-stride 16-byte copy plus a byte tail (device to device).
 __global__ void copy_kernel(uint8_t* __restrict__ dst,
                             const uint8_t* __restrict__ src, size_t n)
@@ -529,6 +543,38 @@ ch_status ch_zero(void* d_ptr, size_t bytes, void* stream)
         blocks = 1;
     hipLaunchKernelGGL(zero_kernel, dim3((unsigned)blocks), dim3(256), 0,
                        (hipStream_t)stream, (uint8_t*)d_ptr, bytes);
+    CH_HIP(hipGetLastError());
+    return CH_OK;
+}
+
+ch_status ch_convert(void* d_dst, int32_t dst_type, const void* d_src,
+                     int32_t src_type, size_t n, int32_t accumulate,
+                     void* stream)
+{
+    if (n == 0)
+        return CH_OK;
+    CH_REQUIRE(d_dst && d_src, "ch_convert: null pointer");
+    CH_REQUIRE((dst_type == CH_F32 || dst_type == CH_F64) &&
+                   (src_type == CH_F32 || src_type == CH_F64),
+               "ch_convert: types must be CH_F32 or CH_F64");
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 8192)
+        blocks = 8192;
+    const hipStream_t s = (hipStream_t)stream;
+    const int acc = accumulate ? 1 : 0;
+    const dim3 g((unsigned)blocks), b(256);
+    if (dst_type == CH_F64 && src_type == CH_F32)
+        hipLaunchKernelGGL((convert_kernel<double, float>), g, b, 0, s,
+                           (double*)d_dst, (const float*)d_src, n, acc);
+    else if (dst_type == CH_F32 && src_type == CH_F64)
+        hipLaunchKernelGGL((convert_kernel<float, double>), g, b, 0, s,
+                           (float*)d_dst, (const double*)d_src, n, acc);
+    else if (dst_type == CH_F32)
+        hipLaunchKernelGGL((convert_kernel<float, float>), g, b, 0, s,
+                           (float*)d_dst, (const float*)d_src, n, acc);
+    else
+        hipLaunchKernelGGL((convert_kernel<double, double>), g, b, 0, s,
+                           (double*)d_dst, (const double*)d_src, n, acc);
     CH_HIP(hipGetLastError());
     return CH_OK;
 }

@@ -131,9 +131,20 @@ def plan_key(shape, voxel_size, cones):
     h = hashlib.sha1()
     h.update(np.asarray(shape, np.int64).tobytes())
     h.update(np.asarray(voxel_size, np.float64).tobytes())
-    for part in cones.key():
-        h.update(part.encode() if isinstance(part, str) else part)
+    h.update(cones_digest(cones))
     return h.hexdigest()
+
+
+def cones_digest(cones):
+    """Content hash of a cone set, computed once per ConeSet (whose arrays
+    are not changed after construction)."""
+    d = getattr(cones, "_digest", None)
+    if d is None:
+        h = hashlib.sha1()
+        for part in cones.key():
+            h.update(part.encode() if isinstance(part, str) else part)
+        d = cones._digest = h.digest()
+    return d
 
 
 def support_box(d_terma, shape, d_box, stream=None):

@@ -52,6 +52,17 @@ ch_status ch_memcpy_d2d(void* d_dst, const void* d_src, size_t bytes,
                         void* stream);
 /* Zero-fills with a kernel (never hipMemset), so it is safe in captures. */
 ch_status ch_zero(void* d_ptr, size_t bytes, void* stream);
+/* Elementwise conversion of n values between FP32 and FP64 grids by a
+ * kernel: d_dst = d_src, or d_dst += d_src with accumulate != 0 (the sum
+ * rounded once in the destination type).  Used by the convolve_c drop-in
+ * to bring its FP64 TERMA to FP32 and add the FP32 result into its FP64
+ * dose on the device (conehead/convolve_c.pyx:17-20 takes and accumulates
+ * float64 arrays). */
+#define CH_F32 0
+#define CH_F64 1
+ch_status ch_convert(void* d_dst, int32_t dst_type, const void* d_src,
+                     int32_t src_type, size_t n, int32_t accumulate,
+                     void* stream);
 ch_status ch_host_alloc(void** h_ptr, size_t bytes);   /* pinned */
 ch_status ch_host_free(void* h_ptr);
 ch_status ch_stream_create(void** stream);

@@ -49,7 +49,10 @@ def test_plan_taps_bitwise(oracle, engine, kern):
         assert np.array_equal(k.view(np.uint64), rk.view(np.uint64))
 
 
-def test_convolve_c_dropin_matches_oracle(oracle, kern):
+@pytest.mark.parametrize("method", ["auto", "fft", "gather"])
+def test_convolve_c_dropin_matches_oracle(oracle, kern, method):
+    """The drop-in in each convolution form (auto takes the FFT form here),
+    accumulating onto a nonzero float64 dose."""
     from conehead_amd.convolve_c import convolve_c
     rng = np.random.default_rng(1)
     shape = (24, 26, 22)
@@ -61,8 +64,35 @@ def test_convolve_c_dropin_matches_oracle(oracle, kern):
     ref = np.full(shape, 0.5)
     oracle.convolve(T, ref, dims, thetas, phis, k8, n_threads=8)
     d = np.full(shape, 0.5)
-    convolve_c(T, d, dims, thetas, phis, k8)
+    convolve_c(T, d, dims, thetas, phis, k8, method=method)
     assert rel_err(d - 0.5, ref - 0.5) < TOL
+    # a second call adds the same dose again (buffers and plans reused)
+    convolve_c(T, d, dims, thetas, phis, k8, method=method)
+    assert rel_err(d - 0.5, 2 * (ref - 0.5)) < TOL
+
+
+def test_convolve_c_dropin_sees_kernel_edits(oracle, kern):
+    """The drop-in keeps cone sets between calls; a kernel edited in place
+    between calls is still read by value."""
+    import copy
+    from conehead_amd.convolve_c import convolve_c
+    rng = np.random.default_rng(3)
+    shape = (16, 18, 14)
+    T = beam_like_terma(shape, rng)
+    thetas = default_thetas(6)
+    k4 = copy.deepcopy(merged_kernel(kern, 4))
+    phis = np.array(k4.keys(), float)
+    dims = np.array([0.5, 0.5, 0.5])
+    d1 = np.zeros(shape)
+    convolve_c(T, d1, dims, thetas, phis, k4)
+    key = "{:.2f}".format(phis[0])
+    k4.cumulative[key] = np.asarray(k4.cumulative[key]) * 2.0
+    ref = np.zeros(shape)
+    oracle.convolve(T, ref, dims, thetas, phis, k4, n_threads=8)
+    d2 = np.zeros(shape)
+    convolve_c(T, d2, dims, thetas, phis, k4)
+    assert rel_err(d2, ref) < TOL
+    assert not np.allclose(d1, d2)
 
 
 def test_raw_48_bins_and_strided_dose(oracle, kern):
