@@ -785,7 +785,7 @@ def run_serve(args, eng, n_req=64, cases_reqs=None, check=None):
         if check is not None and lam == 500:
             reqs[0] = pool[check[0]]
         arr = poisson_arrivals(float(lam), n_req, seed=99 + lam)
-        with DoseServer(be, max_inflight=2, max_batch_jobs=64) as srv:
+        with DoseServer(be, max_inflight=3, max_batch_jobs=16) as srv:
             futs = replay(srv, reqs, arr)
             res = [f.result(timeout=300) for f in futs]
         if check is not None and lam == 500:
@@ -806,7 +806,7 @@ def run_serve(args, eng, n_req=64, cases_reqs=None, check=None):
     for download in (True, False):
         reqs = [pool[int(i)] for i in rng.integers(0, len(pool), n_sat)]
         arr = poisson_arrivals(float(lam), n_sat, seed=99 + lam)
-        with DoseServer(be, max_inflight=2, max_batch_jobs=64,
+        with DoseServer(be, max_inflight=3, max_batch_jobs=16,
                         download=download) as srv:
             futs = replay(srv, reqs, arr)
             for f in futs:
@@ -835,7 +835,8 @@ def run_serve(args, eng, n_req=64, cases_reqs=None, check=None):
             "config": {"workload": "serving: Poisson arrivals (seed 99), "
                                    "%d requests per lambda drawn from the "
                                    "uneven batch, resident densities, HBM "
-                                   "budget admission, 2 stream slots" %
+                                   "budget admission, 3 stream slots of "
+                                   "up to 16 jobs" %
                                    n_req,
                        "method": method},
             "sweep": sweep, "saturation": saturation}
