@@ -21,6 +21,9 @@ COMMON = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument",
+    # a non-void function that falls off its end is undefined behaviour the
+    # device compiler turns into unreachable code: refuse to build it
+    "-Werror=return-type",
 ]
 # Files whose arithmetic must round every operation like the FP64 oracle.
 EXACT = {"dda.hip", "plan.hip", "terma.hip", "raster.hip", "scatter.hip",

@@ -117,7 +117,7 @@ ch_status ch_dda_3d(const double h_dir[3], const int32_t h_shape[3],
     if (e == hipSuccess && ncopy > 0)
         e = hipMemcpy(h_vox_out, d_vox, (size_t)ncopy * 3 * sizeof(int32_t),
                       hipMemcpyDeviceToHost);
-    hipFree(buf);
+    (void)hipFree(buf);
     if (e != hipSuccess)
         return hip_fail(e, "ch_dda_3d", __FILE__, __LINE__);
     *n_out = n;

@@ -117,17 +117,17 @@ void plan_release(ch_plan* p)
 {
     if (!p)
         return;
-    hipFree(p->d_lin);
-    hipFree(p->d_k);
-    hipFree(p->d_k64);
-    hipFree(p->d_off);
-    hipFree(p->d_lin2);
-    hipFree(p->d_ftab);
-    hipFree(p->d_ftT);
-    hipFree(p->d_meta);
-    hipFree(p->d_cum);
-    hipFree(p->d_perm);
-    hipFree(p->d_row_begin);
+    (void)hipFree(p->d_lin);
+    (void)hipFree(p->d_k);
+    (void)hipFree(p->d_k64);
+    (void)hipFree(p->d_off);
+    (void)hipFree(p->d_lin2);
+    (void)hipFree(p->d_ftab);
+    (void)hipFree(p->d_ftT);
+    (void)hipFree(p->d_meta);
+    (void)hipFree(p->d_cum);
+    (void)hipFree(p->d_perm);
+    (void)hipFree(p->d_row_begin);
     spectrum_free(p);
     delete p;
 }
@@ -352,9 +352,9 @@ ch_status ch_plan_create(const int32_t h_shape[3], const double h_vsize[3],
         }
     }
 #undef CH_TRY
-    hipFree(d_dirs);
-    hipFree(d_rows);
-    hipFree(d_cum);
+    (void)hipFree(d_dirs);
+    (void)hipFree(d_rows);
+    (void)hipFree(d_cum);
     if (e != hipSuccess) {
         plan_release(p);
         return hip_fail(e, "ch_plan_create", __FILE__, __LINE__);

@@ -1363,9 +1363,9 @@ static void convset_release(ch_convset* set)
 
 void spectrum_free(ch_plan* p)
 {
-    hipFree(p->d_khat);
+    (void)hipFree(p->d_khat);
     for (int a = 0; a < 3; ++a)
-        hipFree(p->d_tw[a]);
+        (void)hipFree(p->d_tw[a]);
     p->d_khat = nullptr;
     p->d_tw[0] = p->d_tw[1] = p->d_tw[2] = nullptr;
 }
@@ -1504,15 +1504,15 @@ ch_status ch_plan_spectrum(ch_plan* plan, void* stream)
             st = CH_ERR_HIP;
         }
     }
-    hipFree(frame);
-    hipFree(d_idx);
-    hipFree(d_val);
+    (void)hipFree(frame);
+    (void)hipFree(d_idx);
+    (void)hipFree(d_val);
     if (st == CH_OK) {
         plan->d_khat = khat;
     } else {
-        hipFree(khat);
+        (void)hipFree(khat);
         for (int a = 0; a < 3; ++a)
-            hipFree(tw[a]);
+            (void)hipFree(tw[a]);
         plan->d_tw[0] = plan->d_tw[1] = plan->d_tw[2] = nullptr;
         plan->fft_n[0] = plan->fft_n[1] = plan->fft_n[2] = 0;
     }
