@@ -63,6 +63,7 @@ sys.path.insert(0, ROOT)
 METRIC = ("beams/sec + voxel·cone updates/s at ≤0.1% Dmax err; "
           "p50/p99 latency")
 HBM_PEAK_GBS = 8000.0
+MI355X_CLOCK_GHZ = 2.4      # peak engine clock
 BYTES_PER_VOXEL_CONE = 12
 
 
@@ -108,6 +109,35 @@ def traffic_from_profile(name="gather_hetero_pmc.md"):
         return float(m.group(1)) if m else None
     except OSError:
         return None
+
+
+def count_from_profile(name, key):
+    """A figure 'key: N' from a committed profile summary, or None."""
+    path = os.path.join(ROOT, "profiles", name)
+    try:
+        with open(path) as f:
+            m = re.search(r"%s:\s*([0-9.eE+]+)" % re.escape(key), f.read())
+        return float(m.group(1)) if m else None
+    except OSError:
+        return None
+
+
+def valu_issue(kernel_ms, profile, key):
+    """VALU issue time of a launch from its profiled wave64 VALU
+    instruction count: one instruction every 4 cycles on each of the
+    device's SIMDs (4 per CU) at the peak engine clock."""
+    n = count_from_profile(profile, key)
+    if n is None or not kernel_ms:
+        return None
+    from conehead_amd import _lib
+    simds = 4 * (_lib.device_info().get("n_cu") or 256)
+    clock_ghz = MI355X_CLOCK_GHZ
+    issue_ms = n * 4 / (simds * clock_ghz * 1e9) * 1e3
+    return {"valu_instructions": n, "issue_ms": issue_ms,
+            "frac": issue_ms / kernel_ms,
+            "what": "wave64 VALU instructions per launch (profiles/%s) at "
+                    "one per 4 cycles per SIMD, %d SIMDs, %.2f GHz, over "
+                    "the launch time" % (profile, simds, clock_ghz)}
 
 
 def cpu_threads(args):
@@ -341,7 +371,11 @@ def run_hetero(args, eng):
                  "achieved": ta, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                  "frac": ta / HBM_PEAK_GBS, "algorithmic_bytes": terma_bytes,
                  "kernel_ms": t_ms,
-                 "traffic": traffic_from_profile("terma_hetero_pmc.md")}
+                 "traffic": traffic_from_profile("terma_hetero_pmc.md"),
+                 # what bounds it: the FP64 depth walks' instruction issue
+                 "valu_issue": valu_issue(
+                     t_ms, "terma_conv_hetero_sq_counters.md",
+                     "terma_valu_instructions_per_launch")}
         r["rooflines"] = {"terma": terma, "convolution": conv}
         # the step's dominant kernel (the longer of the two)
         r["roofline"] = terma if t_ms >= k_ms else conv
