@@ -56,6 +56,7 @@ struct TermaParams {
     double n3;
     double blocked_one;
     float inv_n3, inv_ns;
+    float src_vox[3];          // (src - origin) / spacing, for cost classes
 };
 
 __device__ __forceinline__ double dot3(const double* a, double b0, double b1,
@@ -304,11 +305,14 @@ __device__ __forceinline__ void wave_lds_sync()
 // of conehead/conehead.py:138-166 (the traversal of dda_3d_c.pyx:76-99,
 // the same t-values, voxels and tie rules as ch_dda_3d), each voxel's
 // density times its path length summed in FP32 with Kahan compensation, in
-// path order (conehead.py:249-254).  The step needs no branch: the tie
+// path order (conehead.py:249-254).  The step needs no branch.  The tie
 // rules (x if t0 < t1 and t0 < t2, else y if t1 <= t0 and t1 < t2, else z)
-// become three masks that select the crossing taken and the axis that
-// advances.  kWalkBatch steps are traversed first and their density loads
-// issued together before the first is used; lanes past the grid exit keep
+// take the smallest crossing t = min(t0, t1, t2) and, among the axes that
+// reach it, z before y before x: z when t2 == t, else y when t1 == t, else
+// x.  The axis taken advances by t_a = fma(dt_a, 1, t_a), the others by
+// fma(dt_a, 0, t_a) = t_a, both rounded as t_a + dt_a and t_a are.
+// kWalkBatch steps are traversed first and their density loads issued
+// together before the first is used; lanes past the grid exit keep
 // stepping with their sums frozen, their range-checked loads reading 0
 // wherever they land.  Byte offsets are 32-bit (ch_terma requires
 // 4 X Y Z < 2^32).
@@ -324,8 +328,9 @@ __device__ __forceinline__ float depth_walk(const float* __restrict__ rho,
     int32_t left0 = d.s0 > 0 ? X - 1 - x : x;
     int32_t left1 = d.s1 > 0 ? Y - 1 - y : y;
     int32_t left2 = d.s2 > 0 ? Z - 1 - z : z;
-    const int32_t st0 = d.s0 * Y * Z, st1 = d.s1 * Z, st2 = d.s2;
-    int32_t lin = (x * Y + y) * Z + z;
+    // byte strides and offset
+    const int32_t st0 = 4 * d.s0 * Y * Z, st1 = 4 * d.s1 * Z, st2 = 4 * d.s2;
+    uint32_t off = 4u * (uint32_t)((x * Y + y) * Z + z);
     double tprev = 0.0;
     float de = 0.0f, dc = 0.0f;
     bool live = true;
@@ -333,25 +338,23 @@ __device__ __forceinline__ float depth_walk(const float* __restrict__ rho,
     const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
         (void*)rho, (short)0, 4 * X * Y * Z, kBufferFlags);
     while (live) {
-        uint32_t off[kWalkBatch];
+        uint32_t offs[kWalkBatch];
         float seg[kWalkBatch];
         bool use[kWalkBatch];
 #pragma unroll
         for (int k = 0; k < kWalkBatch; ++k) {
             use[k] = live;
-            const bool c01 = t0 < t1, c02 = t0 < t2, c12 = t1 < t2;
-            const bool m0 = c01 & c02;
-            const bool m1 = !c01 & c12;
-            const bool m2 = !(m0 | m1);
-            const double t = m0 ? t0 : (m1 ? t1 : t2);
-            off[k] = 4u * (uint32_t)lin;
+            const double t = fmin(fmin(t0, t1), t2);
+            const bool m2 = t2 == t;
+            const bool m1 = !m2 & (t1 == t);
+            const bool m0 = !(m1 | m2);
+            offs[k] = off;
             seg[k] = (float)(t - tprev);
             tprev = t;
-            const double n0 = t0 + dt0, n1 = t1 + dt1, n2 = t2 + dt2;
-            t0 = m0 ? n0 : t0;
-            t1 = m1 ? n1 : t1;
-            t2 = m2 ? n2 : t2;
-            lin += m0 ? st0 : (m1 ? st1 : st2);
+            t0 = fma(dt0, m0 ? 1.0 : 0.0, t0);
+            t1 = fma(dt1, m1 ? 1.0 : 0.0, t1);
+            t2 = fma(dt2, m2 ? 1.0 : 0.0, t2);
+            off += (uint32_t)(m0 ? st0 : (m1 ? st1 : st2));
             left0 -= (int32_t)m0;
             left1 -= (int32_t)m1;
             left2 -= (int32_t)m2;
@@ -361,7 +364,7 @@ __device__ __forceinline__ float depth_walk(const float* __restrict__ rho,
 #pragma unroll
         for (int k = 0; k < kWalkBatch; ++k)
             r[k] = __int_as_float(__builtin_amdgcn_raw_buffer_load_b32(
-                rsrc, (int)off[k], 0, 0));
+                rsrc, (int)offs[k], 0, 0));
 #pragma unroll
         for (int k = 0; k < kWalkBatch; ++k) {
             // a step past the exit adds dc: yk = 0, de keeps its bits (dc
@@ -375,7 +378,6 @@ __device__ __forceinline__ float depth_walk(const float* __restrict__ rho,
     }
     return de;
 }
-
 // Sub-sample values staged per wavefront by the compacted hit test.
 constexpr int kSampleBuf = 512;
 
@@ -442,9 +444,10 @@ __device__ __forceinline__ float brick_value(const TermaParams& P,
 // toward the source), plus 2 n^3 steps for a brick on the aperture's edge
 // (its sub-samples cost about that much).
 constexpr int kCostClasses = 16;
-// Each class is split over kShards counters and sub-lists (brick b in shard
-// b % kShards), so the appending atomics of phase 1 spread over
-// kCostClasses x kShards addresses instead of queueing on a few.
+// Each class is split over kShards counters and sub-lists (the bricks of
+// phase-1 workgroup w in shard w % kShards), so the appending atomics of
+// phase 1 spread over kCostClasses x kShards addresses instead of queueing
+// on a few.
 constexpr int kShards = 16;
 constexpr int kCounters = kCostClasses * kShards;   // 4 per lane
 
@@ -455,7 +458,7 @@ __device__ __forceinline__ int brick_cost_class(const TermaParams& P,
     const int n[3] = {P.X, P.Y, P.Z};
     float d[3], t = 3.0e38f;
     for (int a = 0; a < 3; ++a) {
-        d[a] = (float)((P.src[a] - P.origin[a]) / P.spacing[a]) - c[a];
+        d[a] = P.src_vox[a] - c[a];
         if (d[a] > 0.0f)
             t = fminf(t, ((float)n[a] - 0.5f - c[a]) / d[a]);
         else if (d[a] < 0.0f)
@@ -471,18 +474,14 @@ __device__ __forceinline__ int brick_cost_class(const TermaParams& P,
     return q < 0 ? 0 : (q >= kCostClasses ? kCostClasses - 1 : q);
 }
 
-// Capacity of one shard's list: the bricks b with b % kShards == shard.
-__host__ __device__ __forceinline__ int shard_cap(int n_bricks)
-{
-    return (n_bricks + kShards - 1) / kShards;
-}
-
 // Phase 1: stage 1 of the hit test for eight bricks per wavefront, the
 // 2 x 2 x 2 bricks of one super-brick, one group of 8 lanes each (the cell
-// box reduction stays within each group).  A brick that reads 0
-// everywhere gets T = 0 here; every other brick (all of them when the
-// debug planes are asked for) is appended to its cost class for phase 2,
-// with its stage-1 result.
+// box reduction stays within each group), four super-bricks per workgroup.
+// A brick that reads 0 everywhere gets T = 0 here; every other brick (all
+// of them when the debug planes are asked for) is appended to its cost
+// class for phase 2, with its stage-1 result: the workgroup ranks its
+// bricks per class in LDS, then reserves their slots in its shard's lists
+// with one device atomic per class.
 struct SuperGrid {
     int nbx, nby, nbz;   // bricks per axis
     int sby, sbz;        // super-bricks per axis (y, z)
@@ -502,77 +501,96 @@ __host__ __device__ __forceinline__ int32_t super_bricks(int X, int Y, int Z)
     return (((X + 3) / 4 + 1) / 2) * (((Y + 3) / 4 + 1) / 2) *
            (((Z + 3) / 4 + 1) / 2);
 }
-
-// Brick index of child c (0..7) of super-brick sb, or -1 outside the grid.
-__device__ __forceinline__ int child_brick(const SuperGrid& g, int sb, int c)
+constexpr int kSortSupers = 4;                   // super-bricks per workgroup
+constexpr int kSortBricks = 8 * kSortSupers;     // bricks per workgroup
+__host__ __device__ __forceinline__ int32_t sort_blocks(int X, int Y, int Z)
 {
-    const int sz = sb % g.sbz, sy = (sb / g.sbz) % g.sby,
-              sx = sb / g.sbz / g.sby;
-    const int bx = 2 * sx + (c >> 2), by = 2 * sy + ((c >> 1) & 1),
-              bz = 2 * sz + (c & 1);
-    if (bx >= g.nbx || by >= g.nby || bz >= g.nbz)
-        return -1;
-    return (bx * g.nby + by) * g.nbz + bz;
+    return (super_bricks(X, Y, Z) + kSortSupers - 1) / kSortSupers;
+}
+// Capacity of one shard's list of one class: kSortBricks per workgroup of
+// the shard.
+__host__ __device__ __forceinline__ int32_t shard_cap(int X, int Y, int Z)
+{
+    return (sort_blocks(X, Y, Z) + kShards - 1) / kShards * kSortBricks;
 }
 
-__device__ __forceinline__ void sort_super(const TermaParams& P, int sb,
-                                           int l, int32_t n_bricks)
+__device__ __forceinline__ void sort_block(const TermaParams& P, int blk,
+                                           int32_t* s_cnt, int32_t* s_base)
 {
+    const int t = threadIdx.x, l = t & 63;
+    if (t < kCostClasses)
+        s_cnt[t] = 0;
+    __syncthreads();
     const SuperGrid g = super_grid(P);
-    const int child = l >> 3;
-    const int b = child_brick(g, sb, child);
-    // every lane takes part in the group reductions (a missing child tests
-    // brick 0 and its result is dropped)
-    const Brick k = brick_of(P, b < 0 ? 0 : b);
-    const float v = brick_value(P, k, l);
-    const bool zero = b >= 0 && v == 0.0f && P.debug == nullptr;
-    // bit 8 c set when child c reads 0: lane l writes voxel l of each
-    const uint64_t zmask = __ballot(zero) & 0x0101010101010101ull;
-    if (zmask != 0 && !P.accumulate) {
-        const int dx = l >> 4, dy = (l >> 2) & 3, dz = l & 3;
+    const int sb = blk * kSortSupers + (t >> 6);
+    int q = -1, rank = 0, entry = 0;
+    if (sb < super_bricks(P.X, P.Y, P.Z)) {  // wavefront-uniform
+        const int sxy = sb / g.sbz, sz = sb - sxy * g.sbz;
+        const int sx = sxy / g.sby, sy = sxy - sx * g.sby;
+        // lane l tests child l >> 3; every lane takes part in the group
+        // reductions (a child past the grid tests its clamped neighbour and
+        // its result is dropped)
+        const int c = l >> 3;
+        const int bx = 2 * sx + (c >> 2), by = 2 * sy + ((c >> 1) & 1),
+                  bz = 2 * sz + (c & 1);
+        const bool in = bx < g.nbx && by < g.nby && bz < g.nbz;
+        Brick k;
+        k.x0 = 4 * min(bx, g.nbx - 1);
+        k.y0 = 4 * min(by, g.nby - 1);
+        k.z0 = 4 * min(bz, g.nbz - 1);
+        const float v = brick_value(P, k, l);
+        const bool zero = in && v == 0.0f && P.debug == nullptr;
+        // bit 8 c set when child c reads 0: lane l writes voxel l of each
+        const uint64_t zmask = __ballot(zero) & 0x0101010101010101ull;
+        if (zmask != 0 && !P.accumulate) {
+            const int dx = l >> 4, dy = (l >> 2) & 3, dz = l & 3;
 #pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            if (!((zmask >> (8 * c)) & 1))
-                continue;  // wavefront-uniform
-            const Brick kc = brick_of(P, child_brick(g, sb, c));
-            const int x = kc.x0 + dx, y = kc.y0 + dy, z = kc.z0 + dz;
-            if (x < P.X && y < P.Y && z < P.Z)
-                P.terma[((int64_t)x * P.Y + y) * P.Z + z] = 0.0f;
+            for (int cc = 0; cc < 8; ++cc) {
+                if (!((zmask >> (8 * cc)) & 1))
+                    continue;  // wavefront-uniform
+                const int x = 4 * (2 * sx + (cc >> 2)) + dx,
+                          y = 4 * (2 * sy + ((cc >> 1) & 1)) + dy,
+                          z = 4 * (2 * sz + (cc & 1)) + dz;
+                if (x < P.X && y < P.Y && z < P.Z)
+                    P.terma[((int64_t)x * P.Y + y) * P.Z + z] = 0.0f;
+            }
+        }
+        if (in && !zero && (l & 7) == 0) {
+            q = brick_cost_class(P, k, v < 0.0f);
+            // entry: brick index and stage-1 value (0 -> 0, 1 -> 1, -1 -> 2)
+            const int b = (bx * g.nby + by) * g.nbz + bz;
+            entry = b * 4 + (v > 0.0f ? 1 : (v < 0.0f ? 2 : 0));
+            rank = atomicAdd(&s_cnt[q], 1);
         }
     }
-    if (b >= 0 && !zero && (l & 7) == 0) {
-        const int q = brick_cost_class(P, k, v < 0.0f);
-        const int c = q * kShards + b % kShards;
-        const int at = atomicAdd(&P.counts[c], 1);
-        // entry: brick index and stage-1 value (0 -> 0, 1 -> 1, -1 -> 2)
-        P.list[(int64_t)c * shard_cap(n_bricks) + at] =
-            b * 4 + (v > 0.0f ? 1 : (v < 0.0f ? 2 : 0));
+    __syncthreads();
+    const int shard = blk % kShards;
+    if (t < kCostClasses) {
+        const int n = s_cnt[t];
+        s_base[t] = n > 0 ? atomicAdd(&P.counts[t * kShards + shard], n) : 0;
     }
+    __syncthreads();
+    if (q >= 0)
+        P.list[(int64_t)(q * kShards + shard) * shard_cap(P.X, P.Y, P.Z) +
+               s_base[q] + rank] = entry;
 }
 
-__global__ __launch_bounds__(256) void terma_sort_kernel(TermaParams P,
-                                                         int32_t n_bricks)
+__global__ __launch_bounds__(256) void terma_sort_kernel(TermaParams P)
 {
-    const int sb = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (sb >= super_bricks(P.X, P.Y, P.Z))
-        return;  // wavefront-uniform
-    sort_super(P, sb, threadIdx.x & 63, n_bricks);
+    __shared__ int32_t s_cnt[kCostClasses], s_base[kCostClasses];
+    sort_block(P, blockIdx.x, s_cnt, s_base);
 }
 
 // Phase 1 of several beams on one grid shape in one launch: the blocks of
 // beam i are i * blocks_per_beam ... , its parameters tab[i] (a device
 // table written before the launch and not written by it).
 __global__ __launch_bounds__(256) void terma_sort_set_kernel(
-    const TermaParams* __restrict__ tab, int32_t n_bricks,
-    int32_t blocks_per_beam)
+    const TermaParams* __restrict__ tab, int32_t blocks_per_beam)
 {
+    __shared__ int32_t s_cnt[kCostClasses], s_base[kCostClasses];
     const int beam = blockIdx.x / blocks_per_beam;
-    const TermaParams& P = tab[beam];
-    const int sb = (blockIdx.x - beam * blocks_per_beam) * 4 +
-                   (threadIdx.x >> 6);
-    if (sb >= super_bricks(P.X, P.Y, P.Z))
-        return;  // wavefront-uniform
-    sort_super(P, sb, threadIdx.x & 63, n_bricks);
+    sort_block(tab[beam], blockIdx.x - beam * blocks_per_beam, s_cnt,
+               s_base);
 }
 
 // TERMA of one brick (one wavefront, every lane calls it) whose stage-1 hit
@@ -808,7 +826,7 @@ __device__ __forceinline__ void terma_wave(const TermaParams& P, int g,
         rank = __shfl(rank, owner, 64);
         const int q = kCostClasses - 1 - pick / kShards;
         const int c = q * kShards + pick % kShards;
-        const int e = list[(int64_t)c * shard_cap(n_bricks) + rank];
+        const int e = list[(int64_t)c * shard_cap(P.X, P.Y, P.Z) + rank];
         const Brick k = brick_of(P, e >> 2);
         const float v = (e & 3) == 1 ? 1.0f : ((e & 3) == 2 ? -1.0f : 0.0f);
         bool bad = false;
@@ -937,6 +955,8 @@ static TermaParams make_params(const ch_beam& B, const int32_t* h_shape,
     P.blocked_one = one;
     P.inv_n3 = 1.0f / (float)(ns * ns * ns);
     P.inv_ns = 1.0f / (float)ns;
+    for (int a = 0; a < 3; ++a)
+        P.src_vox[a] = (float)((P.src[a] - P.origin[a]) / P.spacing[a]);
     return P;
 }
 
@@ -950,13 +970,13 @@ inline int32_t terma_bricks(const int32_t* shape)
     return ((shape[0] + 3) / 4) * ((shape[1] + 3) / 4) *
            ((shape[2] + 3) / 4);
 }
-inline size_t terma_list_ints(int32_t n_bricks)
+inline size_t terma_list_ints(const int32_t* shape)
 {
-    return (size_t)kCounters * shard_cap(n_bricks);
+    return (size_t)kCounters * shard_cap(shape[0], shape[1], shape[2]);
 }
-inline size_t terma_list_bytes(int32_t n_beams, int32_t n_bricks)
+inline size_t terma_list_bytes(int32_t n_beams, const int32_t* shape)
 {
-    return (size_t)n_beams * (kCounters + terma_list_ints(n_bricks)) *
+    return (size_t)n_beams * (kCounters + terma_list_ints(shape)) *
            sizeof(int32_t);
 }
 inline size_t terma_box_ints(int32_t n_bricks)
@@ -971,11 +991,13 @@ inline size_t terma_box_bytes(int32_t n_beams, int32_t n_bricks)
 // Points beam i of a launch of n beams at its part of the scratch (lists:
 // counters then lists; boxes: partial boxes, null without any support box).
 static void terma_assign(TermaParams& P, const ch_beam& B, int i, int n,
-                         int32_t n_bricks, int32_t* lists, int32_t* boxes)
+                         const int32_t* shape, int32_t* lists,
+                         int32_t* boxes)
 {
+    const int32_t n_bricks = terma_bricks(shape);
     P.counts = lists + (size_t)i * kCounters;
     P.list = lists + (size_t)n * kCounters +
-             (size_t)i * terma_list_ints(n_bricks);
+             (size_t)i * terma_list_ints(shape);
     P.box = B.d_box && boxes ? boxes + (size_t)i * terma_box_ints(n_bricks)
                              : nullptr;
 }
@@ -988,10 +1010,10 @@ static void terma_assign(TermaParams& P, const ch_beam& B, int i, int n,
 constexpr int kSetBeams = 64;
 constexpr size_t kSetScratchBytes = (size_t)512 << 20;
 static std::vector<int32_t> terma_groups(int32_t n, const ch_beam* beams,
-                                         int32_t n_bricks)
+                                         const int32_t* shape)
 {
-    const size_t per = terma_list_bytes(1, n_bricks) +
-                       terma_box_bytes(1, n_bricks);
+    const size_t per = terma_list_bytes(1, shape) +
+                       terma_box_bytes(1, terma_bricks(shape));
     const int32_t cap = (int32_t)std::max<size_t>(
         1, std::min<size_t>(kSetBeams, kSetScratchBytes / per));
     std::vector<int32_t> starts{0};
@@ -1059,20 +1081,19 @@ static ch_status terma_launch(const TermaParams* h, const TermaParams* d_tab,
                               hipEvent_t ev1, hipStream_t s)
 {
     const unsigned bpb = (unsigned)((n_bricks + 3) / 4);
-    const unsigned spb =
-        (unsigned)((super_bricks(h[0].X, h[0].Y, h[0].Z) + 3) / 4);
+    const unsigned spb = (unsigned)sort_blocks(h[0].X, h[0].Y, h[0].Z);
     const int nc = n * kCounters;
     hipLaunchKernelGGL(terma_clear_kernel, dim3((unsigned)(nc + 255) / 256),
                        dim3(256), 0, s, h[0].counts, nc);
     if (!d_tab) {
         hipExtLaunchKernelGGL(terma_sort_kernel, dim3(spb), dim3(256), 0, s,
-                              ev0, nullptr, 0, h[0], n_bricks);
+                              ev0, nullptr, 0, h[0]);
         hipExtLaunchKernelGGL(terma_kernel, dim3(bpb), dim3(256), 0, s,
                               nullptr, ev1, 0, h[0], n_bricks);
     } else {
         hipExtLaunchKernelGGL(terma_sort_set_kernel, dim3(spb * n),
                               dim3(256), 0, s, ev0, nullptr, 0, d_tab,
-                              n_bricks, (int32_t)spb);
+                              (int32_t)spb);
         hipExtLaunchKernelGGL(terma_set_kernel, dim3(bpb * n), dim3(256), 0,
                               s, nullptr, ev1, 0, d_tab, n_bricks,
                               (int32_t)bpb);
@@ -1186,7 +1207,7 @@ static ch_status beamset_params(ch_beamset* set, int32_t n,
         if (st != CH_OK)
             return st;
     }
-    starts = terma_groups(n, beams, set->n_bricks);
+    starts = terma_groups(n, beams, set->shape);
     if (want) {
         CH_REQUIRE(starts == *want,
                    "ch_beamset_update: the beams share their TERMA grids "
@@ -1200,7 +1221,7 @@ static ch_status beamset_params(ch_beamset* set, int32_t n,
     for (size_t g = 0; g + 1 < starts.size(); ++g)
         for (int32_t b = starts[g]; b < starts[g + 1]; ++b)
             terma_assign(hp[b], beams[b], b - starts[g],
-                         starts[g + 1] - starts[g], set->n_bricks,
+                         starts[g + 1] - starts[g], set->shape,
                          set->lists, set->boxes);
     return CH_OK;
 }
@@ -1245,11 +1266,11 @@ ch_status ch_terma(int32_t n_beams, const ch_beam* h_beams,
     const bool cap = capturing(s);
     const int32_t n_bricks = terma_bricks(h_shape);
     const std::vector<int32_t> starts =
-        terma_groups(n_beams, h_beams, n_bricks);
+        terma_groups(n_beams, h_beams, h_shape);
     const int32_t width = cap ? 1 : largest_group(starts);
     void* lists = nullptr;
     st = stream_scratch(s, kScratchTermaList,
-                        terma_list_bytes(width, n_bricks), &lists);
+                        terma_list_bytes(width, h_shape), &lists);
     if (st != CH_OK)
         return st;
     void* boxes = nullptr;
@@ -1270,7 +1291,7 @@ ch_status ch_terma(int32_t n_beams, const ch_beam* h_beams,
         if (ng == 1 || cap) {
             for (int32_t i = 0; i < ng; ++i) {
                 const int32_t b = g0 + i;
-                terma_assign(hp[b], h_beams[b], 0, 1, n_bricks,
+                terma_assign(hp[b], h_beams[b], 0, 1, h_shape,
                              (int32_t*)lists, (int32_t*)boxes);
                 st = terma_launch(&hp[b], nullptr, 1, &h_beams[b], n_bricks,
                                   b == 0 ? ev0 : nullptr,
@@ -1281,7 +1302,7 @@ ch_status ch_terma(int32_t n_beams, const ch_beam* h_beams,
             continue;
         }
         for (int32_t i = 0; i < ng; ++i)
-            terma_assign(hp[g0 + i], h_beams[g0 + i], i, ng, n_bricks,
+            terma_assign(hp[g0 + i], h_beams[g0 + i], i, ng, h_shape,
                          (int32_t*)lists, (int32_t*)boxes);
         void* d_tab = nullptr;
         st = stage_table(s, &hp[g0], (size_t)ng * sizeof(TermaParams),
@@ -1313,7 +1334,7 @@ ch_status ch_terma_noalloc(int32_t n_beams, const ch_beam* h_beams,
         return st;
     const int32_t n_bricks = terma_bricks(h_shape);
     const size_t need =
-        terma_list_bytes(1, n_bricks) + terma_box_bytes(1, n_bricks);
+        terma_list_bytes(1, h_shape) + terma_box_bytes(1, n_bricks);
     CH_REQUIRE(need <= kNoAllocScratchBytes,
                "ch_terma_noalloc: a %d x %d x %d grid needs %zu bytes of "
                "scratch, more than its %zu (use ch_terma)",
@@ -1329,7 +1350,7 @@ ch_status ch_terma_noalloc(int32_t n_beams, const ch_beam* h_beams,
         return st;
     int32_t* lists = (int32_t*)base;
     int32_t* boxes =
-        (int32_t*)((char*)base + terma_list_bytes(1, n_bricks));
+        (int32_t*)((char*)base + terma_list_bytes(1, h_shape));
     hipStream_t s = (hipStream_t)stream;
     // one beam at a time through the one scratch array (no device table)
     for (int32_t b = 0; b < n_beams; ++b) {
@@ -1338,7 +1359,7 @@ ch_status ch_terma_noalloc(int32_t n_beams, const ch_beam* h_beams,
                           d_debug, nullptr, &P);
         if (st != CH_OK)
             return st;
-        terma_assign(P, h_beams[b], 0, 1, n_bricks, lists, boxes);
+        terma_assign(P, h_beams[b], 0, 1, h_shape, lists, boxes);
         st = terma_launch(&P, nullptr, 1, &h_beams[b], n_bricks, nullptr,
                           nullptr, s);
         if (st != CH_OK)
@@ -1375,13 +1396,13 @@ ch_status ch_beamset_create(int32_t n_beams, const ch_beam* h_beams,
     for (int32_t b = 0; b < n_beams; ++b)
         any_box = any_box || h_beams[b].d_box;
     const std::vector<int32_t> starts =
-        terma_groups(n_beams, h_beams, set->n_bricks);
+        terma_groups(n_beams, h_beams, set->shape);
     const int32_t width = largest_group(starts);
     hipError_t e = hipMalloc((void**)&set->d_tab,
                              (size_t)n_beams * sizeof(TermaParams));
     if (e == hipSuccess)
         e = hipMalloc((void**)&set->lists,
-                      terma_list_bytes(width, set->n_bricks));
+                      terma_list_bytes(width, set->shape));
     if (e == hipSuccess && any_box)
         e = hipMalloc((void**)&set->boxes,
                       terma_box_bytes(width, set->n_bricks));
