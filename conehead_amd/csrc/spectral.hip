@@ -603,6 +603,29 @@ __device__ __forceinline__ BlockXY xcd_major_block(bool on)
     return b;
 }
 
+// The forward y pass (an even number of column groups per beam) puts the
+// two halves of a 128-byte line (column groups 2 gp and 2 gp + 1 of one
+// beam and slab) on one XCD, as consecutive blocks there, and deals these
+// pairs to the XCDs round robin: the slabs whose transforms the support
+// flags skip are then spread over all XCDs, where contiguous ranges would
+// leave some XCDs idle.  Block x = group * n_beams + beam.  Other grids
+// take xcd_major_block.
+__device__ __forceinline__ BlockXY paired_block(unsigned n_beams)
+{
+    const unsigned nx = gridDim.x, total = gridDim.x * gridDim.y;
+    if (nx % (2 * n_beams) != 0 || total % (2 * kXcds) != 0)
+        return xcd_major_block(true);
+    const unsigned h = blockIdx.x + nx * blockIdx.y;
+    const unsigned k = h % kXcds, j = h / kXcds;
+    const unsigned q = (j >> 1) * kXcds + k;   // pair (slab, gp, beam)
+    const unsigned pairs = nx / (2 * n_beams);
+    const unsigned beam = q % n_beams, rest = q / n_beams;
+    BlockXY b;
+    b.x = (2 * (rest % pairs) + (j & 1)) * n_beams + beam;
+    b.y = rest / pairs;
+    return b;
+}
+
 // Line pair w of a z-pass workgroup: lines (x, y) and (x, y + 1) of the
 // grid, y even; valid = false past the last pair.
 struct LinePair {
@@ -621,6 +644,48 @@ __device__ __forceinline__ LinePair line_pair(int64_t p, int32_t X,
     return lp;
 }
 
+// Zero support of a beam's work grid, so the forward passes skip what is
+// zero: z_fwd records per line pair whether it holds a nonzero value (and
+// writes the pair's spectra only then); the forward y pass reads those
+// flags, skips a slab whose pairs are all zero (recording it in the slab
+// flags) and reads only the nonzero rows of the others; the convolution
+// pass reads only the rows x of nonzero slabs.  The flags are int32 at
+// `off` (complex points after the beam's work grid): X * yp pair flags
+// (pair p = x * yp + y / 2), then X slab flags at xyp = X * yp.  off < 0:
+// no flags (every row is read).
+struct Support {
+    int64_t off;
+    int32_t yp, xyp;
+};
+constexpr int kMaxFlagRows = 512;   // pairs per slab and slabs (X) with flags
+
+__host__ __device__ __forceinline__ Support no_support()
+{
+    Support sp;
+    sp.off = -1;
+    sp.yp = sp.xyp = 0;
+    return sp;
+}
+
+// Whether any thread of line pair w (threads tid % W == w) loaded a
+// nonzero value: each wavefront's vote goes to s_vote[wave][w] before the
+// barrier that follows the loads; pair_voted reads it after that barrier.
+template <int W>
+__device__ __forceinline__ void pair_vote(bool nz, int32_t (*s_vote)[16])
+{
+    const uint64_t m = __ballot(nz);
+    const int lane = threadIdx.x & 63;
+    constexpr uint64_t rep = ~0ull / ((1ull << W) - 1);  // lanes w mod W
+    if (lane < W)
+        s_vote[threadIdx.x >> 6][lane] = ((m >> lane) & rep) != 0;
+}
+__device__ __forceinline__ bool pair_voted(const int32_t (*s_vote)[16],
+                                           int w)
+{
+    static_assert(kFftThreads == 256, "four wavefronts vote");
+    return (s_vote[0][w] | s_vote[1][w] | s_vote[2][w] | s_vote[3][w]) != 0;
+}
+
 // Forward z transforms.  Lines (x, y) of T (X x Y x Z, z fastest) are taken
 // in pairs (y even, y + 1) as one complex line T_y + i T_{y+1}, zero-padded
 // to F; after the FFT the two half spectra (k = 0 .. F/2) are separated,
@@ -631,9 +696,10 @@ template <int W>
 __global__ __launch_bounds__(kFftThreads) void z_fwd_kernel(
     BeamIO io, float2* __restrict__ A, int32_t X, int32_t Y, int32_t Z,
     int32_t a_rows, int32_t kp, FftLen L, const float2* __restrict__ g_tw,
-    int32_t inplace)
+    int32_t inplace, Support sp)
 {
     extern __shared__ float2 sm[];
+    __shared__ int32_t s_vote[kFftThreads / 64][16];
     const int F = L.n;
     float2* s_tw = sm;
     float2* s_a = sm + F;
@@ -645,12 +711,14 @@ __global__ __launch_bounds__(kFftThreads) void z_fwd_kernel(
     const int w = threadIdx.x % W;
     const int n0 = threadIdx.x / W;
     constexpr int kStep = kFftThreads / W;
-    const LinePair lp =
-        line_pair((int64_t)group_of_block(io) * W + w, X, Y);
+    const int64_t pair = (int64_t)group_of_block(io) * W + w;
+    const LinePair lp = line_pair(pair, X, Y);
     const bool two = lp.y + 1 < Y;
     const float* row = T + (lp.x * Y + lp.y) * (int64_t)Z;
     const int K = F / 2 + 1;
     float2* out = A + (lp.x * a_rows + lp.y) * (int64_t)kp;
+    int32_t* flags = sp.off >= 0 ? (int32_t*)(A + sp.off) : nullptr;
+    bool nz = false;
     if (L.nfac >= 2) {
         auto load = [&](int n) -> float2 {
             float2 v = make_float2(0.0f, 0.0f);
@@ -659,10 +727,17 @@ __global__ __launch_bounds__(kFftThreads) void z_fwd_kernel(
                 if (two)
                     v.y = row[Z + n];
             }
+            nz = nz || v.x != 0.0f || v.y != 0.0f;
             return v;
         };
         first_any<false, W>(L.fac[0], load, s_a, F);
+        pair_vote<W>(nz, s_vote);
         __syncthreads();
+        // a pair of zero lines has zero spectra: with flags, only the flag
+        // is written
+        const bool keep = !flags || pair_voted(s_vote, w);
+        if (flags && lp.valid && n0 == 0)
+            flags[pair] = keep ? 1 : 0;
         int Ns = L.fac[0];
         const float2* r = s_a;
         if (inplace)
@@ -670,7 +745,7 @@ __global__ __launch_bounds__(kFftThreads) void z_fwd_kernel(
         else
             r = stages_range<false, W>(s_a, s_b, s_tw, F, L.fac, 1, L.nfac, 1,
                                        Ns);
-        if (!lp.valid)
+        if (!lp.valid || !keep)
             return;
         for (int k = n0; k < K; k += kStep) {
             const float2 c = r[k * W + w];
@@ -693,7 +768,9 @@ __global__ __launch_bounds__(kFftThreads) void z_fwd_kernel(
             if (two)
                 v[i].y = row[Z + n];
         }
+        nz = nz || v[i].x != 0.0f || v[i].y != 0.0f;
     }
+    pair_vote<W>(nz, s_vote);
 #pragma unroll
     for (int i = 0; i < kPerThread; ++i) {
         const int n = n0 + i * kStep;
@@ -701,8 +778,11 @@ __global__ __launch_bounds__(kFftThreads) void z_fwd_kernel(
             s_a[n * W + w] = v[i];
     }
     __syncthreads();
+    const bool keep = !flags || pair_voted(s_vote, w);
+    if (flags && lp.valid && n0 == 0)
+        flags[pair] = keep ? 1 : 0;
     const float2* r = lds_fft<false, W>(s_a, s_b, s_tw, L);
-    if (!lp.valid)
+    if (!lp.valid || !keep)
         return;
     for (int k = n0; k < K; k += kStep) {
         const float2 c = r[k * W + w];
@@ -844,17 +924,20 @@ enum ColMode { kColFwd = 0, kColInv = 1, kColConv = 2 };
 // (XCD-major for the y passes).  Rows n >= n_in read as zero; rows n <
 // n_out are written back in place.  kColConv multiplies by K^ (same
 // layout, its own strides) between the forward and the inverse transform.
-// `inplace`: the host allotted one LDS buffer (single_buffer()).
+// `inplace`: the host allotted one LDS buffer (single_buffer()).  With
+// support flags (sp), the forward y pass (outer = slab x, rows y) and the
+// convolution pass (rows x) read only the rows the flags mark nonzero.
 template <int W, int MODE>
 __global__ __launch_bounds__(kFftThreads) void col_kernel(
     BeamIO io, float2* __restrict__ A, int64_t row_stride,
     int64_t outer_stride, int32_t n_cols, int32_t n_in, int32_t n_out,
     FftLen L, const float2* __restrict__ g_tw,
     const float2* __restrict__ khat, int64_t k_row_stride,
-    int64_t k_outer_stride, int32_t inplace)
+    int64_t k_outer_stride, int32_t inplace, Support sp)
 {
     constexpr int kPer = MODE == kColConv ? kPerThreadConv : kPerThread;
     extern __shared__ float2 sm[];
+    __shared__ int32_t s_rows[MODE == kColInv ? 1 : kMaxFlagRows];
     const int F = L.n;
     float2* s_tw = sm;
     float2* s_a = sm + F;
@@ -862,19 +945,50 @@ __global__ __launch_bounds__(kFftThreads) void col_kernel(
     load_twiddles(s_tw, g_tw, F);
     // (the convolution pass reads whole 128-byte lines and measured no
     // faster in XCD-major order)
-    const BlockXY blk = xcd_major_block(MODE != kColConv);
+    const BlockXY blk = MODE == kColFwd ? paired_block(io.n_beams)
+                                        : xcd_major_block(MODE == kColInv);
     const int beam = (int)(blk.x % (unsigned)io.n_beams);
     const int k0 = (int)(blk.x / (unsigned)io.n_beams) * W;
+    if (k0 >= n_cols)
+        return;  // a group added to make the count even
     const int w = threadIdx.x % W;
     const int n0 = threadIdx.x / W;
     constexpr int kStep = kFftThreads / W;
     const bool col_ok = w < n_cols - k0;
     float2* base = A + beam * io.a_beam + blk.y * outer_stride + k0 + w;
+    // rows marked zero by the support flags (workgroup-uniform branches)
+    const bool flagged = MODE != kColInv && sp.off >= 0;
+    if (flagged) {
+        int32_t* flags = (int32_t*)(A + beam * io.a_beam + sp.off);
+        if (MODE == kColFwd) {
+            // the line pairs of slab blk.y; a slab of zero pairs has zero
+            // y transforms: only its flag is written
+            int any = 0;
+            for (int i = threadIdx.x; i < sp.yp; i += kFftThreads) {
+                const int32_t f = flags[(int64_t)blk.y * sp.yp + i];
+                s_rows[i] = f;
+                any |= f;
+            }
+            any = __syncthreads_or(any);
+            if (threadIdx.x == 0)
+                flags[sp.xyp + blk.y] = any ? 1 : 0;
+            if (!any)
+                return;
+        } else {
+            for (int i = threadIdx.x; i < n_in; i += kFftThreads)
+                s_rows[i] = flags[sp.xyp + i];
+            __syncthreads();
+        }
+    }
+    auto row_nz = [&](int n) -> bool {
+        return !flagged || s_rows[MODE == kColFwd ? n >> 1 : n] != 0;
+    };
     if (L.nfac >= 2) {
         const int nf = L.nfac;
         auto load = [&](int n) -> float2 {
-            return (col_ok && n < n_in) ? base[n * row_stride]
-                                        : make_float2(0.0f, 0.0f);
+            return (col_ok && n < n_in && row_nz(n))
+                       ? base[n * row_stride]
+                       : make_float2(0.0f, 0.0f);
         };
         auto store = [&](int n, float2 v) {
             if (col_ok && n < n_out)
@@ -938,7 +1052,7 @@ __global__ __launch_bounds__(kFftThreads) void col_kernel(
     for (int i = 0; i < kPer; ++i) {
         const int n = n0 + i * kStep;
         v[i] = make_float2(0.0f, 0.0f);
-        if (col_ok && n < n_in)
+        if (col_ok && n < n_in && row_nz(n))
             v[i] = base[n * row_stride];
     }
     // K^ for this thread's elements, loaded while the transform runs
@@ -1059,7 +1173,7 @@ FftLen plan_len(const ch_plan* p, int a)
 template <int W>
 ch_status launch_z_fwd(const BeamIO& io, float2* A, int X, int Y, int Z,
                        int a_rows, int kp, const FftLen& L, const float2* tw,
-                       hipStream_t s)
+                       hipStream_t s, const Support& sp)
 {
     const bool ip = single_buffer(L, W);
     const size_t lds = ip ? (size_t)(L.n + L.n * W) * sizeof(float2)
@@ -1071,7 +1185,7 @@ ch_status launch_z_fwd(const BeamIO& io, float2* A, int X, int Y, int Z,
     hipLaunchKernelGGL(z_fwd_kernel<W>,
                        dim3((unsigned)(groups * io.n_beams)),
                        dim3(kFftThreads), lds, s, io, A, X, Y, Z, a_rows, kp,
-                       L, tw, (int32_t)ip);
+                       L, tw, (int32_t)ip, sp);
     CH_HIP(hipGetLastError());
     return CH_OK;
 }
@@ -1102,7 +1216,7 @@ ch_status launch_col(const BeamIO& io, float2* A, int64_t row_stride,
                      int n_out, const FftLen& L, const float2* tw,
                      const float2* khat, int64_t k_row_stride,
                      int64_t k_outer_stride, hipStream_t s,
-                     LaunchTimer* timer)
+                     LaunchTimer* timer, const Support& sp)
 {
     // one LDS buffer when the fused path runs and every stage gives each
     // thread at most one butterfly group
@@ -1112,18 +1226,22 @@ ch_status launch_col(const BeamIO& io, float2* A, int64_t row_stride,
     ch_status st = allow_lds(col_kernel<W, MODE>, col_lds_bytes(L.n, W));
     if (st != CH_OK)
         return st;
-    const int64_t groups = (n_cols + W - 1) / W;
+    // the forward y pass takes an even number of column groups
+    // (paired_block)
+    int64_t groups = (n_cols + W - 1) / W;
+    if (MODE == kColFwd)
+        groups += groups & 1;
     const dim3 grid((unsigned)(groups * io.n_beams), (unsigned)n_outer);
     if (timer)
         timer->launch(col_kernel<W, MODE>, grid, dim3(kFftThreads),
                       (std::uint32_t)lds, io, A, row_stride, outer_stride,
                       n_cols, n_in, n_out, L, tw, khat, k_row_stride,
-                      k_outer_stride, (int32_t)ip);
+                      k_outer_stride, (int32_t)ip, sp);
     else
         hipLaunchKernelGGL((col_kernel<W, MODE>), grid, dim3(kFftThreads),
                            lds, s, io, A, row_stride, outer_stride, n_cols,
                            n_in, n_out, L, tw, khat, k_row_stride,
-                           k_outer_stride, (int32_t)ip);
+                           k_outer_stride, (int32_t)ip, sp);
     CH_HIP(hipGetLastError());
     return CH_OK;
 }
@@ -1131,17 +1249,17 @@ ch_status launch_col(const BeamIO& io, float2* A, int64_t row_stride,
 // W is chosen from the transform length at run time.
 ch_status z_fwd(const BeamIO& io, float2* A, int X, int Y, int Z, int a_rows,
                 int kp, const FftLen& L, const float2* tw, hipStream_t s,
-                int want)
+                int want, const Support& sp)
 {
     switch (lines_per_group(L.n, kFftLdsElems, want)) {
     case 16:
-        return launch_z_fwd<16>(io, A, X, Y, Z, a_rows, kp, L, tw, s);
+        return launch_z_fwd<16>(io, A, X, Y, Z, a_rows, kp, L, tw, s, sp);
     case 8:
-        return launch_z_fwd<8>(io, A, X, Y, Z, a_rows, kp, L, tw, s);
+        return launch_z_fwd<8>(io, A, X, Y, Z, a_rows, kp, L, tw, s, sp);
     case 4:
-        return launch_z_fwd<4>(io, A, X, Y, Z, a_rows, kp, L, tw, s);
+        return launch_z_fwd<4>(io, A, X, Y, Z, a_rows, kp, L, tw, s, sp);
     default:
-        return launch_z_fwd<2>(io, A, X, Y, Z, a_rows, kp, L, tw, s);
+        return launch_z_fwd<2>(io, A, X, Y, Z, a_rows, kp, L, tw, s, sp);
     }
 }
 
@@ -1167,26 +1285,31 @@ ch_status col(const BeamIO& io, float2* A, int64_t row_stride,
               int n_out, const FftLen& L, const float2* tw,
               const float2* khat, int64_t k_row_stride,
               int64_t k_outer_stride, hipStream_t s, int want,
-              LaunchTimer* timer = nullptr)
+              LaunchTimer* timer = nullptr,
+              const Support& sp = no_support())
 {
     switch (lines_per_group(
         L.n, MODE == kColConv ? kFftLdsElemsConv : kFftLdsElems, want)) {
     case 16:
         return launch_col<16, MODE>(io, A, row_stride, outer_stride, n_outer,
                                     n_cols, n_in, n_out, L, tw, khat,
-                                    k_row_stride, k_outer_stride, s, timer);
+                                    k_row_stride, k_outer_stride, s, timer,
+                                    sp);
     case 8:
         return launch_col<8, MODE>(io, A, row_stride, outer_stride, n_outer,
                                    n_cols, n_in, n_out, L, tw, khat,
-                                   k_row_stride, k_outer_stride, s, timer);
+                                   k_row_stride, k_outer_stride, s, timer,
+                                   sp);
     case 4:
         return launch_col<4, MODE>(io, A, row_stride, outer_stride, n_outer,
                                    n_cols, n_in, n_out, L, tw, khat,
-                                   k_row_stride, k_outer_stride, s, timer);
+                                   k_row_stride, k_outer_stride, s, timer,
+                                   sp);
     default:
         return launch_col<2, MODE>(io, A, row_stride, outer_stride, n_outer,
                                    n_cols, n_in, n_out, L, tw, khat,
-                                   k_row_stride, k_outer_stride, s, timer);
+                                   k_row_stride, k_outer_stride, s, timer,
+                                   sp);
     }
 }
 
@@ -1217,10 +1340,28 @@ int32_t smooth_at_least(int32_t n)
     }
 }
 
-// Complex points of one beam's work grid.
+// The support flags of a beam's work grid (after its complex points), or
+// none when a slab or its line pairs exceed the flag rows of the kernels.
+Support plan_support(const ch_plan* p)
+{
+    const int X = p->shape[0], yp = (p->shape[1] + 1) / 2;
+    if (X > kMaxFlagRows || yp > kMaxFlagRows)
+        return no_support();
+    Support sp;
+    sp.off = (int64_t)X * p->fft_n[1] * p->fft_kp;
+    sp.yp = yp;
+    sp.xyp = X * yp;
+    return sp;
+}
+
+// Complex points of one beam's work grid, its support flags included
+// (a multiple of 32, so every beam's grid starts 256-byte aligned).
 size_t beam_points(const ch_plan* p)
 {
-    return (size_t)p->shape[0] * p->fft_n[1] * p->fft_kp;
+    const size_t grid = (size_t)p->shape[0] * p->fft_n[1] * p->fft_kp;
+    const Support sp = plan_support(p);
+    const size_t flags = sp.off < 0 ? 0 : ((size_t)sp.xyp + p->shape[0] + 1) / 2;
+    return (grid + flags + 31) / 32 * 32;
 }
 
 // The five passes over the beams of `io` (all on plan p), work grids in A.
@@ -1236,19 +1377,21 @@ ch_status spectral_passes(const ch_plan* p, BeamIO io, float2* A,
     io.a_beam = (int64_t)beam_points(p);
     const int64_t slab = (int64_t)Fy * kp;
     const int32_t* W = p->fft_lines;
-    ch_status st = z_fwd(io, A, X, Y, Z, Fy, kp, Lz, p->d_tw[2], s, W[0]);
+    const Support sp = plan_support(p);
+    ch_status st =
+        z_fwd(io, A, X, Y, Z, Fy, kp, Lz, p->d_tw[2], s, W[0], sp);
     if (st == CH_OK)
         st = col<kColFwd>(io, A, kp, slab, X, kz, Y, Fy, Ly, p->d_tw[1],
-                          nullptr, 0, 0, s, W[1]);
+                          nullptr, 0, 0, s, W[1], nullptr, sp);
     if (st == CH_OK) {
         if (time_conv) {
             LaunchTimer timer(s);
             st = col<kColConv>(io, A, slab, kp, Fy, kz, X, X, Lx, p->d_tw[0],
-                               p->d_khat, slab, kp, s, W[2], &timer);
+                               p->d_khat, slab, kp, s, W[2], &timer, sp);
             timer.stop();
         } else {
             st = col<kColConv>(io, A, slab, kp, Fy, kz, X, X, Lx, p->d_tw[0],
-                               p->d_khat, slab, kp, s, W[2]);
+                               p->d_khat, slab, kp, s, W[2], nullptr, sp);
         }
     }
     if (st == CH_OK)
@@ -1491,7 +1634,8 @@ ch_status ch_plan_spectrum(ch_plan* plan, void* stream)
         const int64_t slab = (int64_t)F[1] * kp;
         // the whole frame is the grid here: no zero rows, all rows kept
         const BeamIO io = single_beam(frame, nullptr, 0);
-        st = z_fwd(io, khat, F[0], F[1], F[2], F[1], kp, Lz, tw[2], s, 0);
+        st = z_fwd(io, khat, F[0], F[1], F[2], F[1], kp, Lz, tw[2], s, 0,
+                   no_support());
         if (st == CH_OK)
             st = col<kColFwd>(io, khat, kp, slab, F[0], kz, F[1], F[1], Ly,
                               tw[1], nullptr, 0, 0, s, 0);
