@@ -10,6 +10,8 @@ it through zero-padded FP32 FFTs.  Checks:
     at s (a closed form that a dropped tap, a wrong sign or a transposed
     axis fails);
   - accumulate mode, and radiological plans refused;
+  - the zero-support skips: a sparse T after dense ones on the same work
+    grid, alone and in a batch, and an all-zero T;
   - the direct gather on the same plan.
 Tolerance: max |D - D_ref| / max(D_ref) <= 1e-5 (target 1e-3; FP32 FFTs
 land near 3e-7)."""
@@ -127,6 +129,45 @@ def test_accumulate_and_direct_gather(engine, kern):
     base = rng.random(shape).astype(np.float32)
     both = run_fft(plan, T, accumulate_onto=base)
     assert rel_err(both - base, gather) < TOL
+
+
+def test_zero_support_leaves_no_stale_values(engine, kern):
+    """The forward passes skip zero line pairs and zero slabs of T (support
+    flags behind the work grid) and leave those parts of the work grid
+    unwritten.  A sparse T convolved right after dense ones on the same
+    work grid must give the same bits whatever the dense T was, agree with
+    the gather, and an all-zero T must give a dose of exact zeros."""
+    rng = np.random.default_rng(17)
+    shape, vs = (24, 17, 20), (0.35, 0.3, 0.4)   # odd y: an unpaired line
+    cs = cone_set(kern, 6, 4)
+    plan = engine.plan(shape, vs, cs)
+    dense = (0.5 + rng.random(shape)).astype(np.float32)
+    sparse = np.zeros(shape, np.float32)
+    sparse[3] = 1.0 + rng.random(shape[1:])        # a full slab
+    sparse[11, 4:9] = 1.0 + rng.random((5, shape[2]))
+    sparse[11, 16] = 2.0                            # the unpaired line only
+    sparse[19, 0:2, 7] = 3.0                        # one line pair, one z
+    after_a = (run_fft(plan, dense), run_fft(plan, sparse))[1]
+    after_b = (run_fft(plan, 3.0 * dense), run_fft(plan, sparse))[1]
+    assert np.array_equal(after_a, after_b)
+    d_t = DeviceArray.from_host(sparse)
+    d_box = DeviceArray((6,), np.int32)
+    E.support_box(d_t, shape, d_box)
+    d_g = DeviceArray(shape, np.float32)
+    E.convolve(plan, d_t, d_g, d_box)
+    assert rel_err(after_a, d_g.download()) < TOL
+    # in a batch each job has its own work grid and flags
+    jobs = []
+    for T in (dense, sparse, np.zeros(shape, np.float32), sparse[::-1]):
+        jobs.append((plan, DeviceArray.from_host(np.ascontiguousarray(T)),
+                     DeviceArray(shape, np.float32), None, False))
+    E.convolve_fft_batch(jobs)
+    assert np.array_equal(jobs[1][2].download(), after_a)
+    assert not jobs[2][2].download().any()
+    assert np.array_equal(jobs[3][2].download(),
+                          run_fft(plan, np.ascontiguousarray(sparse[::-1])))
+    run_fft(plan, dense)
+    assert not run_fft(plan, np.zeros(shape, np.float32)).any()
 
 
 def test_radiological_plan_refused(engine, kern):
