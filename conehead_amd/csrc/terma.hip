@@ -394,9 +394,9 @@ constexpr int kSampleBuf = 512;
 // for a 10 x 10 cm field -- go to phase 2 in order of their estimated cost,
 // the longest depth walks first, so they start together and the cheaper
 // ones fill in behind them.
-// Bricks of 4 x 4 x 4 voxels, one per wavefront, indexed z fastest: the
-// corner voxel of brick b, and lane l's voxel (x0 + l / 16, y0 + l / 4 % 4,
-// z0 + l % 4).
+// Bricks of 4 x 4 x 4 voxels (one per wavefront in phase 2), indexed z
+// fastest: the corner voxel of brick b, and lane l's voxel (x0 + l / 16,
+// y0 + l / 4 % 4, z0 + l % 4).
 struct Brick {
     int x0, y0, z0;
 };
