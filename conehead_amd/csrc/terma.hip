@@ -57,6 +57,10 @@ struct TermaParams {
     double blocked_one;
     float inv_n3, inv_ns;
     float src_vox[3];          // (src - origin) / spacing, for cost classes
+    // 100 tr0.src, 100 tr2.src and -100 n.src, for fast_cells, which is
+    // used only when |ci_src|, |cj_src| < 1e5 (fast_ok)
+    double ci_src, cj_src, k100;
+    int32_t fast_ok;
 };
 
 __device__ __forceinline__ double dot3(const double* a, double b0, double b1,
@@ -273,11 +277,11 @@ constexpr int kDppHalfMirror = 0x141;   // row_half_mirror
 // conehead.py:50-68 at cell coordinates (times 100, not floored) of an
 // image: raster cell floor(c) + 2000 - 1, a -1 index wrapping to the last
 // row or column, looked up in the beam's aperture.
-__device__ __forceinline__ float transmission(const TermaParams& P, double ci,
-                                              double cj)
+__device__ __forceinline__ float transmission_at(const TermaParams& P,
+                                                 double fi, double fj)
 {
-    const double px = floor(ci) + 2000.0;
-    const double pz = floor(cj) + 2000.0;
+    const double px = fi + 2000.0;
+    const double pz = fj + 2000.0;
     if (px < 0 || px > 3999 || pz < 0 || pz > 3999)
         return 0.0f;
     int i = (int)px - 1, j = (int)pz - 1;
@@ -290,6 +294,47 @@ __device__ __forceinline__ float transmission(const TermaParams& P, double ci,
     if (P.ap_kind == CH_APERTURE_RECTS)
         return rects_value(P.ap_table, P.ap_n, i, j);
     return P.raster[i * 4000 + j];
+}
+// (fi, fj: the floors of the cell coordinates, all the lookup uses)
+__device__ __forceinline__ float transmission(const TermaParams& P, double ci,
+                                              double cj)
+{
+    return transmission_at(P, floor(ci), floor(cj));
+}
+
+// The floors of a sub-sample's cell coordinates (plane_image's ci, cj)
+// from the same geometry rearranged: with u = source - p,
+//   ci = 100 tr0.src + (-100 n.src) (tr0.u) / (n.u)
+// (cj with tr2), the products fused and the quotient from a reciprocal
+// refined by one Newton step.  For |n.u| >= 1 cm, |ci|, |cj| < 1e6 and
+// |100 tr0.src|, |100 tr2.src| < 1e5 this and plane_image's rounded
+// evaluation are each within 1e-8 cell of the exact value (a few dozen
+// roundings of quantities below 2.2e6 at 2^-53), so when both coordinates
+// lie more than 1e-6 cell from an integer their floors are plane_image's,
+// and |n.u| >= 1e-6 (no geometry flag).  Returns false otherwise:
+// plane_image then decides.
+__device__ __forceinline__ bool fast_cells(const TermaParams& P, double p0,
+                                           double p1, double p2, double& fi,
+                                           double& fj)
+{
+    if (!P.fast_ok)
+        return false;  // uniform over the launch
+    const double u0 = P.src[0] - p0, u1 = P.src[1] - p1,
+                 u2 = P.src[2] - p2;
+    const double d = fma(P.vy[2], u2, fma(P.vy[1], u1, P.vy[0] * u0));
+    const double ni = fma(P.tr0[2], u2, fma(P.tr0[1], u1, P.tr0[0] * u0));
+    const double nj = fma(P.tr2[2], u2, fma(P.tr2[1], u1, P.tr2[0] * u0));
+    double r = __builtin_amdgcn_rcp(d);
+    r = r * fma(-d, r, 2.0);
+    const double s = P.k100 * r;
+    const double ci = fma(s, ni, P.ci_src), cj = fma(s, nj, P.cj_src);
+    fi = floor(ci);
+    fj = floor(cj);
+    const double ei = ci - fi, ej = cj - fj;
+    constexpr double kMargin = 1e-6;
+    return fabs(d) >= 1.0 && fabs(ci) < 1e6 && fabs(cj) < 1e6 &&
+           ei > kMargin && ei < 1.0 - kMargin && ej > kMargin &&
+           ej < 1.0 - kMargin;
 }
 
 // LDS of one wavefront exchanged between its own lanes: the writes are
@@ -669,16 +714,24 @@ __device__ __forceinline__ float terma_brick(const TermaParams& P,
                     const int ix = (int)(((float)q + 0.5f) * inv_ns);
                     const int iy = q - ix * ns;
                     const int src = lanes[g0 + j];
-                    const PlaneImage c = plane_image(
-                        P, sample_pos(P, 0, x0 + (src >> 4), off[0], ix),
-                        sample_pos(P, 1, y0 + ((src >> 2) & 3), off[1], iy),
-                        sample_pos(P, 2, z0 + (src & 3), off[2], iz),
-                        ndotn_src);
+                    const double p0 =
+                        sample_pos(P, 0, x0 + (src >> 4), off[0], ix);
+                    const double p1 =
+                        sample_pos(P, 1, y0 + ((src >> 2) & 3), off[1], iy);
+                    const double p2 =
+                        sample_pos(P, 2, z0 + (src & 3), off[2], iz);
                     double tv = 0.0;  // a sample in the source plane adds 0
-                    if (fabs(c.ndotu) < 1e-6)
-                        bad = true;
-                    else
-                        tv = (double)transmission(P, c.ci, c.cj) / n3;
+                    double fi, fj;
+                    if (fast_cells(P, p0, p1, p2, fi, fj)) {
+                        tv = (double)transmission_at(P, fi, fj) / n3;
+                    } else {
+                        const PlaneImage c =
+                            plane_image(P, p0, p1, p2, ndotn_src);
+                        if (fabs(c.ndotu) < 1e-6)
+                            bad = true;
+                        else
+                            tv = (double)transmission(P, c.ci, c.cj) / n3;
+                    }
                     buf[k] = tv;
                 }
                 wave_lds_sync();
@@ -693,10 +746,17 @@ __device__ __forceinline__ float terma_brick(const TermaParams& P,
             for (int ix = 0; ix < ns; ++ix)
                 for (int iy = 0; iy < ns; ++iy)
                     for (int iz = 0; iz < ns; ++iz) {
-                        const PlaneImage c = plane_image(
-                            P, sample_pos(P, 0, x, off[0], ix),
-                            sample_pos(P, 1, y, off[1], iy),
-                            sample_pos(P, 2, z, off[2], iz), ndotn_src);
+                        const double p0 = sample_pos(P, 0, x, off[0], ix);
+                        const double p1 = sample_pos(P, 1, y, off[1], iy);
+                        const double p2 = sample_pos(P, 2, z, off[2], iz);
+                        double fi, fj;
+                        if (fast_cells(P, p0, p1, p2, fi, fj)) {
+                            blocked = blocked +
+                                      (double)transmission_at(P, fi, fj) / n3;
+                            continue;
+                        }
+                        const PlaneImage c =
+                            plane_image(P, p0, p1, p2, ndotn_src);
                         if (fabs(c.ndotu) < 1e-6) {
                             bad = true;
                             continue;
@@ -957,6 +1017,12 @@ static TermaParams make_params(const ch_beam& B, const int32_t* h_shape,
     P.inv_ns = 1.0f / (float)ns;
     for (int a = 0; a < 3; ++a)
         P.src_vox[a] = (float)((P.src[a] - P.origin[a]) / P.spacing[a]);
+    P.ci_src = 100.0 * (P.tr0[0] * P.src[0] + P.tr0[1] * P.src[1] +
+                        P.tr0[2] * P.src[2]);
+    P.cj_src = 100.0 * (P.tr2[0] * P.src[0] + P.tr2[1] * P.src[1] +
+                        P.tr2[2] * P.src[2]);
+    P.k100 = -100.0 * P.ndotn_src;
+    P.fast_ok = fabs(P.ci_src) < 1e5 && fabs(P.cj_src) < 1e5;
     return P;
 }
 
