@@ -802,7 +802,7 @@ def run_serve(args, eng, n_req=64, cases_reqs=None, check=None):
     dose is compared with the oracle."""
     from conehead_amd.batch import BatchEngine
     from conehead_amd.device import Stream
-    from conehead_amd.serve import DoseServer, poisson_arrivals, replay
+    from conehead_amd.serve import DoseServer, consume, poisson_arrivals
     log("serve: generating phantoms")
     cases, pool = cases_reqs or uneven_requests()
     method = "fft" if args.method == "auto" else args.method
@@ -822,12 +822,14 @@ def run_serve(args, eng, n_req=64, cases_reqs=None, check=None):
         if check is not None and lam == 500:
             reqs[0] = pool[check[0]]
         arr = poisson_arrivals(float(lam), n, seed=99 + lam)
+        first = []
+        # each result is taken as it is done and dropped, as a client would
         with DoseServer(be, max_inflight=3, max_batch_jobs=16) as srv:
-            futs = replay(srv, reqs, arr)
-            res = [f.result(timeout=300) for f in futs]
+            consume(srv, reqs, arr,
+                    take=lambda i, r: first.append(r[0]) if i == 0 else None)
         if check is not None and lam == 500:
-            served_err = err_over_dmax(res[0][0], check[1])
-        del res
+            served_err = err_over_dmax(first[0], check[1])
+        del first
         st = srv.stats()
         st["lambda_req_per_s"] = lam
         sweep.append(st)
@@ -845,11 +847,8 @@ def run_serve(args, eng, n_req=64, cases_reqs=None, check=None):
         arr = poisson_arrivals(float(lam), n_sat, seed=99 + lam)
         with DoseServer(be, max_inflight=3, max_batch_jobs=16,
                         download=download) as srv:
-            futs = replay(srv, reqs, arr)
-            for f in futs:
-                r = f.result(timeout=300)
-                if not download:
-                    be.release([r])
+            consume(srv, reqs, arr,
+                    take=None if download else lambda i, r: be.release([r]))
         st = srv.stats()
         st["lambda_req_per_s"] = lam
         saturation["host" if download else "device_resident"] = st

@@ -213,13 +213,37 @@ def poisson_arrivals(rate_per_s, n, seed=99):
     return np.cumsum(rng.exponential(1.0 / rate_per_s, size=n))
 
 
-def replay(server, requests, arrivals):
-    """Submit requests at the given offsets (s from now); returns futures."""
+def replay(server, requests, arrivals, out=None):
+    """Submit requests at the given offsets (s from now); returns futures.
+    With `out` (a list), each future is appended to it as it is submitted,
+    so another thread can take results while the stream is still arriving
+    (see consume)."""
     t0 = time.perf_counter()
-    futs = []
+    futs = [] if out is None else out
     for req, t in zip(requests, arrivals):
         dt = t0 + t - time.perf_counter()
         if dt > 0:
             time.sleep(dt)
         futs.append(server.submit(req))
     return futs
+
+
+def consume(server, requests, arrivals, take=None, timeout=300):
+    """Replays the stream (in a thread) while this thread takes each result
+    as soon as it is done, in order, hands it to take(i, result) if given
+    and drops it: a consumer that does not hold every result returns the
+    pinned buffers of delivered doses to their pool as it goes, as a
+    serving client would."""
+    out = []
+    th = threading.Thread(target=replay, args=(server, requests, arrivals,
+                                               out), daemon=True)
+    th.start()
+    for i in range(len(requests)):
+        while len(out) <= i:
+            time.sleep(0.0002)
+        r = out[i].result(timeout=timeout)
+        out[i] = None
+        if take is not None:
+            take(i, r)
+        del r
+    th.join()
