@@ -88,6 +88,24 @@ def test_water_square_field(oracle, engine):
             np.ones(shape, np.float32), 0.0, blk)
 
 
+def test_samples_on_cell_boundaries(oracle, engine):
+    """Sub-samples whose plane images fall on raster cell boundaries: with
+    3 mm voxels and 3 sub-samples per axis the samples sit on a 1 mm grid,
+    and the layer at y = 0 (the isocentre plane) images onto multiples of
+    10 cells, the field edges of set_square(10) included.  The fast cell
+    path declines every such sample (within 1e-6 cell of an integer) and
+    the definition's plane image decides: the blocked fraction is still
+    bitwise the oracle's."""
+    shape = (40, 8, 40)
+    origin = np.array([-6.0, 0.1, -6.0])
+    blk = Block()
+    blk.set_square(10)
+    rng = np.random.default_rng(12)
+    dens = (1.0 + 0.2 * rng.standard_normal(shape)).clip(0.1, None)
+    compare(oracle, engine, shape, [0.3] * 3, origin,
+            dens.astype(np.float32), 0.0, blk)
+
+
 def test_heterogeneous_oblique_mlc(oracle, engine):
     n, s = 48, 0.5
     shape = (n, n, n)
