@@ -342,11 +342,16 @@ def run_hetero(args, eng):
         "fft": (4 * nv + hs) + (hs + full) + kernel_bytes["fft"] +
                (full + hs) + (hs + 4 * nv)}
     lines = 16                      # the convolution pass's line group
+    # its stages' half forms: frame at least twice the grid, first and last
+    # radix even (radices 16, 8, 4, 2 first, then 3 and 5: a power of two)
+    half_forms = (2 * shape[0] <= frame[0] and frame[0] > 16
+                  and frame[0] & (frame[0] - 1) == 0)
     while lines > 2 and frame[0] * lines > 4096:
         lines //= 2
     kernel_name = {"gather": "gather_single (direct stencil gather)",
-                   "fft": "col_kernel<%d, 2> (kColConv: x transforms, "
-                          "times K^, inverse x transforms)" % lines}
+                   "fft": "col_kernel<%d, 2, %s> (kColConv: x transforms, "
+                          "times K^, inverse x transforms)"
+                          % (lines, "true" if half_forms else "false")}
     # TERMA: the sort pass and the brick kernel, bracketed by events the
     # library attaches to them; 8 B per voxel (a 4 B density read and a 4 B
     # TERMA write)

@@ -20,7 +20,11 @@
 //   col     inverse y transforms, rows y < Y kept;
 //   z_inv   two half spectra per complex inverse FFT into the dose lines.
 // The transforms are mixed-radix (2, 3, 4, 5, 8, 16) Stockham FFTs in LDS,
-// W lines per workgroup; frame lengths are products of 2, 3 and 5.
+// W lines per workgroup; frame lengths are products of 2, 3 and 5.  Where
+// the frame is at least twice the grid, half of every forward transform's
+// input is padding and half of every inverse transform's output is cropped:
+// the first stage then skips those rows (half_in) and the last stage those
+// outputs (half_out), when its radix is even.
 #include "ch_internal.h"
 
 #include <algorithm>
@@ -97,26 +101,55 @@ __device__ __forceinline__ void dft4(float2& a, float2& b, float2& c,
     d = csub(t1, t3);
 }
 
+// dft4 of (a, b, 0, 0), c and d not read: a + c and b + d are a and b, so
+// every nonzero output has the bits dft4 gives it (only the sign of an
+// exact zero may differ).
 template <bool INV>
+__device__ __forceinline__ void dft4h(float2& a, float2& b, float2& c,
+                                      float2& d)
+{
+    const float2 t3 = rot<INV>(b);
+    c = csub(a, b);
+    d = csub(a, t3);
+    const float2 a0 = a;
+    a = cadd(a0, b);
+    b = cadd(a0, t3);
+}
+
+// dft<INV, H>: the DFT of v in place.  H: the upper half v[R/2 ..] of an
+// even-length v is zero and not read (the first layer of each
+// decomposition drops its zero terms, as in dft4h).
+template <bool INV, bool H = false>
 __device__ __forceinline__ void dft(float2 (&v)[2])
 {
-    dft2<INV>(v[0], v[1]);
+    if (H)
+        v[1] = v[0];
+    else
+        dft2<INV>(v[0], v[1]);
 }
 
-template <bool INV>
+template <bool INV, bool H = false>
 __device__ __forceinline__ void dft(float2 (&v)[4])
 {
-    dft4<INV>(v[0], v[1], v[2], v[3]);
+    if (H)
+        dft4h<INV>(v[0], v[1], v[2], v[3]);
+    else
+        dft4<INV>(v[0], v[1], v[2], v[3]);
 }
 
-template <bool INV>
+template <bool INV, bool H = false>
 __device__ __forceinline__ void dft(float2 (&v)[8])
 {
     // even and odd halves, then X[k] = E[k] + w^k O[k], X[k+4] = E - w^k O
     float2 e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
     float2 o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
-    dft4<INV>(e0, e1, e2, e3);
-    dft4<INV>(o0, o1, o2, o3);
+    if (H) {
+        dft4h<INV>(e0, e1, e2, e3);
+        dft4h<INV>(o0, o1, o2, o3);
+    } else {
+        dft4<INV>(e0, e1, e2, e3);
+        dft4<INV>(o0, o1, o2, o3);
+    }
     const float h = 0.70710678118654752f;
     // w^1 = (h, -h) forward, (h, h) inverse; w^3 = (-h, -h) / (-h, h)
     const float2 w1o = INV ? make_float2(h * (o1.x - o1.y), h * (o1.x + o1.y))
@@ -135,9 +168,10 @@ __device__ __forceinline__ void dft(float2 (&v)[8])
     v[7] = csub(e3, w3o);
 }
 
-template <bool INV>
+template <bool INV, bool H = false>
 __device__ __forceinline__ void dft(float2 (&v)[3])
 {
+    static_assert(!H, "odd length");
     const float s3 = 0.86602540378443865f;  // sin(pi / 3)
     const float2 t1 = cadd(v[1], v[2]);
     const float2 t2 = csub(v[0], cscale(t1, 0.5f));
@@ -147,9 +181,10 @@ __device__ __forceinline__ void dft(float2 (&v)[3])
     v[2] = csub(t2, t3);
 }
 
-template <bool INV>
+template <bool INV, bool H = false>
 __device__ __forceinline__ void dft(float2 (&v)[5])
 {
+    static_assert(!H, "odd length");
     const float c1 = 0.30901699437494742f;   // cos(2 pi / 5)
     const float c2 = -0.80901699437494742f;  // cos(4 pi / 5)
     const float s1 = 0.95105651629515357f;   // sin(2 pi / 5)
@@ -168,7 +203,7 @@ __device__ __forceinline__ void dft(float2 (&v)[5])
     v[3] = csub(a2, b2);
 }
 
-template <bool INV>
+template <bool INV, bool H = false>
 __device__ __forceinline__ void dft(float2 (&v)[16])
 {
     // 4 x 4: DFT4 over n1 of x[4 n1 + n2], twiddle W16^(n2 k1), DFT4 over
@@ -177,8 +212,12 @@ __device__ __forceinline__ void dft(float2 (&v)[16])
     const float h = 0.70710678118654752f;
     const float sg = INV ? 1.0f : -1.0f;
 #pragma unroll
-    for (int n2 = 0; n2 < 4; ++n2)
-        dft4<INV>(v[n2], v[4 + n2], v[8 + n2], v[12 + n2]);
+    for (int n2 = 0; n2 < 4; ++n2) {
+        if (H)
+            dft4h<INV>(v[n2], v[4 + n2], v[8 + n2], v[12 + n2]);
+        else
+            dft4<INV>(v[n2], v[4 + n2], v[8 + n2], v[12 + n2]);
+    }
     // after this v[n2 + 4 k1] holds y[n2][k1]; W16^m = (cos, sg sin)
     const float2 w1 = make_float2(c8, sg * s8), w2 = make_float2(h, sg * h),
                  w3 = make_float2(s8, sg * c8), w6 = make_float2(-h, sg * h),
@@ -290,8 +329,10 @@ __device__ float2* lds_fft(float2* a, float2* b, const float2* tw,
 // forward last stage, the product with K^ and its inverse first stage,
 // whose radices match because the inverse takes the factors in reverse.
 
-// First stage (Ns = 1): rows n = j + r * F / R come from load(n).
-template <int R, bool INV, int W, class Load>
+// First stage (Ns = 1): rows n = j + r * F / R come from load(n).  H
+// (R even): rows n >= F / 2, i.e. r >= R / 2, are zero padding and are
+// neither loaded nor added.
+template <int R, bool INV, int W, bool H, class Load>
 __device__ __forceinline__ void first_stage(const Load& load,
                                             float2* __restrict__ dst, int F)
 {
@@ -301,8 +342,9 @@ __device__ __forceinline__ void first_stage(const Load& load,
         float2 v[R];
 #pragma unroll
         for (int r = 0; r < R; ++r)
-            v[r] = load(j + r * nb);
-        dft<INV>(v);
+            v[r] = (H && r >= R / 2) ? make_float2(0.0f, 0.0f)
+                                     : load(j + r * nb);
+        dft<INV, H>(v);
 #pragma unroll
         for (int r = 0; r < R; ++r)
             dst[(j * R + r) * W + w] = v[r];
@@ -310,7 +352,9 @@ __device__ __forceinline__ void first_stage(const Load& load,
 }
 
 // Last stage (Ns = F / R): outputs k = j + r * F / R go to store(k, v).
-template <int R, bool INV, int W, class Store>
+// H (R even): only outputs k < F / 2 (r < R / 2) are wanted; the others are
+// neither stored nor computed.
+template <int R, bool INV, int W, bool H, class Store>
 __device__ __forceinline__ void last_stage(const float2* __restrict__ src,
                                            int F,
                                            const float2* __restrict__ tw,
@@ -332,7 +376,7 @@ __device__ __forceinline__ void last_stage(const float2* __restrict__ src,
         }
         dft<INV>(v);
 #pragma unroll
-        for (int r = 0; r < R; ++r)
+        for (int r = 0; r < (H ? R / 2 : R); ++r)
             store(j + r * nb, v[r]);
     }
 }
@@ -481,32 +525,66 @@ __device__ void stages_range_ip(float2* buf, const float2* tw, int F,
     }
 }
 
-template <bool INV, int W, class Load>
+// H: rows n >= F / 2 read as zero and R is even (the half forms; R = 2
+// stands for any radix the others do not name).
+template <bool INV, int W, bool H, class Load>
 __device__ __forceinline__ void first_any(int R, const Load& load,
                                           float2* dst, int F)
 {
-    switch (R) {
-    case 16: first_stage<16, INV, W>(load, dst, F); break;
-    case 8: first_stage<8, INV, W>(load, dst, F); break;
-    case 4: first_stage<4, INV, W>(load, dst, F); break;
-    case 2: first_stage<2, INV, W>(load, dst, F); break;
-    case 3: first_stage<3, INV, W>(load, dst, F); break;
-    default: first_stage<5, INV, W>(load, dst, F); break;
+    if constexpr (H) {
+        switch (R) {
+        case 16: first_stage<16, INV, W, true>(load, dst, F); break;
+        case 8: first_stage<8, INV, W, true>(load, dst, F); break;
+        case 4: first_stage<4, INV, W, true>(load, dst, F); break;
+        default: first_stage<2, INV, W, true>(load, dst, F); break;
+        }
+    } else {
+        switch (R) {
+        case 16: first_stage<16, INV, W, false>(load, dst, F); break;
+        case 8: first_stage<8, INV, W, false>(load, dst, F); break;
+        case 4: first_stage<4, INV, W, false>(load, dst, F); break;
+        case 2: first_stage<2, INV, W, false>(load, dst, F); break;
+        case 3: first_stage<3, INV, W, false>(load, dst, F); break;
+        default: first_stage<5, INV, W, false>(load, dst, F); break;
+        }
     }
 }
 
-template <bool INV, int W, class Store>
+// H: only outputs k < F / 2 are wanted and R is even.
+template <bool INV, int W, bool H, class Store>
 __device__ __forceinline__ void last_any(int R, const float2* src, int F,
                                          const float2* tw, const Store& st)
 {
-    switch (R) {
-    case 16: last_stage<16, INV, W>(src, F, tw, st); break;
-    case 8: last_stage<8, INV, W>(src, F, tw, st); break;
-    case 4: last_stage<4, INV, W>(src, F, tw, st); break;
-    case 2: last_stage<2, INV, W>(src, F, tw, st); break;
-    case 3: last_stage<3, INV, W>(src, F, tw, st); break;
-    default: last_stage<5, INV, W>(src, F, tw, st); break;
+    if constexpr (H) {
+        switch (R) {
+        case 16: last_stage<16, INV, W, true>(src, F, tw, st); break;
+        case 8: last_stage<8, INV, W, true>(src, F, tw, st); break;
+        case 4: last_stage<4, INV, W, true>(src, F, tw, st); break;
+        default: last_stage<2, INV, W, true>(src, F, tw, st); break;
+        }
+    } else {
+        switch (R) {
+        case 16: last_stage<16, INV, W, false>(src, F, tw, st); break;
+        case 8: last_stage<8, INV, W, false>(src, F, tw, st); break;
+        case 4: last_stage<4, INV, W, false>(src, F, tw, st); break;
+        case 2: last_stage<2, INV, W, false>(src, F, tw, st); break;
+        case 3: last_stage<3, INV, W, false>(src, F, tw, st); break;
+        default: last_stage<5, INV, W, false>(src, F, tw, st); break;
+        }
     }
+}
+
+// Whether a transform of length L reading rows n < n_in (writing outputs
+// k < n_out) takes the half form of its first (last) stage: the frame is at
+// least twice the grid and the stage's radix is even.  The kernels take the
+// answer as a template flag, so each holds one form of the stage.
+bool half_in(const FftLen& L, int n_in)
+{
+    return L.nfac >= 2 && 2 * n_in <= L.n && L.fac[0] % 2 == 0;
+}
+bool half_out(const FftLen& L, int n_out)
+{
+    return L.nfac >= 2 && 2 * n_out <= L.n && L.fac[L.nfac - 1] % 2 == 0;
 }
 
 template <int W, class KLoad>
@@ -691,8 +769,9 @@ __device__ __forceinline__ bool pair_voted(const int32_t (*s_vote)[16],
 // to F; after the FFT the two half spectra (k = 0 .. F/2) are separated,
 // X_y[k] = (C[k] + conj C[F-k]) / 2 and X_{y+1}[k] = (C[k] - conj C[F-k]) /
 // 2i, and written to A at row (x * a_rows + y) with pitch kp.  Workgroup:
-// W pairs, thread tid on pair tid % W.
-template <int W>
+// W pairs, thread tid on pair tid % W.  H: the first stage's half form
+// (half_in: rows z >= F / 2 are padding).
+template <int W, bool H>
 __global__ __launch_bounds__(kFftThreads) void z_fwd_kernel(
     BeamIO io, float2* __restrict__ A, int32_t X, int32_t Y, int32_t Z,
     int32_t a_rows, int32_t kp, FftLen L, const float2* __restrict__ g_tw,
@@ -730,7 +809,7 @@ __global__ __launch_bounds__(kFftThreads) void z_fwd_kernel(
             nz = nz || v.x != 0.0f || v.y != 0.0f;
             return v;
         };
-        first_any<false, W>(L.fac[0], load, s_a, F);
+        first_any<false, W, H>(L.fac[0], load, s_a, F);
         pair_vote<W>(nz, s_vote);
         __syncthreads();
         // a pair of zero lines has zero spectra: with flags, only the flag
@@ -798,8 +877,9 @@ __global__ __launch_bounds__(kFftThreads) void z_fwd_kernel(
 // extension conj X[F-k] above F/2, imaginary parts of the k = 0 bin and,
 // for even F, the k = F/2 bin dropped) as C = X_y + i X_{y+1}; the inverse
 // FFT gives line y in the real and line y + 1 in the imaginary part; z < Z
-// is stored or added.
-template <int W>
+// is stored or added.  H: the last stage's half form (half_out: z < Z <=
+// F / 2 kept).
+template <int W, bool H>
 __global__ __launch_bounds__(kFftThreads) void z_inv_kernel(
     BeamIO io, const float2* __restrict__ A, int32_t X, int32_t Y,
     int32_t Z, int32_t a_rows, int32_t kp, FftLen L,
@@ -843,7 +923,7 @@ __global__ __launch_bounds__(kFftThreads) void z_inv_kernel(
             }
             return make_float2(a.x - b.y, a.y + b.x);
         };
-        first_any<true, W>(L.fac[0], load, s_a, F);
+        first_any<true, W, false>(L.fac[0], load, s_a, F);
         __syncthreads();
         int Ns = L.fac[0];
         const float2* cur = s_a;
@@ -866,7 +946,7 @@ __global__ __launch_bounds__(kFftThreads) void z_inv_kernel(
                     out[Z + n] = c.y;
             }
         };
-        last_any<true, W>(L.fac[L.nfac - 1], cur, F, s_tw, store);
+        last_any<true, W, H>(L.fac[L.nfac - 1], cur, F, s_tw, store);
         return;
     }
     float2 va[kPerThread], vb[kPerThread];
@@ -924,10 +1004,11 @@ enum ColMode { kColFwd = 0, kColInv = 1, kColConv = 2 };
 // (XCD-major for the y passes).  Rows n >= n_in read as zero; rows n <
 // n_out are written back in place.  kColConv multiplies by K^ (same
 // layout, its own strides) between the forward and the inverse transform.
-// `inplace`: the host allotted one LDS buffer (single_buffer()).  With
+// `inplace`: the host allotted one LDS buffer (single_buffer()).  H: the
+// half forms of the first and last stages (col_half).  With
 // support flags (sp), the forward y pass (outer = slab x, rows y) and the
 // convolution pass (rows x) read only the rows the flags mark nonzero.
-template <int W, int MODE>
+template <int W, int MODE, bool H>
 __global__ __launch_bounds__(kFftThreads) void col_kernel(
     BeamIO io, float2* __restrict__ A, int64_t row_stride,
     int64_t outer_stride, int32_t n_cols, int32_t n_in, int32_t n_out,
@@ -936,6 +1017,10 @@ __global__ __launch_bounds__(kFftThreads) void col_kernel(
     int64_t k_outer_stride, int32_t inplace, Support sp)
 {
     constexpr int kPer = MODE == kColConv ? kPerThreadConv : kPerThread;
+    // H (col_half): the forward transforms read rows n < F / 2 only and the
+    // inverse ones keep outputs k < F / 2 only
+    constexpr bool HI = H && MODE != kColInv;
+    constexpr bool HO = H && MODE != kColFwd;
     extern __shared__ float2 sm[];
     __shared__ int32_t s_rows[MODE == kColInv ? 1 : kMaxFlagRows];
     const int F = L.n;
@@ -994,7 +1079,7 @@ __global__ __launch_bounds__(kFftThreads) void col_kernel(
             if (col_ok && n < n_out)
                 base[n * row_stride] = v;
         };
-        first_any<MODE == kColInv, W>(L.fac[0], load, s_a, F);
+        first_any<MODE == kColInv, W, HI>(L.fac[0], load, s_a, F);
         __syncthreads();
         int Ns = L.fac[0];
         float2* cur;
@@ -1014,12 +1099,12 @@ __global__ __launch_bounds__(kFftThreads) void col_kernel(
                 Ns = L.fac[nf - 1];
                 stages_range_ip<true, W>(s_a, s_tw, F, L.fac, nf - 2, 0, -1,
                                          Ns);
-                last_any<true, W>(L.fac[0], s_a, F, s_tw, store);
+                last_any<true, W, HO>(L.fac[0], s_a, F, s_tw, store);
             } else {
                 stages_range_ip<MODE == kColInv, W>(s_a, s_tw, F, L.fac, 1,
                                                     nf - 1, 1, Ns);
-                last_any<MODE == kColInv, W>(L.fac[nf - 1], s_a, F, s_tw,
-                                             store);
+                last_any<MODE == kColInv, W, HO>(L.fac[nf - 1], s_a, F,
+                                                 s_tw, store);
             }
             return;
         }
@@ -1039,11 +1124,12 @@ __global__ __launch_bounds__(kFftThreads) void col_kernel(
             Ns = L.fac[nf - 1];
             cur = stages_range<true, W>(oth, cur, s_tw, F, L.fac, nf - 2, 0,
                                         -1, Ns);
-            last_any<true, W>(L.fac[0], cur, F, s_tw, store);
+            last_any<true, W, HO>(L.fac[0], cur, F, s_tw, store);
         } else {
             cur = stages_range<MODE == kColInv, W>(s_a, s_b, s_tw, F, L.fac,
                                                    1, nf - 1, 1, Ns);
-            last_any<MODE == kColInv, W>(L.fac[nf - 1], cur, F, s_tw, store);
+            last_any<MODE == kColInv, W, HO>(L.fac[nf - 1], cur, F, s_tw,
+                                             store);
         }
         return;
     }
@@ -1178,11 +1264,13 @@ ch_status launch_z_fwd(const BeamIO& io, float2* A, int X, int Y, int Z,
     const bool ip = single_buffer(L, W);
     const size_t lds = ip ? (size_t)(L.n + L.n * W) * sizeof(float2)
                           : z_lds_bytes(L.n, W);
-    ch_status st = allow_lds(z_fwd_kernel<W>, z_lds_bytes(L.n, W));
+    auto kernel = half_in(L, Z) ? z_fwd_kernel<W, true>
+                                : z_fwd_kernel<W, false>;
+    ch_status st = allow_lds(kernel, z_lds_bytes(L.n, W));
     if (st != CH_OK)
         return st;
     const int64_t groups = ((int64_t)X * ((Y + 1) / 2) + W - 1) / W;
-    hipLaunchKernelGGL(z_fwd_kernel<W>,
+    hipLaunchKernelGGL(kernel,
                        dim3((unsigned)(groups * io.n_beams)),
                        dim3(kFftThreads), lds, s, io, A, X, Y, Z, a_rows, kp,
                        L, tw, (int32_t)ip, sp);
@@ -1198,16 +1286,28 @@ ch_status launch_z_inv(const BeamIO& io, const float2* A, int X, int Y,
     const bool ip = single_buffer(L, W);
     const size_t lds = ip ? (size_t)(L.n + L.n * W) * sizeof(float2)
                           : z_lds_bytes(L.n, W);
-    ch_status st = allow_lds(z_inv_kernel<W>, z_lds_bytes(L.n, W));
+    auto kernel = half_out(L, Z) ? z_inv_kernel<W, true>
+                                 : z_inv_kernel<W, false>;
+    ch_status st = allow_lds(kernel, z_lds_bytes(L.n, W));
     if (st != CH_OK)
         return st;
     const int64_t groups = ((int64_t)X * ((Y + 1) / 2) + W - 1) / W;
-    hipLaunchKernelGGL(z_inv_kernel<W>,
+    hipLaunchKernelGGL(kernel,
                        dim3((unsigned)(groups * io.n_beams)),
                        dim3(kFftThreads), lds, s, io, A, X, Y, Z, a_rows, kp,
                        L, tw, (int32_t)ip);
     CH_HIP(hipGetLastError());
     return CH_OK;
+}
+
+// The half forms apply when every transform of the pass has them: rows
+// n >= F / 2 zero for the forward transforms (the frame is at least twice
+// the grid), outputs k >= F / 2 dropped for the inverse ones.
+template <int MODE>
+bool col_half(const FftLen& L, int n_in, int n_out)
+{
+    return (MODE == kColInv || half_in(L, n_in)) &&
+           (MODE == kColFwd || half_out(L, n_out));
 }
 
 template <int W, int MODE>
@@ -1223,7 +1323,9 @@ ch_status launch_col(const BeamIO& io, float2* A, int64_t row_stride,
     const bool ip = single_buffer(L, W);
     const size_t lds = ip ? (size_t)(L.n + L.n * W) * sizeof(float2)
                           : col_lds_bytes(L.n, W);
-    ch_status st = allow_lds(col_kernel<W, MODE>, col_lds_bytes(L.n, W));
+    const bool h = col_half<MODE>(L, n_in, n_out);
+    auto kernel = h ? col_kernel<W, MODE, true> : col_kernel<W, MODE, false>;
+    ch_status st = allow_lds(kernel, col_lds_bytes(L.n, W));
     if (st != CH_OK)
         return st;
     // the forward y pass takes an even number of column groups
@@ -1233,12 +1335,12 @@ ch_status launch_col(const BeamIO& io, float2* A, int64_t row_stride,
         groups += groups & 1;
     const dim3 grid((unsigned)(groups * io.n_beams), (unsigned)n_outer);
     if (timer)
-        timer->launch(col_kernel<W, MODE>, grid, dim3(kFftThreads),
+        timer->launch(kernel, grid, dim3(kFftThreads),
                       (std::uint32_t)lds, io, A, row_stride, outer_stride,
                       n_cols, n_in, n_out, L, tw, khat, k_row_stride,
                       k_outer_stride, (int32_t)ip, sp);
     else
-        hipLaunchKernelGGL((col_kernel<W, MODE>), grid, dim3(kFftThreads),
+        hipLaunchKernelGGL(kernel, grid, dim3(kFftThreads),
                            lds, s, io, A, row_stride, outer_stride, n_cols,
                            n_in, n_out, L, tw, khat, k_row_stride,
                            k_outer_stride, (int32_t)ip, sp);
