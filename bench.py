@@ -205,6 +205,29 @@ def host_info():
             "cpu_model": model}
 
 
+def conv_half_forms(n, frame):
+    """The convolution pass's template flag H for n voxels on a frame of
+    `frame` points: bit 0 when its forward transform's first stage skips the
+    padding rows, bit 1 when its inverse transform's last stage skips the
+    cropped outputs (both need frame >= 2 n and an even radix in that stage).
+    The radices are ordered as the library orders them: 16, 8, 4, 2, then 3
+    and 5, with the last even radix moved to the end when two or more are
+    even and an odd one would come last."""
+    facs, m = [], frame
+    for r in (16, 8, 4, 2, 3, 5):
+        while m % r == 0:
+            facs.append(r)
+            m //= r
+    even = 0
+    while even < len(facs) and facs[even] % 2 == 0:
+        even += 1
+    if 2 <= even < len(facs):
+        facs.append(facs.pop(even - 1))
+    if len(facs) < 2 or 2 * n > frame:
+        return 0
+    return int(facs[0] % 2 == 0) | (int(facs[-1] % 2 == 0) << 1)
+
+
 def err_over_dmax(got, ref):
     return float(np.abs(got - ref).max() / ref.max())
 
@@ -342,16 +365,13 @@ def run_hetero(args, eng):
         "fft": (4 * nv + hs) + (hs + full) + kernel_bytes["fft"] +
                (full + hs) + (hs + 4 * nv)}
     lines = 16                      # the convolution pass's line group
-    # its stages' half forms: frame at least twice the grid, first and last
-    # radix even (radices 16, 8, 4, 2 first, then 3 and 5: a power of two)
-    half_forms = (2 * shape[0] <= frame[0] and frame[0] > 16
-                  and frame[0] & (frame[0] - 1) == 0)
+    half_forms = conv_half_forms(shape[0], frame[0])
     while lines > 2 and frame[0] * lines > 4096:
         lines //= 2
     kernel_name = {"gather": "gather_single (direct stencil gather)",
-                   "fft": "col_kernel<%d, 2, %s> (kColConv: x transforms, "
+                   "fft": "col_kernel<%d, 2, %d> (kColConv: x transforms, "
                           "times K^, inverse x transforms)"
-                          % (lines, "true" if half_forms else "false")}
+                          % (lines, half_forms)}
     # TERMA: the sort pass and the brick kernel, bracketed by events the
     # library attaches to them; 8 B per voxel (a 4 B density read and a 4 B
     # TERMA write)

@@ -1005,10 +1005,11 @@ enum ColMode { kColFwd = 0, kColInv = 1, kColConv = 2 };
 // n_out are written back in place.  kColConv multiplies by K^ (same
 // layout, its own strides) between the forward and the inverse transform.
 // `inplace`: the host allotted one LDS buffer (single_buffer()).  H: the
-// half forms of the first and last stages (col_half).  With
-// support flags (sp), the forward y pass (outer = slab x, rows y) and the
-// convolution pass (rows x) read only the rows the flags mark nonzero.
-template <int W, int MODE, bool H>
+// half forms (col_half), bit 0 the first stage's (half_in), bit 1 the last
+// stage's (half_out).  With support flags (sp), the forward y pass (outer =
+// slab x, rows y) and the convolution pass (rows x) read only the rows the
+// flags mark nonzero.
+template <int W, int MODE, int H>
 __global__ __launch_bounds__(kFftThreads) void col_kernel(
     BeamIO io, float2* __restrict__ A, int64_t row_stride,
     int64_t outer_stride, int32_t n_cols, int32_t n_in, int32_t n_out,
@@ -1017,10 +1018,10 @@ __global__ __launch_bounds__(kFftThreads) void col_kernel(
     int64_t k_outer_stride, int32_t inplace, Support sp)
 {
     constexpr int kPer = MODE == kColConv ? kPerThreadConv : kPerThread;
-    // H (col_half): the forward transforms read rows n < F / 2 only and the
-    // inverse ones keep outputs k < F / 2 only
-    constexpr bool HI = H && MODE != kColInv;
-    constexpr bool HO = H && MODE != kColFwd;
+    // the forward transform reads rows n < F / 2 only (HI), the inverse one
+    // keeps outputs k < F / 2 only (HO)
+    constexpr bool HI = (H & 1) != 0 && MODE != kColInv;
+    constexpr bool HO = (H & 2) != 0 && MODE != kColFwd;
     extern __shared__ float2 sm[];
     __shared__ int32_t s_rows[MODE == kColInv ? 1 : kMaxFlagRows];
     const int F = L.n;
@@ -1235,6 +1236,9 @@ ch_status allow_lds(K kernel, size_t bytes)
     return CH_OK;
 }
 
+// Radices 16, 8, 4, 2, then 3 and 5; when an odd radix would come last and
+// two or more radices are even, the last even one moves to the end, so that
+// the first and the last stage both have an even radix (half_in, half_out).
 bool fft_factor(int n, FftLen& L)
 {
     L.n = n;
@@ -1246,6 +1250,15 @@ bool fft_factor(int n, FftLen& L)
             L.fac[L.nfac++] = r;
             m /= r;
         }
+    int even = 0;
+    while (even < L.nfac && L.fac[even] % 2 == 0)
+        ++even;
+    if (even >= 2 && even < L.nfac) {
+        const int r = L.fac[even - 1];
+        for (int i = even - 1; i + 1 < L.nfac; ++i)
+            L.fac[i] = L.fac[i + 1];
+        L.fac[L.nfac - 1] = r;
+    }
     return m == 1;
 }
 
@@ -1300,14 +1313,27 @@ ch_status launch_z_inv(const BeamIO& io, const float2* A, int X, int Y,
     return CH_OK;
 }
 
-// The half forms apply when every transform of the pass has them: rows
-// n >= F / 2 zero for the forward transforms (the frame is at least twice
-// the grid), outputs k >= F / 2 dropped for the inverse ones.
+// The half forms of a column pass (col_kernel's H): bit 0 for its forward
+// transform's first stage, bit 1 for its inverse transform's last stage.
 template <int MODE>
-bool col_half(const FftLen& L, int n_in, int n_out)
+int col_half(const FftLen& L, int n_in, int n_out)
 {
-    return (MODE == kColInv || half_in(L, n_in)) &&
-           (MODE == kColFwd || half_out(L, n_out));
+    return (MODE != kColInv && half_in(L, n_in) ? 1 : 0) |
+           (MODE != kColFwd && half_out(L, n_out) ? 2 : 0);
+}
+
+template <int W, int MODE>
+auto col_variant(int h)
+{
+    if constexpr (MODE == kColFwd)
+        return h ? col_kernel<W, MODE, 1> : col_kernel<W, MODE, 0>;
+    else if constexpr (MODE == kColInv)
+        return h ? col_kernel<W, MODE, 2> : col_kernel<W, MODE, 0>;
+    else
+        return h == 3   ? col_kernel<W, MODE, 3>
+               : h == 2 ? col_kernel<W, MODE, 2>
+               : h == 1 ? col_kernel<W, MODE, 1>
+                        : col_kernel<W, MODE, 0>;
 }
 
 template <int W, int MODE>
@@ -1323,8 +1349,7 @@ ch_status launch_col(const BeamIO& io, float2* A, int64_t row_stride,
     const bool ip = single_buffer(L, W);
     const size_t lds = ip ? (size_t)(L.n + L.n * W) * sizeof(float2)
                           : col_lds_bytes(L.n, W);
-    const bool h = col_half<MODE>(L, n_in, n_out);
-    auto kernel = h ? col_kernel<W, MODE, true> : col_kernel<W, MODE, false>;
+    auto kernel = col_variant<W, MODE>(col_half<MODE>(L, n_in, n_out));
     ch_status st = allow_lds(kernel, col_lds_bytes(L.n, W));
     if (st != CH_OK)
         return st;
