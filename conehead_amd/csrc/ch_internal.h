@@ -190,8 +190,9 @@ enum ScratchSlot {
     kScratchBoxPartials = 0,  // per-block partial support boxes
     kScratchStatus = 1,       // TERMA geometry flag (one int32)
     kScratchSpectrum = 2,     // complex work grids of the spectral passes
-    kScratchTermaList = 3,    // TERMA brick lists by cost class + counters
-    kScratchSlots = 4
+    kScratchTermaList = 3,    // TERMA brick lists by cost class
+    kScratchTermaCounters = 4,  // TERMA counter states (zero when allocated)
+    kScratchSlots = 5
 };
 // zero_new: a newly allocated buffer is zero-filled (synchronously, under
 // the allocator's lock) before it is handed out.

@@ -495,6 +495,18 @@ constexpr int kCostClasses = 16;
 // on a few.
 constexpr int kShards = 16;
 constexpr int kCounters = kCostClasses * kShards;   // 4 per lane
+// A beam's counter state (int32, kept between calls, zero when allocated):
+// the kCounters append counters, two base rows of kCounters, and the call
+// word.  The counters only grow (unsigned, wrapping): a call's count of a
+// list is its counter minus the base row of the call's parity, the totals
+// the previous call's phase 2 left there, and its entries sit at the
+// counter's values modulo the list's power-of-two capacity.  Phase 1 adds
+// one to the call word (its block 0), phase 2 reads it and writes the
+// totals into the other base row (its wavefront 0), so no launch has to
+// clear the counters between calls.
+constexpr int kBaseRows = kCounters;                // base rows at +256, +512
+constexpr int kCallWord = 3 * kCounters;
+constexpr int kCounterInts = 3 * kCounters + 64;    // 256-byte multiple
 
 __device__ __forceinline__ int brick_cost_class(const TermaParams& P,
                                                 const Brick& k, bool edge)
@@ -553,10 +565,16 @@ __host__ __device__ __forceinline__ int32_t sort_blocks(int X, int Y, int Z)
     return (super_bricks(X, Y, Z) + kSortSupers - 1) / kSortSupers;
 }
 // Capacity of one shard's list of one class: kSortBricks per workgroup of
-// the shard.
+// the shard, rounded up to a power of two (entries go to counter values
+// modulo the capacity).
 __host__ __device__ __forceinline__ int32_t shard_cap(int X, int Y, int Z)
 {
-    return (sort_blocks(X, Y, Z) + kShards - 1) / kShards * kSortBricks;
+    const int32_t need =
+        (sort_blocks(X, Y, Z) + kShards - 1) / kShards * kSortBricks;
+    int32_t cap = kSortBricks;
+    while (cap < need)
+        cap *= 2;
+    return cap;
 }
 
 __device__ __forceinline__ void sort_block(const TermaParams& P, int blk,
@@ -612,12 +630,20 @@ __device__ __forceinline__ void sort_block(const TermaParams& P, int blk,
     const int shard = blk % kShards;
     if (t < kCostClasses) {
         const int n = s_cnt[t];
-        s_base[t] = n > 0 ? atomicAdd(&P.counts[t * kShards + shard], n) : 0;
+        s_base[t] = n > 0 ? (int32_t)atomicAdd(
+                                (unsigned*)&P.counts[t * kShards + shard],
+                                (unsigned)n)
+                          : 0;
     }
+    if (blk == 0 && t == 0)
+        P.counts[kCallWord] += 1;  // read by this call's phase 2 only
     __syncthreads();
-    if (q >= 0)
-        P.list[(int64_t)(q * kShards + shard) * shard_cap(P.X, P.Y, P.Z) +
-               s_base[q] + rank] = entry;
+    if (q >= 0) {
+        const int32_t cap = shard_cap(P.X, P.Y, P.Z);
+        const uint32_t at = ((uint32_t)s_base[q] + (uint32_t)rank) &
+                            (uint32_t)(cap - 1);
+        P.list[(int64_t)(q * kShards + shard) * cap + at] = entry;
+    }
 }
 
 __global__ __launch_bounds__(256) void terma_sort_kernel(TermaParams P)
@@ -846,19 +872,33 @@ __device__ __forceinline__ void terma_wave(const TermaParams& P, int g,
                                            int32_t* __restrict__ s_lane,
                                            int32_t n_bricks)
 {
-    const int32_t* __restrict__ counts = P.counts;
     const int32_t* __restrict__ list = P.list;
-    // the counters in order of descending cost class, shards in order:
-    // position j = (kCostClasses - 1 - q) * kShards + shard, lane l holding
-    // j = 4 l .. 4 l + 3; incl = inclusive prefix of the lanes' sums
+    // Lane l holds positions j = 4 l .. 4 l + 3 of the counters in order of
+    // descending cost class, shards in order (j = (kCostClasses - 1 - q) *
+    // kShards + shard): counters q * kShards + 4 (l & 3) + i, i = 0 .. 3,
+    // four consecutive ints read as one 16-byte load, with the same four of
+    // both base rows (chosen by the call's parity once the loads are in, so
+    // no load waits for another).  The count of a list is its counter minus
+    // the base (the previous call's total).
+    const int c0 = (kCostClasses - 1 - (l >> 2)) * kShards + 4 * (l & 3);
+    const uint4 cnt = *(const uint4*)(P.counts + c0);
+    const uint4 b0 = *(const uint4*)(P.counts + kBaseRows + c0);
+    const uint4 b1 = *(const uint4*)(P.counts + 2 * kBaseRows + c0);
+    const uint32_t call = (uint32_t)P.counts[kCallWord];
+    const uint4 b = (call & 1) ? b1 : b0;
+    if (g == 0) {
+        // this call's totals become the next call's base (the other row)
+        *(uint4*)(P.counts + kBaseRows * (1 + (~call & 1)) + c0) = cnt;
+    }
+    const uint32_t cv[4] = {cnt.x, cnt.y, cnt.z, cnt.w};
+    const uint32_t bv[4] = {b.x, b.y, b.z, b.w};
     int c4[4], sum = 0;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        const int j = 4 * l + i;
-        const int q = kCostClasses - 1 - j / kShards;
-        c4[i] = counts[q * kShards + j % kShards];
+        c4[i] = (int)(cv[i] - bv[i]);
         sum += c4[i];
     }
+    // incl = inclusive prefix of the lanes' sums
     int incl = sum;
     for (int o = 1; o < 64; o <<= 1) {
         const int u = __shfl_up(incl, o, 64);
@@ -872,30 +912,38 @@ __device__ __forceinline__ void terma_wave(const TermaParams& P, int g,
         const uint64_t past = __ballot(incl > g);
         const int owner = __ffsll((unsigned long long)past) - 1;
         int before = incl - sum, pick = 0, rank = 0;
+        uint32_t pbase = 0;
         bool found = false;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             if (!found && g < before + c4[i]) {
-                pick = 4 * l + i;
+                pick = c0 + i;
                 rank = g - before;
+                pbase = bv[i];
                 found = true;
             }
             before += c4[i];
         }
-        pick = __shfl(pick, owner, 64);
+        const int c = __shfl(pick, owner, 64);
         rank = __shfl(rank, owner, 64);
-        const int q = kCostClasses - 1 - pick / kShards;
-        const int c = q * kShards + pick % kShards;
-        const int e = list[(int64_t)c * shard_cap(P.X, P.Y, P.Z) + rank];
-        const Brick k = brick_of(P, e >> 2);
-        const float v = (e & 3) == 1 ? 1.0f : ((e & 3) == 2 ? -1.0f : 0.0f);
-        bool bad = false;
-        T = terma_brick(P, k, v, l, s_samp, s_lane, bad);
-        if (__ballot(bad) != 0 && l == 0)
-            atomicOr(P.status, 1);
-        x = k.x0 + (l >> 4);
-        y = k.y0 + ((l >> 2) & 3);
-        z = k.z0 + (l & 3);
+        pbase = (uint32_t)__shfl((int)pbase, owner, 64);
+        const int32_t cap = shard_cap(P.X, P.Y, P.Z);
+        const uint32_t at = (pbase + (uint32_t)rank) & (uint32_t)(cap - 1);
+        const int e = list[(int64_t)c * cap + at];
+        // an entry of this call is a brick index of this grid; anything else
+        // (a list slot no phase 1 of this state wrote) is skipped
+        if ((unsigned)(e >> 2) < (unsigned)n_bricks) {  // wavefront-uniform
+            const Brick k = brick_of(P, e >> 2);
+            const float v =
+                (e & 3) == 1 ? 1.0f : ((e & 3) == 2 ? -1.0f : 0.0f);
+            bool bad = false;
+            T = terma_brick(P, k, v, l, s_samp, s_lane, bad);
+            if (__ballot(bad) != 0 && l == 0)
+                atomicOr(P.status, 1);
+            x = k.x0 + (l >> 4);
+            y = k.y0 + ((l >> 2) & 3);
+            z = k.z0 + (l & 3);
+        }
     }
     if (P.box) {
         // one partial box per wavefront; box_finalize reduces them
@@ -956,8 +1004,9 @@ __global__ __launch_bounds__(64) void terma_prep_kernel(int32_t* status)
         status[0] = 0;
 }
 
-// Zeroes the cost-class counters before phase 1 (a kernel, so the call can
-// be captured in a graph).
+// Zeroes the beams' counter states before phase 1 where the scratch is
+// shared by every call on the device (a kernel, so the call can be captured
+// in a graph).
 __global__ __launch_bounds__(256) void terma_clear_kernel(int32_t* counts,
                                                           int n)
 {
@@ -1027,10 +1076,9 @@ static TermaParams make_params(const ch_beam& B, const int32_t* h_shape,
 }
 
 // Device memory TERMA launches need beyond their inputs and outputs, per
-// beam of one launch: kCounters cost-class counters and the brick lists of
-// phase 1 and, with a support box, one partial box per phase-2 wavefront.
-// The counters of all the launch's beams come first, so one clear kernel
-// zeroes them.
+// beam of one launch: a counter state (kCounterInts, kept between calls),
+// the brick lists of phase 1 and, with a support box, one partial box per
+// phase-2 wavefront.
 inline int32_t terma_bricks(const int32_t* shape)
 {
     return ((shape[0] + 3) / 4) * ((shape[1] + 3) / 4) *
@@ -1042,8 +1090,12 @@ inline size_t terma_list_ints(const int32_t* shape)
 }
 inline size_t terma_list_bytes(int32_t n_beams, const int32_t* shape)
 {
-    return (size_t)n_beams * (kCounters + terma_list_ints(shape)) *
-           sizeof(int32_t);
+    return (size_t)n_beams * terma_list_ints(shape) * sizeof(int32_t);
+}
+// The beams' counter states (kCounterInts each), zero when allocated.
+inline size_t terma_counter_bytes(int32_t n_beams)
+{
+    return (size_t)n_beams * kCounterInts * sizeof(int32_t);
 }
 inline size_t terma_box_ints(int32_t n_bricks)
 {
@@ -1054,16 +1106,16 @@ inline size_t terma_box_bytes(int32_t n_beams, int32_t n_bricks)
     return (size_t)n_beams * terma_box_ints(n_bricks) * sizeof(int32_t);
 }
 
-// Points beam i of a launch of n beams at its part of the scratch (lists:
-// counters then lists; boxes: partial boxes, null without any support box).
-static void terma_assign(TermaParams& P, const ch_beam& B, int i, int n,
-                         const int32_t* shape, int32_t* lists,
-                         int32_t* boxes)
+// Points beam i of a launch at its part of the scratch (counters: the
+// counter states; lists: the cost-class lists; boxes: partial boxes, null
+// without any support box).
+static void terma_assign(TermaParams& P, const ch_beam& B, int i,
+                         const int32_t* shape, int32_t* counters,
+                         int32_t* lists, int32_t* boxes)
 {
     const int32_t n_bricks = terma_bricks(shape);
-    P.counts = lists + (size_t)i * kCounters;
-    P.list = lists + (size_t)n * kCounters +
-             (size_t)i * terma_list_ints(shape);
+    P.counts = counters + (size_t)i * kCounterInts;
+    P.list = lists + (size_t)i * terma_list_ints(shape);
     P.box = B.d_box && boxes ? boxes + (size_t)i * terma_box_ints(n_bricks)
                              : nullptr;
 }
@@ -1079,7 +1131,8 @@ static std::vector<int32_t> terma_groups(int32_t n, const ch_beam* beams,
                                          const int32_t* shape)
 {
     const size_t per = terma_list_bytes(1, shape) +
-                       terma_box_bytes(1, terma_bricks(shape));
+                       terma_box_bytes(1, terma_bricks(shape)) +
+                       terma_counter_bytes(1);
     const int32_t cap = (int32_t)std::max<size_t>(
         1, std::min<size_t>(kSetBeams, kSetScratchBytes / per));
     std::vector<int32_t> starts{0};
@@ -1136,21 +1189,26 @@ static ch_status terma_params(const ch_beam& B, int b, const int32_t* shape,
     return CH_OK;
 }
 
-// The launches of n beams on one grid: clear their counters, phase 1,
-// phase 2 and the support boxes.  h holds the beams' parameters with their
-// scratch assigned; with d_tab (a device copy of h) the beams share each
-// launch, without it (n == 1) the parameters go by value.  ev0 / ev1
-// (nullable) are attached to the start of phase 1 and the end of phase 2.
+// The launches of n beams on one grid: phase 1, phase 2 and the support
+// boxes (first the clearing of their counter states when `clear`).  h holds
+// the beams' parameters with their scratch assigned; with d_tab (a device
+// copy of h) the beams share each launch, without it (n == 1) the
+// parameters go by value.  ev0 / ev1 (nullable) are attached to the start
+// of phase 1 and the end of phase 2.
 static ch_status terma_launch(const TermaParams* h, const TermaParams* d_tab,
                               int32_t n, const ch_beam* beams,
                               int32_t n_bricks, hipEvent_t ev0,
-                              hipEvent_t ev1, hipStream_t s)
+                              hipEvent_t ev1, hipStream_t s,
+                              bool clear = false)
 {
     const unsigned bpb = (unsigned)((n_bricks + 3) / 4);
     const unsigned spb = (unsigned)sort_blocks(h[0].X, h[0].Y, h[0].Z);
-    const int nc = n * kCounters;
-    hipLaunchKernelGGL(terma_clear_kernel, dim3((unsigned)(nc + 255) / 256),
-                       dim3(256), 0, s, h[0].counts, nc);
+    if (clear) {
+        const int nc = n * kCounterInts;
+        hipLaunchKernelGGL(terma_clear_kernel,
+                           dim3((unsigned)(nc + 255) / 256), dim3(256), 0, s,
+                           h[0].counts, nc);
+    }
     if (!d_tab) {
         hipExtLaunchKernelGGL(terma_sort_kernel, dim3(spb), dim3(256), 0, s,
                               ev0, nullptr, 0, h[0]);
@@ -1233,7 +1291,8 @@ struct ch_beamset : ch::GraphHeld {
     std::vector<ch_beam> beams;       // d_terma, d_box, accumulate per beam
     std::vector<int32_t> starts;      // launch groups (terma_groups)
     ch::TermaParams* d_tab = nullptr;
-    int32_t* lists = nullptr;         // counters and lists, largest group
+    int32_t* counters = nullptr;      // counter states, largest group
+    int32_t* lists = nullptr;         // cost-class lists, largest group
     int32_t* boxes = nullptr;         // partial boxes (null without boxes)
     int32_t* flags = nullptr;         // geometry flag per beam (own d_status
                                       // overrides)
@@ -1247,6 +1306,7 @@ static void beamset_release(ch_beamset* set)
 {
     (void)hipDeviceSynchronize();  // a queued launch may still read it
     (void)hipFree(set->d_tab);
+    (void)hipFree(set->counters);
     (void)hipFree(set->lists);
     (void)hipFree(set->boxes);
     (void)hipFree(set->flags);
@@ -1286,9 +1346,8 @@ static ch_status beamset_params(ch_beamset* set, int32_t n,
     }
     for (size_t g = 0; g + 1 < starts.size(); ++g)
         for (int32_t b = starts[g]; b < starts[g + 1]; ++b)
-            terma_assign(hp[b], beams[b], b - starts[g],
-                         starts[g + 1] - starts[g], set->shape,
-                         set->lists, set->boxes);
+            terma_assign(hp[b], beams[b], b - starts[g], set->shape,
+                         set->counters, set->lists, set->boxes);
     return CH_OK;
 }
 
@@ -1339,6 +1398,11 @@ ch_status ch_terma(int32_t n_beams, const ch_beam* h_beams,
                         terma_list_bytes(width, h_shape), &lists);
     if (st != CH_OK)
         return st;
+    void* counters = nullptr;
+    st = stream_scratch(s, kScratchTermaCounters, terma_counter_bytes(width),
+                        &counters, true);
+    if (st != CH_OK)
+        return st;
     void* boxes = nullptr;
     if (any_box) {
         st = stream_scratch(s, kScratchBoxPartials,
@@ -1357,8 +1421,9 @@ ch_status ch_terma(int32_t n_beams, const ch_beam* h_beams,
         if (ng == 1 || cap) {
             for (int32_t i = 0; i < ng; ++i) {
                 const int32_t b = g0 + i;
-                terma_assign(hp[b], h_beams[b], 0, 1, h_shape,
-                             (int32_t*)lists, (int32_t*)boxes);
+                terma_assign(hp[b], h_beams[b], 0, h_shape,
+                             (int32_t*)counters, (int32_t*)lists,
+                             (int32_t*)boxes);
                 st = terma_launch(&hp[b], nullptr, 1, &h_beams[b], n_bricks,
                                   b == 0 ? ev0 : nullptr,
                                   b == n_beams - 1 ? ev1 : nullptr, s);
@@ -1368,8 +1433,9 @@ ch_status ch_terma(int32_t n_beams, const ch_beam* h_beams,
             continue;
         }
         for (int32_t i = 0; i < ng; ++i)
-            terma_assign(hp[g0 + i], h_beams[g0 + i], i, ng, h_shape,
-                         (int32_t*)lists, (int32_t*)boxes);
+            terma_assign(hp[g0 + i], h_beams[g0 + i], i, h_shape,
+                         (int32_t*)counters, (int32_t*)lists,
+                         (int32_t*)boxes);
         void* d_tab = nullptr;
         st = stage_table(s, &hp[g0], (size_t)ng * sizeof(TermaParams),
                          &d_tab);
@@ -1399,8 +1465,8 @@ ch_status ch_terma_noalloc(int32_t n_beams, const ch_beam* h_beams,
     if (st != CH_OK)
         return st;
     const int32_t n_bricks = terma_bricks(h_shape);
-    const size_t need =
-        terma_list_bytes(1, h_shape) + terma_box_bytes(1, n_bricks);
+    const size_t need = terma_counter_bytes(1) + terma_list_bytes(1, h_shape) +
+                        terma_box_bytes(1, n_bricks);
     CH_REQUIRE(need <= kNoAllocScratchBytes,
                "ch_terma_noalloc: a %d x %d x %d grid needs %zu bytes of "
                "scratch, more than its %zu (use ch_terma)",
@@ -1414,9 +1480,12 @@ ch_status ch_terma_noalloc(int32_t n_beams, const ch_beam* h_beams,
     st = noalloc_scratch(&base);
     if (st != CH_OK)
         return st;
-    int32_t* lists = (int32_t*)base;
-    int32_t* boxes =
-        (int32_t*)((char*)base + terma_list_bytes(1, h_shape));
+    // counter state, lists, boxes; the state is cleared by every call (the
+    // array is shared by every call on the device)
+    int32_t* counters = (int32_t*)base;
+    int32_t* lists = (int32_t*)((char*)base + terma_counter_bytes(1));
+    int32_t* boxes = (int32_t*)((char*)base + terma_counter_bytes(1) +
+                                terma_list_bytes(1, h_shape));
     hipStream_t s = (hipStream_t)stream;
     // one beam at a time through the one scratch array (no device table)
     for (int32_t b = 0; b < n_beams; ++b) {
@@ -1425,9 +1494,9 @@ ch_status ch_terma_noalloc(int32_t n_beams, const ch_beam* h_beams,
                           d_debug, nullptr, &P);
         if (st != CH_OK)
             return st;
-        terma_assign(P, h_beams[b], 0, 1, h_shape, lists, boxes);
+        terma_assign(P, h_beams[b], 0, h_shape, counters, lists, boxes);
         st = terma_launch(&P, nullptr, 1, &h_beams[b], n_bricks, nullptr,
-                          nullptr, s);
+                          nullptr, s, true);
         if (st != CH_OK)
             return st;
     }
@@ -1469,6 +1538,10 @@ ch_status ch_beamset_create(int32_t n_beams, const ch_beam* h_beams,
     if (e == hipSuccess)
         e = hipMalloc((void**)&set->lists,
                       terma_list_bytes(width, set->shape));
+    if (e == hipSuccess)
+        e = hipMalloc((void**)&set->counters, terma_counter_bytes(width));
+    if (e == hipSuccess)
+        e = hipMemset(set->counters, 0, terma_counter_bytes(width));
     if (e == hipSuccess && any_box)
         e = hipMalloc((void**)&set->boxes,
                       terma_box_bytes(width, set->n_bricks));

@@ -183,3 +183,60 @@ def test_geometry_error_sticky_until_checked(engine):
     E.terma_check(s)
     s.sync()
     s.close()
+
+
+def test_repeated_calls_keep_their_counters(engine):
+    """Phase 1's cost-class counters are kept between calls: they only
+    grow, each call counts its bricks from the totals the previous call
+    left, and no launch clears them.  Calls on one stream that alternate two
+    grids (different list capacities), two beams sharing a launch, and a
+    captured graph replayed between eager calls all give the bits of each
+    grid's first call."""
+    from conehead_amd.device import Stream
+    src = Source("varian_clinac_6MV")
+    src.gantry = 30.0
+    blk = Block()
+    blk.set_square(6.0)
+    rng = np.random.default_rng(5)
+    s = Stream()
+    cases = []
+    for shape, d in (((40, 36, 44), 0.25), ((24, 20, 28), 0.5)):
+        spacing = (d, d, d)
+        origin = tuple(-0.5 * n * d for n in shape)
+        dens = rng.uniform(0.2, 1.8, shape).astype(np.float32)
+        d_rho = DeviceArray.from_host(dens)
+        d_t = DeviceArray(shape, np.float32)
+        d_t2 = DeviceArray(shape, np.float32)
+        ap = engine.aperture(blk)
+        beams = [E.beam_struct(src, SETTINGS, ap, d_rho, t, None)
+                 for t in (d_t, d_t2)]
+        cases.append((shape, origin, spacing, beams, d_t, d_t2, d_rho, ap))
+    first = []
+    for shape, origin, spacing, beams, d_t, *_ in cases:
+        E.terma(beams[:1], shape, origin, spacing, stream=s)
+        first.append(d_t.download(stream=s))
+        assert first[-1].max() > 0
+    for rep in range(3):
+        for (shape, origin, spacing, beams, d_t, d_t2, *_), ref in zip(
+                cases, first):
+            E.terma(beams[:1], shape, origin, spacing, stream=s)
+            assert np.array_equal(d_t.download(stream=s), ref)
+            E.terma(beams, shape, origin, spacing, stream=s)  # one launch
+            assert np.array_equal(d_t.download(stream=s), ref)
+            assert np.array_equal(d_t2.download(stream=s), ref)
+    (shape_a, origin_a, spacing_a, beams_a, t_a, *_) = cases[0]
+    (shape_b, origin_b, spacing_b, beams_b, t_b, *_) = cases[1]
+    with E.Graph.capture(s) as cap:
+        E.terma(beams_a[:1], shape_a, origin_a, spacing_a, stream=s,
+                check=False)
+    g = cap.graph
+    for rep in range(3):
+        E.terma(beams_b[:1], shape_b, origin_b, spacing_b, stream=s)
+        assert np.array_equal(t_b.download(stream=s), first[1])
+        g.launch(s)
+        s.sync()
+        E.terma_check(s)
+        assert np.array_equal(t_a.download(stream=s), first[0])
+    g.close()
+    s.sync()
+    s.close()
