@@ -151,7 +151,8 @@ timings = 0          # candidates timed by tune_fft / tune_pipeline so far
 
 
 def fft_key(plan, n_beams):
-    return "fft/%dx%dx%d/%d/%d" % (plan.shape + (plan.n_cones, n_beams))
+    # "fft2": the choices reach 16 lines wherever 16 frames fit 4096 points
+    return "fft2/%dx%dx%d/%d/%d" % (plan.shape + (plan.n_cones, n_beams))
 
 
 def tune_fft(plan, n_beams=1, reps=3, stream=None, use_cache=True):
@@ -180,8 +181,10 @@ def tune_fft(plan, n_beams=1, reps=3, stream=None, use_cache=True):
           for _ in range(n_beams)]
     ds = [DeviceArray(plan.shape) for _ in range(n_beams)]
     jobs = [(plan, t, d, None, False) for t, d in zip(ts, ds)]
+    plan.set_fft_lines((16,) * 5)
+    most = plan.fft_lines()              # the most each pass takes
     plan.set_fft_lines((0,) * 5)
-    top = plan.fft_lines()               # the defaults: the most that fit
+    top = plan.fft_lines()               # the defaults
     e0, e1 = Event(), Event()
     times = {}
 
@@ -204,7 +207,7 @@ def tune_fft(plan, n_beams=1, reps=3, stream=None, use_cache=True):
     timed(best)
     for k in range(5):
         for w in (2, 4, 8, 16):
-            if w > top[k] or w == best[k]:
+            if w > most[k] or w == best[k]:
                 continue
             trial = list(best)
             trial[k] = w

@@ -44,9 +44,13 @@ constexpr int kFftMax = 1024;      // longest transform
 // buffers (ping-pong) unless single_buffer() allows one.
 constexpr int kFftLdsElemsConv = 4096;
 constexpr int kFftLdsElems = 2048;
+// F * W a pass takes when more lines are asked for (the autotuner): with
+// one buffer, 16 lines of 256 points keep all 256 threads busy in every
+// radix-16 stage, which pays when many beams share a launch.
+constexpr int kFftLdsElemsMax = 4096;
 // Elements of one line group per thread in the unfused path (transforms
-// of one stage), for each limit.
-constexpr int kPerThread = kFftLdsElems / kFftThreads;
+// of one stage, F <= 16, so F * W <= 256 and one per thread covers it).
+constexpr int kPerThread = 8;
 constexpr int kPerThreadConv = kFftLdsElemsConv / kFftThreads;
 // Work grids one batched launch may use (beams beyond it take further
 // launches).
@@ -1194,17 +1198,20 @@ __global__ void scatter_frame_kernel(float* __restrict__ frame,
 
 // ---------------------------------------------------------------- host --
 
-// Lines per workgroup: the most (up to 16) with F * W <= limit, which
-// kPerThread elements per thread always cover; `want` (2, 4, 8 or 16)
-// takes fewer when they fit.
+// Lines per workgroup: by default the most (up to 16) with F * W <=
+// limit; `want` (2, 4, 8 or 16) is taken when F * want <= kFftLdsElemsMax.
 int lines_per_group(int F, int limit = kFftLdsElems, int want = 0)
 {
-    int w = 16;
-    while (w > 2 && F * w > limit)
-        w /= 2;
-    if ((want == 2 || want == 4 || want == 8 || want == 16) && want <= w)
+    auto most = [F](int lim) {
+        int w = 16;
+        while (w > 2 && F * w > lim)
+            w /= 2;
+        return w;
+    };
+    if ((want == 2 || want == 4 || want == 8 || want == 16) &&
+        want <= most(kFftLdsElemsMax))
         return want;
-    return w;
+    return most(limit);
 }
 
 size_t z_lds_bytes(int F, int W)

@@ -269,8 +269,9 @@ ch_status ch_plan_get_tile(const ch_plan* plan, int32_t* tx, int32_t* ty,
 /* Lines per workgroup of the FFT form's five passes (z forward, y
  * forward, x convolution, y inverse, z inverse); set by the autotuner.
  * Each value is 0 (the default: the most lines, up to 16, whose frames fit
- * the pass's LDS buffer) or 2, 4, 8 or 16; a value whose frames do not fit
- * falls back to the default.  Every choice computes the same butterflies
+ * 2048 points, 4096 for the convolution pass) or 2, 4, 8 or 16, taken when
+ * its frames fit 4096 points; a value whose frames do not fit falls back to
+ * the default.  Every choice computes the same butterflies
  * on every line, so the dose does not depend on it.  get returns the
  * values the passes use. */
 ch_status ch_plan_set_fft_lines(ch_plan* plan, const int32_t lines[5]);
