@@ -360,12 +360,20 @@ __device__ __forceinline__ void wave_lds_sync()
 // together before the first is used; lanes past the grid exit keep
 // stepping with their sums frozen, their range-checked loads reading 0
 // wherever they land.  Byte offsets are 32-bit (ch_terma requires
-// 4 X Y Z < 2^32).
+// 4 X Y Z < 2^32).  PACK (every axis of at most kPackMax voxels): the
+// steps left before each axis leaves the grid are one word of three 10-bit
+// fields, field a = 511 - left_a counting up, whose top bit (the guard) is
+// set on the step that takes axis a past the grid, when left_a would go
+// below zero; one select, one add and one test per step instead of three
+// counters.  A field stops at most kWalkBatch - 1 steps past 512, far from
+// carrying into the next.
 constexpr int kWalkBatch = 8;
+constexpr int kPackMax = 512;
 
-__device__ __forceinline__ float depth_walk(const float* __restrict__ rho,
-                                            int X, int Y, int Z, int x, int y,
-                                            int z, const Dda& d)
+template <bool PACK>
+__device__ __forceinline__ float depth_walk_t(const float* __restrict__ rho,
+                                              int X, int Y, int Z, int x,
+                                              int y, int z, const Dda& d)
 {
     double t0 = d.t0, t1 = d.t1, t2 = d.t2;
     const double dt0 = d.dt0, dt1 = d.dt1, dt2 = d.dt2;
@@ -373,6 +381,11 @@ __device__ __forceinline__ float depth_walk(const float* __restrict__ rho,
     int32_t left0 = d.s0 > 0 ? X - 1 - x : x;
     int32_t left1 = d.s1 > 0 ? Y - 1 - y : y;
     int32_t left2 = d.s2 > 0 ? Z - 1 - z : z;
+    constexpr uint32_t kGuards = (1u << 9) | (1u << 19) | (1u << 29);
+    uint32_t pk = PACK ? (511u - (uint32_t)left0) |
+                             ((511u - (uint32_t)left1) << 10) |
+                             ((511u - (uint32_t)left2) << 20)
+                       : 0u;
     // byte strides and offset
     const int32_t st0 = 4 * d.s0 * Y * Z, st1 = 4 * d.s1 * Z, st2 = 4 * d.s2;
     uint32_t off = 4u * (uint32_t)((x * Y + y) * Z + z);
@@ -400,10 +413,15 @@ __device__ __forceinline__ float depth_walk(const float* __restrict__ rho,
             t1 = fma(dt1, m1 ? 1.0 : 0.0, t1);
             t2 = fma(dt2, m2 ? 1.0 : 0.0, t2);
             off += (uint32_t)(m0 ? st0 : (m1 ? st1 : st2));
-            left0 -= (int32_t)m0;
-            left1 -= (int32_t)m1;
-            left2 -= (int32_t)m2;
-            live = live & ((left0 | left1 | left2) >= 0);
+            if (PACK) {
+                pk += m0 ? 1u : (m1 ? (1u << 10) : (1u << 20));
+                live = live & ((pk & kGuards) == 0u);
+            } else {
+                left0 -= (int32_t)m0;
+                left1 -= (int32_t)m1;
+                left2 -= (int32_t)m2;
+                live = live & ((left0 | left1 | left2) >= 0);
+            }
         }
         float r[kWalkBatch];
 #pragma unroll
@@ -668,6 +686,7 @@ __global__ __launch_bounds__(256) void terma_sort_set_kernel(
 // test gave v: hit-test stages 2 and 3, then per voxel the radiological
 // depth, off-axis distance, fluence, softening, horn and the polyenergetic
 // sum (conehead.py:71-377).  Returns the lane's T (0 outside the grid).
+template <bool PACK>
 __device__ __forceinline__ float terma_brick(const TermaParams& P,
                                              const Brick& k, float v, int l,
                                              double* __restrict__ s_samp,
@@ -802,7 +821,7 @@ __device__ __forceinline__ float terma_brick(const TermaParams& P,
             d.init(dir[0], dir[1], dir[2], P.spacing[0], P.spacing[1],
                    P.spacing[2]);
             const float de =
-                depth_walk(P.density, P.X, P.Y, P.Z, x, y, z, d);
+                depth_walk_t<PACK>(P.density, P.X, P.Y, P.Z, x, y, z, d);
 
             // off-axis distance (conehead.py:286-314)
             double oad = 0.0;
@@ -867,6 +886,7 @@ __device__ __forceinline__ float terma_brick(const TermaParams& P,
 // descending cost class, so the longest depth walks are dispatched first
 // and the rest fill in behind them (wavefronts past the last brick exit at
 // once).  Each wavefront writes its support-box partial.
+template <bool PACK>
 __device__ __forceinline__ void terma_wave(const TermaParams& P, int g,
                                            int l, double* __restrict__ s_samp,
                                            int32_t* __restrict__ s_lane,
@@ -937,7 +957,7 @@ __device__ __forceinline__ void terma_wave(const TermaParams& P, int g,
             const float v =
                 (e & 3) == 1 ? 1.0f : ((e & 3) == 2 ? -1.0f : 0.0f);
             bool bad = false;
-            T = terma_brick(P, k, v, l, s_samp, s_lane, bad);
+            T = terma_brick<PACK>(P, k, v, l, s_samp, s_lane, bad);
             if (__ballot(bad) != 0 && l == 0)
                 atomicOr(P.status, 1);
             x = k.x0 + (l >> 4);
@@ -970,14 +990,16 @@ __device__ __forceinline__ void terma_wave(const TermaParams& P, int g,
     }
 }
 
+// PACK: every axis of the grid has at most kPackMax voxels (depth_walk_t).
+template <bool PACK>
 __global__ __launch_bounds__(256) void terma_kernel(TermaParams P,
                                                     int32_t n_bricks)
 {
     __shared__ double s_samp[4][kSampleBuf];
     __shared__ int32_t s_lane[4][64];
     const int w = threadIdx.x >> 6;
-    terma_wave(P, blockIdx.x * 4 + w, threadIdx.x & 63, s_samp[w], s_lane[w],
-               n_bricks);
+    terma_wave<PACK>(P, blockIdx.x * 4 + w, threadIdx.x & 63, s_samp[w],
+                     s_lane[w], n_bricks);
 }
 
 // Phase 2 of several beams in one launch, beam by beam (the blocks of beam
@@ -985,6 +1007,7 @@ __global__ __launch_bounds__(256) void terma_kernel(TermaParams P,
 // or two beams, whose rays toward one source cross the same density lines
 // and read one aperture table, so they share the caches; the next beam's
 // longest walks start while the last of the previous beam finish.
+template <bool PACK>
 __global__ __launch_bounds__(256) void terma_set_kernel(
     const TermaParams* __restrict__ tab, int32_t n_bricks,
     int32_t blocks_per_beam)
@@ -994,7 +1017,7 @@ __global__ __launch_bounds__(256) void terma_set_kernel(
     const int w = threadIdx.x >> 6;
     const int beam = blockIdx.x / blocks_per_beam;
     const int g = (blockIdx.x - beam * blocks_per_beam) * 4 + w;
-    terma_wave(tab[beam], g, threadIdx.x & 63, s_samp[w], s_lane[w],
+    terma_wave<PACK>(tab[beam], g, threadIdx.x & 63, s_samp[w], s_lane[w],
                n_bricks);
 }
 
@@ -1203,6 +1226,8 @@ static ch_status terma_launch(const TermaParams* h, const TermaParams* d_tab,
 {
     const unsigned bpb = (unsigned)((n_bricks + 3) / 4);
     const unsigned spb = (unsigned)sort_blocks(h[0].X, h[0].Y, h[0].Z);
+    const bool pack =
+        h[0].X <= kPackMax && h[0].Y <= kPackMax && h[0].Z <= kPackMax;
     if (clear) {
         const int nc = n * kCounterInts;
         hipLaunchKernelGGL(terma_clear_kernel,
@@ -1212,15 +1237,17 @@ static ch_status terma_launch(const TermaParams* h, const TermaParams* d_tab,
     if (!d_tab) {
         hipExtLaunchKernelGGL(terma_sort_kernel, dim3(spb), dim3(256), 0, s,
                               ev0, nullptr, 0, h[0]);
-        hipExtLaunchKernelGGL(terma_kernel, dim3(bpb), dim3(256), 0, s,
-                              nullptr, ev1, 0, h[0], n_bricks);
+        hipExtLaunchKernelGGL(pack ? terma_kernel<true> : terma_kernel<false>,
+                              dim3(bpb), dim3(256), 0, s, nullptr, ev1, 0,
+                              h[0], n_bricks);
     } else {
         hipExtLaunchKernelGGL(terma_sort_set_kernel, dim3(spb * n),
                               dim3(256), 0, s, ev0, nullptr, 0, d_tab,
                               (int32_t)spb);
-        hipExtLaunchKernelGGL(terma_set_kernel, dim3(bpb * n), dim3(256), 0,
-                              s, nullptr, ev1, 0, d_tab, n_bricks,
-                              (int32_t)bpb);
+        hipExtLaunchKernelGGL(pack ? terma_set_kernel<true>
+                                   : terma_set_kernel<false>,
+                              dim3(bpb * n), dim3(256), 0, s, nullptr, ev1, 0,
+                              d_tab, n_bricks, (int32_t)bpb);
     }
     CH_HIP(hipGetLastError());
     for (int32_t i = 0; i < n; ++i)
