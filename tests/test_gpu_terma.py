@@ -88,6 +88,22 @@ def test_water_square_field(oracle, engine):
             np.ones(shape, np.float32), 0.0, blk)
 
 
+def test_long_axis_walk(oracle, engine):
+    """A grid with an axis above 512 voxels (600 along the beam axis y):
+    the depth walk keeps one counter per axis there (grids up to 512 per
+    axis pack them into one word), and its walks cross hundreds of voxels
+    toward the source."""
+    shape = (10, 600, 10)
+    spacing = [0.5, 0.05, 0.5]
+    origin = np.array([-2.25, 0.025, -2.25])
+    rng = np.random.default_rng(13)
+    dens = (1.0 + 0.3 * rng.standard_normal(shape)).clip(0.1, None)
+    blk = Block()
+    blk.set_square(4)
+    compare(oracle, engine, shape, spacing, origin,
+            dens.astype(np.float32), 15.0, blk)
+
+
 def test_samples_on_cell_boundaries(oracle, engine):
     """Sub-samples whose plane images fall on raster cell boundaries: with
     3 mm voxels and 3 sub-samples per axis the samples sit on a 1 mm grid,
