@@ -43,8 +43,8 @@ struct TermaParams {
     const float* density;
     float* terma;
     int32_t* box;              // support-box partials of this beam or null
-    int32_t* counts;           // this beam's cost-class counters (phase 1)
-    int32_t* list;             // and brick lists
+    int32_t* counts;           // this beam's counter state (kCounterInts)
+    int32_t* list;             // and cost-class brick lists
     float* debug;              // 4 planes of X*Y*Z or null
     int32_t* status;           // geometry error flag
     int32_t accumulate;        // terma += this beam
