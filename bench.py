@@ -33,8 +33,9 @@ The same JSON line carries
                 and host round trips, scatter convolution with atomics) on
                 beams of the batched workload, and the batched/plain ratio;
   uneven        32 phantoms of differing grids, voxel sizes, beam counts and
-                cone sets as one batch, with the error of one beam against
-                the oracle;
+                cone sets as one batch (the gather's tile and the FFT form's
+                lines per workgroup autotuned per plan), with the error of
+                one beam against the oracle;
   serve         Poisson request streams drawn from the uneven workload,
                 lambda swept 10-500 req/s through the request loop (p50/p99
                 latency, beams/s), its capacity at 2000 req/s, and one
@@ -766,6 +767,25 @@ def tune_requests(eng, be, reqs, s):
     return seen
 
 
+def tune_fft_requests(eng, reqs, s):
+    """Autotune the FFT lines per workgroup of every distinct plan, timed on
+    as many beams as the requests hold for it (one launch per pass takes
+    them all).  Returns the number of plans whose lines changed."""
+    from conehead_amd import autotune
+    plans = {}
+    for r in reqs:
+        p = eng.plan(r.shape, r.spacing, r.cones)
+        plans.setdefault(p.handle, [p, 0])[1] += len(r.beams)
+    changed = 0
+    for p, nb in plans.values():
+        p.set_fft_lines((0,) * 5)
+        before = p.fft_lines()
+        lines, _ = autotune.tune_fft(p, n_beams=nb, stream=s,
+                                     use_cache=False)
+        changed += tuple(lines) != tuple(before)
+    return changed
+
+
 def run_uneven(args, eng, methods=("gather", "fft"), cases_reqs=None):
     """The uneven batch (32 phantoms) as one batch with each method; returns
     (line, the oracle dose of the smallest case's first beam and its index
@@ -779,11 +799,14 @@ def run_uneven(args, eng, methods=("gather", "fft"), cases_reqs=None):
     upd = sum(c.voxel_cone_updates() * len(c.beams) for c in cases)
     steps = max(1, min(args.steps, 5))
     per, outs, tiles = {}, {}, {}
+    fft_tuned = None
     for method in methods:
         log("uneven: %s" % method)
         be = BatchEngine(eng, method=method)
         if method == "gather":
             tiles = tune_requests(eng, be, reqs, s)
+        else:
+            fft_tuned = tune_fft_requests(eng, reqs, s)
         ms, gms, outs[method] = time_batches(be, reqs, s, steps)
         per[method] = {"ms_per_batch": ms, "gpu_ms_per_batch": gms,
                        "beams_per_s": nb / (ms * 1e-3)}
@@ -801,7 +824,8 @@ def run_uneven(args, eng, methods=("gather", "fft"), cases_reqs=None):
                                 "lung/bone ellipsoids, 1-9 beams, 48-144 "
                                 "cones, one batch",
                     "beams": nb, "method": head,
-                    "distinct_tiles": len(set(tiles.values()))},
+                    "distinct_tiles": len(set(tiles.values())),
+                    "fft_plans_retuned": fft_tuned},
          "voxel_cone_updates_per_s": upd / (ms * 1e-3),
          "methods": per}
     check = None
