@@ -817,10 +817,17 @@ __global__ __launch_bounds__(kFftThreads) void z_fwd_kernel(
         pair_vote<W>(nz, s_vote);
         __syncthreads();
         // a pair of zero lines has zero spectra: with flags, only the flag
-        // is written
+        // is written, and a workgroup of zero pairs stops here
         const bool keep = !flags || pair_voted(s_vote, w);
         if (flags && lp.valid && n0 == 0)
             flags[pair] = keep ? 1 : 0;
+        if (flags) {
+            bool any = false;
+            for (int q = 0; q < W; ++q)
+                any = any || pair_voted(s_vote, q);
+            if (!any)
+                return;  // workgroup-uniform
+        }
         int Ns = L.fac[0];
         const float2* r = s_a;
         if (inplace)
